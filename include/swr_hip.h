@@ -1,0 +1,217 @@
+/*
+ * swr_hip.h — C ABI of the MI355X (gfx950) draw path.
+ *
+ * This is the boundary a caller of the reference's Rasterizer / VertexProcessor binds to.
+ * Every entry point names the reference interface it replaces (path:line in
+ * trenki2/SoftwareRenderer, src/renderer/...).  Plain C types only: pointers, sizes, ints.
+ *
+ * Conventions
+ *   - Every function returns an int status: SWR_OK (0) or a negative SWR_ERR_* code.
+ *     swr_last_error() gives a human-readable message for the last failure on a context.
+ *   - Buffers may be host or device memory (detected with hipPointerGetAttributes).
+ *     Host buffers are staged through device arenas owned by the context; device buffers
+ *     are used in place.  Render targets in host memory are copied in before and out after
+ *     the draw (the pixel programs read-modify-write them, as the reference's shaders do).
+ *   - swr_draw_* are synchronous, like the reference: they return once the render targets
+ *     hold the final values.  swr_draw_*_async enqueue on the context stream and return.
+ *   - One context per host thread; a context is not thread-safe (the reference is not either).
+ */
+#ifndef SWR_HIP_H
+#define SWR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------------ */
+#define SWR_OK                 0
+#define SWR_ERR_INVALID_ARG   -1  /* bad enum, negative size, null where a pointer is required */
+#define SWR_ERR_UNSUPPORTED   -2  /* program / varying count combination not built          */
+#define SWR_ERR_HIP           -3  /* a HIP runtime call failed                                 */
+#define SWR_ERR_NO_DEVICE     -4  /* no gfx950 device                                          */
+#define SWR_ERR_CAPACITY      -5  /* an arena could not be grown / a fixed limit was exceeded  */
+#define SWR_ERR_INDEX_RANGE   -6  /* an index addresses past a sized attribute buffer          */
+#define SWR_ERR_CLIP_OVERFLOW -7  /* a clipped polygon exceeded SWR_MAX_CLIP_VERTS vertices     */
+
+/* ---- limits (IRasterizer.h:33-39, VertexConfig.h:34) -------------------------------- */
+#define SWR_BLOCK_SIZE         8
+#define SWR_MAX_AVARS         16
+#define SWR_MAX_PVARS         16
+#define SWR_MAX_VERTEX_ATTRIBS 8
+#define SWR_BATCH_PRIMITIVES 1024  /* VertexProcessor.cpp:110 */
+#define SWR_MAX_CLIP_VERTS    32   /* polygon vertices after 6 planes; 3 + 6*4 is the convex bound */
+
+/* ---- enums (same values as the reference's enum classes) ---------------------------- */
+/* RasterMode, Rasterizer.h:45-49 */
+#define SWR_RASTER_SPAN      0
+#define SWR_RASTER_BLOCK     1
+#define SWR_RASTER_ADAPTIVE  2
+/* DrawMode, VertexProcessor.h:42-46 */
+#define SWR_DRAW_POINT       0
+#define SWR_DRAW_LINE        1
+#define SWR_DRAW_TRIANGLE    2
+/* CullMode, VertexProcessor.h:49-53 */
+#define SWR_CULL_NONE        0
+#define SWR_CULL_CCW         1
+#define SWR_CULL_CW          2
+
+/* ---- RasterizerVertex (IRasterizer.h:42-53): 36 floats, 144 bytes -------------------- */
+typedef struct swr_rasterizer_vertex {
+    float x, y, z, w;
+    float avar[SWR_MAX_AVARS];
+    float pvar[SWR_MAX_PVARS];
+} swr_rasterizer_vertex;
+
+/* ---- vertex programs ------------------------------------------------------------------
+ * The reference binds a C++ static function (VertexProcessor.h:77-85).  Across a C ABI a
+ * shader is a device program id plus its uniforms; the statics AttribCount / AVarCount /
+ * PVarCount keep their meaning (VertexShaderBase.h:37-43): the clipper interpolates only
+ * the first avar_count / pvar_count varyings (PolyClipper.h:42-45).
+ *
+ *   SWR_VS_DUMMY          writes nothing (DummyVertexShader, VertexShaderBase.h:53)
+ *   SWR_VS_RAW            attrib 0 holds {x,y,z,w, avar[avar_count], pvar[pvar_count]} floats
+ *   SWR_VS_POS_RGB        attrib 0 = {x,y,z,r,g,b}: out = (x,y,z,1), avar[0..2] = rgb
+ *                         (Benchmark.cpp:32-49, VertexProcessorTest.cpp:59-76)
+ *   SWR_VS_MVP_POS_RGB_UV attrib 0 = {px,py,pz,r,g,b,u,v}: out = M*(p,1), avar = rgb, pvar = uv
+ *   SWR_VS_MVP_POS_UV     attrib 0 = {px,py,pz,nx,ny,nz,u,v} (ObjData::VertexArrayData):
+ *                         out = M*(p,1), pvar = uv (Box.cpp:67-88)
+ * M is row-major (mvp[4*r+c]); out[r] = (((0 + M[r][0]*x) + M[r][1]*y) + M[r][2]*z) + M[r][3]*1,
+ * the sequential dot product of vector_math.h:55-60, 548-556, with no contraction.
+ */
+#define SWR_VS_DUMMY           0
+#define SWR_VS_RAW             1
+#define SWR_VS_POS_RGB         2
+#define SWR_VS_MVP_POS_RGB_UV  3
+#define SWR_VS_MVP_POS_UV      4
+
+typedef struct swr_vertex_shader_desc {
+    int32_t program;
+    int32_t attrib_count;      /* VertexShaderBase::AttribCount */
+    int32_t avar_count;        /* VertexShaderBase::AVarCount   */
+    int32_t pvar_count;        /* VertexShaderBase::PVarCount   */
+    float   mvp[16];           /* uniform block of the MVP programs */
+} swr_vertex_shader_desc;
+
+/* ---- pixel programs -------------------------------------------------------------------
+ * The reference binds PixelShader::drawPixel (Rasterizer.h:90-96, PixelShaderBase.h:114-119).
+ * The statics InterpolateZ / InterpolateW / AVarCount / PVarCount keep their meaning
+ * (PixelShaderBase.h:44-53).  Targets are width x height pixels with a row pitch in pixels;
+ * pixels outside the target are never written.
+ *
+ *   SWR_PS_NULL           no-op (NullPixelShader, PixelShaderBase.h:139)
+ *   SWR_PS_DEPTH_CHECKER  target1 f32 depth, target0 u32 RGBA8:
+ *                         if (z < depth) { depth = z; color = pack(avar[0..2] * checker(pvar[0..1])) }
+ *                         checker = ((floor(u*cells) + floor(v*cells)) odd) ? 1 : 0.5
+ *   SWR_PS_RGB            target0 u32: (int)(r*255) << 16 | (int)(g*255) << 8 | (int)(b*255)
+ *                         (RasterizerTest.cpp:38-48, VertexProcessorTest.cpp:39-49)
+ *   SWR_PS_COVERAGE       target0 u32 = 1 (Benchmark.cpp:22-25)
+ *   SWR_PS_HASH           target0 u32 order-sensitive hash, target1 u32 invocation count:
+ *                         h = mix(h, key), then mix over the bits of z (if InterpolateZ),
+ *                         w and invw (if InterpolateW), avar[0..A), pvar[0..P);
+ *                         mix(h, x) = (h ^ x) * 16777619
+ *   SWR_PS_LOG            target0 u32 invocation count; appends one record per invocation to
+ *                         log (SWR_LOG_WORDS u32 words each) at an atomic slot:
+ *                         {x, y, key, seq, z, w, invw, avar[16], pvar[16]} (floats as bits);
+ *                         *log_count receives the number of invocations even past capacity.
+ *   SWR_PS_VISIBILITY     target0 u32 primitive key+1 (0 = none), target1 f32 depth,
+ *                         target2 float2 barycentrics: depth-tested visibility buffer
+ * "key" identifies the primitive in draw order: (batch << 15) + slot, where slot is the
+ * triangle's position in the batch's index list after clipping (VertexProcessor.cpp:254-261);
+ * for swr_draw_triangle_list it is the triangle's position in the list.
+ */
+#define SWR_PS_NULL            0
+#define SWR_PS_DEPTH_CHECKER   1
+#define SWR_PS_RGB             2
+#define SWR_PS_COVERAGE        3
+#define SWR_PS_HASH            4
+#define SWR_PS_LOG             5
+#define SWR_PS_VISIBILITY      6
+
+#define SWR_LOG_WORDS          (7 + SWR_MAX_AVARS + SWR_MAX_PVARS)
+#define SWR_HASH_SEED          2166136261u
+
+typedef struct swr_pixel_shader_desc {
+    int32_t program;
+    int32_t interpolate_z;     /* PixelShaderBase::InterpolateZ */
+    int32_t interpolate_w;     /* PixelShaderBase::InterpolateW */
+    int32_t avar_count;        /* PixelShaderBase::AVarCount    */
+    int32_t pvar_count;        /* PixelShaderBase::PVarCount    */
+    int32_t width, height;     /* target size in pixels        */
+    int32_t pitch;             /* row pitch in pixels           */
+    void   *target0;
+    void   *target1;
+    void   *target2;
+    uint32_t *log;             /* SWR_PS_LOG records            */
+    uint64_t  log_capacity;    /* in records                    */
+    uint64_t *log_count;       /* device or host u64            */
+    float   checker_cells;     /* SWR_PS_DEPTH_CHECKER uniform  */
+    int32_t reserved;
+} swr_pixel_shader_desc;
+
+/* ---- context ------------------------------------------------------------------------ */
+typedef struct swr_context swr_context;
+
+/* Creates a context on HIP device `device` with the reference's default state:
+ * Span, scissor (0,0,0,0), NullPixelShader (Rasterizer.h:67-72); cull CW, depth (0,1),
+ * DummyVertexShader (VertexProcessor.cpp:29-35); viewport (0,0,0,0). */
+int  swr_create(swr_context **out, int device);
+int  swr_destroy(swr_context *ctx);
+const char *swr_last_error(const swr_context *ctx);
+/* Use `stream` (a hipStream_t, or NULL for the context's own stream) for all work. */
+int  swr_set_stream(swr_context *ctx, void *stream);
+int  swr_synchronize(swr_context *ctx);
+
+/* ---- Rasterizer (Rasterizer.h:52-141) ------------------------------------------------ */
+int  swr_set_raster_mode(swr_context *ctx, int mode);                        /* Rasterizer.h:75-78 */
+int  swr_set_scissor_rect(swr_context *ctx, int x, int y, int w, int h);     /* Rasterizer.h:81-87 */
+int  swr_set_pixel_shader(swr_context *ctx, const swr_pixel_shader_desc *d);/* Rasterizer.h:90-96 */
+/* IRasterizer seam: window-space vertices; triangles whose first index is -1 are skipped.
+ * IRasterizer.h:70 / Rasterizer.h:134-141 (drawTriangle, :111-114, is a list of one). */
+int  swr_draw_triangle_list(swr_context *ctx, const swr_rasterizer_vertex *vertices,
+                            size_t vertex_count, const int32_t *indices, size_t index_count);
+/* IRasterizer.h:66 / Rasterizer.h:125-132 */
+int  swr_draw_line_list(swr_context *ctx, const swr_rasterizer_vertex *vertices,
+                        size_t vertex_count, const int32_t *indices, size_t index_count);
+/* IRasterizer.h:62 / Rasterizer.h:116-123 */
+int  swr_draw_point_list(swr_context *ctx, const swr_rasterizer_vertex *vertices,
+                         size_t vertex_count, const int32_t *indices, size_t index_count);
+
+/* ---- VertexProcessor (VertexProcessor.h:56-91) ---------------------------------------- */
+int  swr_set_viewport(swr_context *ctx, int x, int y, int w, int h);         /* VertexProcessor.cpp:43-54 */
+int  swr_set_depth_range(swr_context *ctx, float n, float f);                /* VertexProcessor.cpp:56-60 */
+int  swr_set_cull_mode(swr_context *ctx, int mode);                          /* VertexProcessor.cpp:62-65 */
+int  swr_set_vertex_shader(swr_context *ctx, const swr_vertex_shader_desc *d);/* VertexProcessor.h:77-85 */
+/* VertexProcessor.cpp:67-76.  buffer_bytes = 0 means "unknown" (no index range check, as in
+ * the reference); otherwise indices addressing past the buffer fail with SWR_ERR_INDEX_RANGE. */
+int  swr_set_vertex_attrib_pointer(swr_context *ctx, int index, int stride,
+                                   const void *buffer, size_t buffer_bytes);
+/* VertexProcessor.cpp:78-120.  count indices; triangles use count/3 complete triples. */
+int  swr_draw_elements(swr_context *ctx, int mode, size_t count, const int32_t *indices);
+/* Same, enqueued on the context stream without waiting.  Device buffers only.  The context's
+ * arenas must already be large enough (swr_reserve, or a previous synchronous draw of the
+ * same scene); otherwise the draw leaves the targets untouched and the next swr_synchronize
+ * returns SWR_ERR_CAPACITY. */
+int  swr_draw_elements_async(swr_context *ctx, int mode, size_t count, const int32_t *indices);
+int  swr_reserve(swr_context *ctx, size_t primitives, size_t tile_pairs);
+
+/* ---- statistics of the last completed draw ------------------------------------------- */
+typedef struct swr_draw_stats {
+    uint64_t input_primitives;   /* triangles / lines / points submitted          */
+    uint64_t raster_primitives;  /* primitives after clip, cull and setup          */
+    uint64_t tile_pairs;         /* (primitive, 8x8 tile) pairs binned             */
+    uint64_t tiles;              /* tiles of the render target                     */
+    float    stage_ms[8];        /* per-stage GPU time when profiling is enabled:   */
+                                 /* 0 geometry, 1 scans, 2 emit, 3 sort, 4 ranges, 5 raster */
+    uint32_t stage_launches[8];
+} swr_draw_stats;
+int  swr_get_draw_stats(swr_context *ctx, swr_draw_stats *out);
+/* Record HIP events around each stage (on the stream the stage runs on). */
+int  swr_set_profiling(swr_context *ctx, int enable);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SWR_HIP_H */

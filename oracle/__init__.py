@@ -1,0 +1,213 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes access to the CPU oracle and the reference build.
+
+``liboracle.so`` is the plain-C restatement of trenki2/SoftwareRenderer's draw path
+(``swr_oracle.c``); ``_ref/libswr_ref.so`` is the reference's own ``src/renderer`` sources
+compiled with ``ref_driver.cpp`` (built only where ``/root/reference`` exists).  Both expose
+the same entry points, so tests call them interchangeably through :class:`CpuRenderer`.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg import this
+module, and only as the checker / the CPU baseline; the product never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libswr_ref.so")
+REF_SRC = "/root/reference/src/renderer"
+
+MAX_AVARS = 16
+MAX_PVARS = 16
+LOG_WORDS = 7 + MAX_AVARS + MAX_PVARS
+HASH_SEED = 2166136261
+
+# enums / program ids of include/swr_hip.h
+RASTER_SPAN, RASTER_BLOCK, RASTER_ADAPTIVE = 0, 1, 2
+DRAW_POINT, DRAW_LINE, DRAW_TRIANGLE = 0, 1, 2
+CULL_NONE, CULL_CCW, CULL_CW = 0, 1, 2
+VS_DUMMY, VS_RAW, VS_POS_RGB, VS_MVP_POS_RGB_UV, VS_MVP_POS_UV = 0, 1, 2, 3, 4
+PS_NULL, PS_DEPTH_CHECKER, PS_RGB, PS_COVERAGE, PS_HASH, PS_LOG, PS_VISIBILITY = range(7)
+
+
+class VertexShaderDesc(C.Structure):
+    _fields_ = [("program", C.c_int32), ("attrib_count", C.c_int32),
+                ("avar_count", C.c_int32), ("pvar_count", C.c_int32),
+                ("mvp", C.c_float * 16)]
+
+
+class PixelShaderDesc(C.Structure):
+    _fields_ = [("program", C.c_int32), ("interpolate_z", C.c_int32),
+                ("interpolate_w", C.c_int32), ("avar_count", C.c_int32),
+                ("pvar_count", C.c_int32), ("width", C.c_int32), ("height", C.c_int32),
+                ("pitch", C.c_int32), ("target0", C.c_void_p), ("target1", C.c_void_p),
+                ("target2", C.c_void_p), ("log", C.c_void_p), ("log_capacity", C.c_uint64),
+                ("log_count", C.c_void_p), ("checker_cells", C.c_float),
+                ("reserved", C.c_int32)]
+
+
+class PipelineState(C.Structure):
+    _fields_ = [("raster_mode", C.c_int32), ("scissor", C.c_int32 * 4),
+                ("viewport", C.c_int32 * 4), ("depth_near", C.c_float),
+                ("depth_far", C.c_float), ("cull_mode", C.c_int32), ("reserved", C.c_int32),
+                ("vs", VertexShaderDesc), ("ps", PixelShaderDesc),
+                ("attrib", C.c_void_p * 8), ("stride", C.c_int32 * 8)]
+
+
+RASTERIZER_VERTEX = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("w", "<f4"),
+                              ("avar", "<f4", (MAX_AVARS,)), ("pvar", "<f4", (MAX_PVARS,))])
+
+
+def build(quiet: bool = True) -> None:
+    """Compile liboracle.so (and _ref/ when the reference sources are present)."""
+    out = subprocess.run(["make", "-C", HERE, "all"], capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + out.stdout + out.stderr)
+    if not quiet:
+        print(out.stdout)
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+class Targets:
+    """Host render targets for one pixel program (what the program reads and writes)."""
+
+    def __init__(self, program: int, width: int, height: int, log_capacity: int = 0,
+                 depth_clear: float = 1.0):
+        self.program, self.width, self.height = program, width, height
+        shape = (height, width)
+        self.t0 = self.t1 = self.log = None
+        self.log_count = np.zeros(1, np.uint64)
+        if program == PS_DEPTH_CHECKER:
+            self.t0 = np.zeros(shape, np.uint32)
+            self.t1 = np.full(shape, depth_clear, np.float32)
+        elif program in (PS_RGB, PS_COVERAGE):
+            self.t0 = np.zeros(shape, np.uint32)
+        elif program == PS_HASH:
+            self.t0 = np.full(shape, HASH_SEED, np.uint32)
+            self.t1 = np.zeros(shape, np.uint32)
+        elif program == PS_LOG:
+            self.t0 = np.zeros(shape, np.uint32)
+            self.log = np.zeros((max(log_capacity, 1), LOG_WORDS), np.uint32)
+        self.log_capacity = log_capacity
+
+    def log_records(self) -> np.ndarray:
+        n = int(min(self.log_count[0], self.log_capacity))
+        return self.log[:n]
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class Pipeline:
+    """Everything Rasterizer + VertexProcessor hold for a draw, in reference terms."""
+
+    def __init__(self, width=640, height=480, raster_mode=RASTER_SPAN, cull_mode=CULL_CW,
+                 scissor=None, viewport=None, depth_range=(0.0, 1.0),
+                 vs_program=VS_MVP_POS_RGB_UV, vs_counts=(1, 3, 2), mvp=None,
+                 ps_program=PS_HASH, ps_z=True, ps_w=True, ps_a=3, ps_p=2, checker_cells=8.0):
+        self.width, self.height = width, height
+        self.raster_mode, self.cull_mode = raster_mode, cull_mode
+        self.scissor = tuple(scissor) if scissor is not None else (0, 0, width, height)
+        self.viewport = tuple(viewport) if viewport is not None else (0, 0, width, height)
+        self.depth_range = depth_range
+        self.vs_program, self.vs_counts = vs_program, tuple(vs_counts)
+        self.mvp = np.eye(4, dtype=np.float32) if mvp is None else np.asarray(mvp, np.float32)
+        self.ps_program, self.ps_z, self.ps_w = ps_program, ps_z, ps_w
+        self.ps_a, self.ps_p, self.checker_cells = ps_a, ps_p, checker_cells
+
+    def state(self, targets: Targets, attribs=()):
+        st = PipelineState()
+        st.raster_mode = self.raster_mode
+        st.scissor[:] = self.scissor
+        st.viewport[:] = self.viewport
+        st.depth_near, st.depth_far = self.depth_range
+        st.cull_mode = self.cull_mode
+        st.vs.program = self.vs_program
+        st.vs.attrib_count, st.vs.avar_count, st.vs.pvar_count = self.vs_counts
+        st.vs.mvp[:] = [float(x) for x in self.mvp.reshape(16)]
+        ps = st.ps
+        ps.program = self.ps_program
+        ps.interpolate_z, ps.interpolate_w = int(self.ps_z), int(self.ps_w)
+        ps.avar_count, ps.pvar_count = self.ps_a, self.ps_p
+        ps.width, ps.height, ps.pitch = self.width, self.height, self.width
+        ps.target0, ps.target1 = _ptr(targets.t0), _ptr(targets.t1)
+        ps.log = _ptr(targets.log)
+        ps.log_capacity = targets.log_capacity
+        ps.log_count = _ptr(targets.log_count)
+        ps.checker_cells = self.checker_cells
+        for i, (buf, stride) in enumerate(attribs):
+            st.attrib[i] = buf.ctypes.data
+            st.stride[i] = stride
+        return st
+
+    def targets(self, log_capacity=0) -> Targets:
+        return Targets(self.ps_program, self.width, self.height, log_capacity)
+
+
+class CpuRenderer:
+    """The oracle (kind='oracle') or the reference build (kind='ref')."""
+
+    def __init__(self, kind: str = "oracle"):
+        path = ORACLE_SO if kind == "oracle" else REF_SO
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built (run oracle.build())")
+        self.kind = kind
+        self.lib = C.CDLL(path)
+        pre = "swr_oracle_" if kind == "oracle" else "swr_ref_"
+        sig = [C.POINTER(PipelineState), C.c_int, C.c_size_t, C.c_void_p]
+        self._elements = getattr(self.lib, pre + "draw_elements")
+        self._elements.argtypes, self._elements.restype = sig, C.c_int
+        lsig = [C.POINTER(PipelineState), C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t]
+        self._lists = {}
+        for name in ("triangle", "line", "point"):
+            f = getattr(self.lib, pre + f"draw_{name}_list")
+            f.argtypes, f.restype = lsig, C.c_int
+            self._lists[name] = f
+        self._setup = getattr(self.lib, pre + "triangle_setup")
+        self._setup.argtypes = [C.c_void_p] * 3 + [C.c_int, C.c_int, C.c_void_p]
+        self._setup.restype = C.c_int
+        self._clip = getattr(self.lib, pre + "clip_triangle")
+        self._clip.argtypes = [C.c_void_p] * 3 + [C.c_int, C.c_int, C.c_void_p, C.c_int]
+        self._clip.restype = C.c_int
+
+    def draw_elements(self, pipe: Pipeline, targets: Targets, attribs, mode, indices):
+        idx = np.ascontiguousarray(indices, dtype=np.int32)
+        attribs = [(np.ascontiguousarray(b), s) for b, s in attribs]
+        st = pipe.state(targets, attribs)
+        rc = self._elements(C.byref(st), mode, idx.size, idx.ctypes.data)
+        if rc != 0:
+            raise RuntimeError(f"{self.kind} draw_elements failed: {rc}")
+        return targets
+
+    def draw_list(self, pipe: Pipeline, targets: Targets, prim: str, vertices, indices):
+        v = np.ascontiguousarray(vertices, dtype=RASTERIZER_VERTEX)
+        idx = np.ascontiguousarray(indices, dtype=np.int32)
+        st = pipe.state(targets)
+        rc = self._lists[prim](C.byref(st), v.ctypes.data, v.size, idx.ctypes.data, idx.size)
+        if rc != 0:
+            raise RuntimeError(f"{self.kind} draw_{prim}_list failed: {rc}")
+        return targets
+
+    def triangle_setup(self, v0, v1, v2, na, np_):
+        out = np.zeros(1 + 9 + 3 * (2 + na + np_), np.float32)
+        vs = [np.ascontiguousarray(v, dtype=RASTERIZER_VERTEX) for v in (v0, v1, v2)]
+        k = self._setup(vs[0].ctypes.data, vs[1].ctypes.data, vs[2].ctypes.data, na, np_,
+                        out.ctypes.data)
+        return out[: (k if k else 10)]
+
+    def clip_triangle(self, v0, v1, v2, na, np_, max_out=64):
+        out = np.zeros(max_out, RASTERIZER_VERTEX)
+        vs = [np.ascontiguousarray(v, dtype=RASTERIZER_VERTEX) for v in (v0, v1, v2)]
+        n = self._clip(vs[0].ctypes.data, vs[1].ctypes.data, vs[2].ctypes.data, na, np_,
+                       out.ctypes.data, max_out)
+        if n < 0:
+            raise RuntimeError(f"clip failed {n}")
+        return out[: min(n, max_out)], n

@@ -1,0 +1,255 @@
+"""Synthetic scenes for the draw path: the workloads the tests and the benchmark draw.
+
+Every generator is deterministic in its seed and returns host numpy arrays; the same
+buffers go to the GPU path and to the CPU checkers.
+
+* :func:`sphere_scene` — UV sphere, 64 stacks x 128 slices (16,128 triangles), per-vertex RGB
+  from a seeded uniform and uv = (slice/128, stack/64), camera at z=3, 60 degree fovy,
+  near 0.1, far 10.  Vertex layout ``{px,py,pz, r,g,b, u,v}`` (32 bytes).
+* :func:`soup_scene` — triangle soup: NDC centroids uniform in [-1.2,1.2]^2, view depth
+  uniform in [0.05,12], screen area log-normal (median 20 px, sigma 1.5), random
+  orientation and facing, 30% of index slots reusing an earlier vertex.
+* :func:`benchmark_scene` — the scene of the reference's Benchmark.cpp:66-105 (40,960
+  triangles with x,y,z,r,g,b from .NET ``System.Random(0)``, Random.cpp).
+* :func:`clip_scene` — clip-space triangles that straddle every frustum plane.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+VERTEX_FLOATS = 8  # px,py,pz, r,g,b, u,v
+
+
+@dataclass
+class Scene:
+    vertices: np.ndarray          # float32 [V, k]
+    indices: np.ndarray           # int32 [3T]
+    mvp: np.ndarray               # float32 [4,4], row-major: clip = M @ (p, 1)
+    width: int
+    height: int
+    info: dict = field(default_factory=dict)
+
+    @property
+    def triangles(self) -> int:
+        return self.indices.size // 3
+
+    @property
+    def stride(self) -> int:
+        return self.vertices.shape[1] * 4
+
+
+def perspective(fovy_deg: float, aspect: float, near: float, far: float) -> np.ndarray:
+    """OpenGL-style projection (the matrix of vector_math.h:737-759)."""
+    rad = math.radians(fovy_deg) / 2.0
+    cot = math.cos(rad) / math.sin(rad)
+    m = np.zeros((4, 4))
+    m[0, 0] = cot / aspect
+    m[1, 1] = cot
+    m[2, 2] = -(far + near) / (far - near)
+    m[2, 3] = -2.0 * near * far / (far - near)
+    m[3, 2] = -1.0
+    return m
+
+
+def lookat(eye, center, up) -> np.ndarray:
+    """View matrix (the matrix of vector_math.h:772-794)."""
+    eye, center, up = (np.asarray(v, float) for v in (eye, center, up))
+    f = center - eye
+    f /= np.linalg.norm(f)
+    s = np.cross(f, up)
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    m = np.eye(4)
+    m[0, :3], m[1, :3], m[2, :3] = s, u, -f
+    t = np.eye(4)
+    t[:3, 3] = -eye
+    return m @ t
+
+
+def camera(width: int, height: int, eye=(0.0, 0.0, 3.0)):
+    view = lookat(eye, (0, 0, 0), (0, 1, 0))
+    proj = perspective(60.0, width / height, 0.1, 10.0)
+    return proj @ view, view
+
+
+def sphere_scene(stacks: int = 64, slices: int = 128, seed: int = 1, width: int = 640,
+                 height: int = 480, radius: float = 1.0) -> Scene:
+    rng = np.random.default_rng(seed)
+    s = np.arange(stacks + 1)[:, None]
+    k = np.arange(slices + 1)[None, :]
+    theta = np.pi * s / stacks
+    phi = 2.0 * np.pi * k / slices
+    pos = np.stack([radius * np.sin(theta) * np.cos(phi),
+                    radius * np.cos(theta) * np.ones_like(phi),
+                    -radius * np.sin(theta) * np.sin(phi)], axis=-1).reshape(-1, 3)
+    uv = np.stack([np.broadcast_to(k / slices, (stacks + 1, slices + 1)),
+                   np.broadcast_to(s / stacks, (stacks + 1, slices + 1))], -1).reshape(-1, 2)
+    rgb = rng.random((pos.shape[0], 3))
+    verts = np.concatenate([pos, rgb, uv], axis=1).astype(np.float32)
+
+    def vid(a, b):
+        return a * (slices + 1) + b
+
+    tris = []
+    for a in range(stacks):
+        for b in range(slices):
+            p00, p10, p11, p01 = vid(a, b), vid(a + 1, b), vid(a + 1, b + 1), vid(a, b + 1)
+            if a != 0:
+                tris.append((p00, p10, p01))
+            if a != stacks - 1:
+                tris.append((p10, p11, p01))
+    idx = np.asarray(tris, np.int32)
+    # outward-facing triangles are counter-clockwise seen from outside
+    p = verts[:, :3].astype(np.float64)
+    nrm = np.cross(p[idx[:, 1]] - p[idx[:, 0]], p[idx[:, 2]] - p[idx[:, 0]])
+    outward = np.einsum("ij,ij->i", nrm, p[idx[:, 0]] + p[idx[:, 1]] + p[idx[:, 2]])
+    assert np.all(outward > 0), "sphere winding"
+    mvp, _ = camera(width, height)
+    return Scene(verts, idx.reshape(-1), mvp.astype(np.float32), width, height,
+                 {"name": "sphere", "stacks": stacks, "slices": slices, "seed": seed})
+
+
+def soup_scene(ntri: int = 1_000_000, seed: int = 2, width: int = 1920, height: int = 1080,
+               reuse: float = 0.30, median_area: float = 20.0, sigma: float = 1.5) -> Scene:
+    """Triangle soup: ~30% of triangles cross a side or the near plane, ~50% face away.
+
+    Vertex reuse: a triangle is, with probability q, the parallelogram partner of the
+    previous (fresh) triangle, sharing its edge (v1, v2); a fraction f = q / (1 + q) of
+    triangles are partners and 2f/3 of the index slots reuse a vertex, so f = 1.5 * reuse.
+    """
+    rng = np.random.default_rng(seed)
+    aspect = width / height
+    mvp, view = camera(width, height)
+    inv_view = np.linalg.inv(view)
+    tan_half = math.tan(math.radians(30.0))
+
+    f_partner = 1.5 * reuse
+    q = f_partner / (1.0 - f_partner)
+    partner = rng.random(ntri) < q
+    partner[0] = False
+    # a partner's predecessor must be fresh: in each run of candidates keep every other one
+    ar = np.arange(ntri)
+    starts = partner & ~np.r_[False, partner[:-1]]
+    run_start = np.maximum.accumulate(np.where(starts, ar, 0))
+    partner &= (ar - run_start) % 2 == 0
+    fresh = ~partner
+
+    nf = int(fresh.sum())
+    c = rng.uniform(-1.2, 1.2, (nf, 2))
+    depth = rng.uniform(0.05, 12.0, nf)
+    area = np.exp(math.log(median_area) + sigma * rng.standard_normal(nf))
+    shape = rng.standard_normal((nf, 3, 2))
+    shape -= shape.mean(axis=1, keepdims=True)
+    a0 = 0.5 * np.abs((shape[:, 1, 0] - shape[:, 0, 0]) * (shape[:, 2, 1] - shape[:, 0, 1])
+                      - (shape[:, 2, 0] - shape[:, 0, 0]) * (shape[:, 1, 1] - shape[:, 0, 1]))
+    shape *= np.sqrt(area / np.maximum(a0, 1e-12))[:, None, None]
+    ndc = c[:, None, :] + shape * np.array([2.0 / width, 2.0 / height])
+    dv = depth[:, None] * (1.0 + rng.uniform(-0.1, 0.1, (nf, 3)))
+    view_pts = np.stack([ndc[..., 0] * dv * tan_half * aspect, ndc[..., 1] * dv * tan_half,
+                         -dv, np.ones_like(dv)], axis=-1)
+    world = view_pts @ inv_view.T  # [nf, 3, 4]
+
+    # assemble vertices and indices in triangle order: a fresh triangle adds 3 vertices,
+    # a partner (v2', v1', v1' + v2' - v0' of its fresh predecessor) adds 1
+    nverts = 3 * nf + int(partner.sum())
+    nnew = np.where(fresh, 3, 1)
+    off = np.cumsum(nnew) - nnew
+    rank = np.cumsum(fresh) - 1  # index into `world` of each fresh triangle
+    pos = np.empty((nverts, 3))
+    idx = np.empty((ntri, 3), np.int64)
+    tf = np.nonzero(fresh)[0]
+    wf = world[rank[tf], :, :3]
+    for k in range(3):
+        pos[off[tf] + k] = wf[:, k]
+        idx[tf, k] = off[tf] + k
+    tp = np.nonzero(partner)[0]
+    wp = world[rank[tp - 1], :, :3]
+    pos[off[tp]] = wp[:, 1] + wp[:, 2] - wp[:, 0]
+    idx[tp, 0] = off[tp - 1] + 2
+    idx[tp, 1] = off[tp - 1] + 1
+    idx[tp, 2] = off[tp]
+    rgb = rng.random((nverts, 3))
+    uv = rng.random((nverts, 2))
+    verts = np.concatenate([pos, rgb, uv], axis=1).astype(np.float32)
+    return Scene(verts, idx.reshape(-1).astype(np.int32), mvp.astype(np.float32), width, height,
+                 {"name": "soup", "ntri": ntri, "seed": seed, "reuse": reuse})
+
+
+class DotNetRandom:
+    """System.Random (Knuth subtractive generator) as published by .NET, the generator of
+    the reference's Random.cpp:7-84; NextDouble = InternalSample() * (1 / int.MaxValue)."""
+
+    MBIG = 2147483647
+    MSEED = 161803398
+
+    def __init__(self, seed: int = 0):
+        sa = [0] * 56
+        sub = self.MBIG if seed == -2147483648 else abs(seed)
+        mj = self.MSEED - sub
+        sa[55] = mj
+        mk = 1
+        for i in range(1, 55):
+            ii = (21 * i) % 55
+            sa[ii] = mk
+            mk = mj - mk
+            if mk < 0:
+                mk += self.MBIG
+            mj = sa[ii]
+        for _ in range(1, 5):
+            for i in range(1, 56):
+                sa[i] -= sa[1 + (i + 30) % 55]
+                if sa[i] < 0:
+                    sa[i] += self.MBIG
+        self.sa, self.inext, self.inextp = sa, 0, 21
+
+    def _sample(self) -> int:
+        a = self.inext + 1
+        if a >= 56:
+            a = 1
+        b = self.inextp + 1
+        if b >= 56:
+            b = 1
+        r = self.sa[a] - self.sa[b]
+        if r == self.MBIG:
+            r -= 1
+        if r < 0:
+            r += self.MBIG
+        self.sa[a] = r
+        self.inext, self.inextp = a, b
+        return r
+
+    def next_double(self) -> float:
+        return self._sample() * (1.0 / self.MBIG)
+
+
+def benchmark_scene(ntri: int = 4096 * 10, width: int = 640, height: int = 480) -> Scene:
+    """The reference Benchmark (Benchmark.cpp:66-105): {x,y,z,r,g,b} per vertex, w = 1."""
+    rnd = DotNetRandom(0)
+    vals = np.array([rnd.next_double() for _ in range(ntri * 3 * 6)], np.float64)
+    verts = vals.astype(np.float32).reshape(ntri * 3, 6)
+    idx = np.arange(ntri * 3, dtype=np.int32)
+    return Scene(verts, idx, np.eye(4, dtype=np.float32), width, height,
+                 {"name": "benchmark", "ntri": ntri})
+
+
+def clip_scene(ntri: int = 2000, seed: int = 3, width: int = 256, height: int = 192,
+               navar: int = 3, npvar: int = 2, spread: float = 1.6) -> Scene:
+    """Clip-space triangles {x,y,z,w, avar[navar], pvar[npvar]} for the raw vertex program.
+    Vertices are spread up to `spread` times the frustum so most triangles cross planes."""
+    rng = np.random.default_rng(seed)
+    n = 3 * ntri
+    w = rng.uniform(0.2, 2.0, n)
+    xyz = rng.uniform(-spread, spread, (n, 3)) * w[:, None]
+    # keep triangles reasonably sized: pull each triangle's vertices toward its first vertex
+    xyz = xyz.reshape(ntri, 3, 3)
+    shrink = rng.uniform(0.05, 1.0, (ntri, 1, 1))
+    xyz = xyz[:, :1, :] + (xyz - xyz[:, :1, :]) * shrink
+    xyz = xyz.reshape(n, 3)
+    var = rng.random((n, navar + npvar))
+    verts = np.concatenate([xyz, w[:, None], var], axis=1).astype(np.float32)
+    idx = np.arange(n, dtype=np.int32)
+    return Scene(verts, idx, np.eye(4, dtype=np.float32), width, height,
+                 {"name": "clip", "ntri": ntri, "seed": seed})
