@@ -160,7 +160,8 @@ typedef struct swr_context swr_context;
 int  swr_create(swr_context **out, int device);
 int  swr_destroy(swr_context *ctx);
 const char *swr_last_error(const swr_context *ctx);
-/* Use `stream` (a hipStream_t, or NULL for the context's own stream) for all work. */
+/* Use `stream` (a hipStream_t; NULL is the null stream) for all later work.  A context
+ * starts on a non-blocking stream of its own. */
 int  swr_set_stream(swr_context *ctx, void *stream);
 int  swr_synchronize(swr_context *ctx);
 

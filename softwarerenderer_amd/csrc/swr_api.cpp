@@ -1,0 +1,759 @@
+// swr_api.cpp — the C ABI (include/swr_hip.h): context state, device arenas, and the
+// launch sequence of one draw.
+//
+// Draw sequence for triangles (all on the context stream):
+//   begin      clear tile ranges and the status words
+//   count      geometry pass 1: per input triangle, fan size and (prims, pairs) per phase
+//   scans      exclusive scans -> draw-order positions; totals + arena overflow published
+//   emit       geometry pass 2: primitive records and (tile, primitive) pairs in draw order
+//   sort       stable radix sort of the pairs by tile id
+//   ranges     per-tile [begin, end)
+//   raster     one wave64 per 8x8 tile
+// A synchronous draw reads the status words back; when an arena overflowed, nothing was
+// rasterized (every later kernel checks the overflow word first), so the arenas are grown
+// to the published totals and the draw is issued again.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "swr_common.h"
+#include "swr_internal.h"
+
+using namespace swr;
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+enum Stage { S_GEOM, S_SCAN, S_EMIT, S_SORT, S_RANGES, S_RASTER, S_COUNT };
+
+}  // namespace
+
+struct swr_context {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // Rasterizer (Rasterizer.h:67-72 defaults)
+    int raster_mode = SWR_RASTER_SPAN;
+    int sc[4] = {0, 0, 0, 0};
+    swr_pixel_shader_desc ps{};
+
+    // VertexProcessor (VertexProcessor.cpp:29-35 defaults)
+    int vp[4] = {0, 0, 0, 0};
+    float dn = 0.0f, df = 1.0f;
+    int cull = SWR_CULL_CW;
+    swr_vertex_shader_desc vs{};
+    struct Attr {
+        const void *ptr = nullptr;
+        int stride = 0;
+        size_t bytes = 0;
+    } attr[SWR_MAX_VERTEX_ATTRIBS];
+
+    // arenas
+    DevBuf idx, attr0, wverts, app, app_ex, grp, grp_ex, scan_tmp, prims, pk_a, pv_a, pk_b, pv_b,
+        ranges, hist, hist_ex, hist_tmp, status, tgt[3], log, logcount;
+    uint64_t prim_cap = 0, pair_cap = 0;
+    unsigned long long *status_host = nullptr;
+
+    bool profiling = false;
+    hipEvent_t ev[S_COUNT + 1] = {};
+    swr_draw_stats stats{};
+    bool async_pending = false;
+};
+
+namespace {
+
+int fail(swr_context *c, int code, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
+int fail(swr_context *c, int code, const char *fmt, ...)
+{
+    if (c) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        c->err = buf;
+    }
+    return code;
+}
+
+#define HIP_TRY(c, expr)                                                                      \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail((c), SWR_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));      \
+    } while (0)
+
+hipError_t ensure(swr_context *c, DevBuf &b, size_t bytes)
+{
+    if (bytes == 0)
+        bytes = 256;
+    if (b.bytes >= bytes)
+        return hipSuccess;
+    if (b.p) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess)
+            return e;
+        (void)hipFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    size_t want = bytes + bytes / 4;
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return e;
+    }
+    b.bytes = want;
+    return hipSuccess;
+}
+
+bool is_device_ptr(const void *p)
+{
+    if (!p)
+        return false;
+    hipPointerAttribute_t at;
+    hipError_t e = hipPointerGetAttributes(&at, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
+}
+
+uint64_t vs_need_bytes(const swr_vertex_shader_desc &vs)
+{
+    switch (vs.program) {
+    case SWR_VS_RAW: return 4ull * (4 + vs.avar_count + vs.pvar_count);
+    case SWR_VS_POS_RGB: return 24;
+    case SWR_VS_MVP_POS_RGB_UV:
+    case SWR_VS_MVP_POS_UV: return 32;
+    default: return 0;
+    }
+}
+
+// Stages a host or device buffer: device memory is used in place, host memory is copied.
+int resolve_in(swr_context *c, const void *src, size_t bytes, DevBuf &stage, const void **out)
+{
+    if (!src || is_device_ptr(src)) {
+        *out = src;
+        return SWR_OK;
+    }
+    HIP_TRY(c, ensure(c, stage, bytes));
+    HIP_TRY(c, hipMemcpyAsync(stage.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+    *out = stage.p;
+    return SWR_OK;
+}
+
+struct TargetMap {
+    void *dev[3] = {nullptr, nullptr, nullptr};
+    void *host[3] = {nullptr, nullptr, nullptr};
+    size_t bytes[3] = {0, 0, 0};
+    uint32_t *log = nullptr;
+    uint32_t *log_host = nullptr;
+    size_t log_bytes = 0;
+    unsigned long long *logcount = nullptr;
+    unsigned long long *logcount_host = nullptr;
+};
+
+int targets_needed(int program, bool need[3])
+{
+    need[0] = need[1] = need[2] = false;
+    switch (program) {
+    case SWR_PS_DEPTH_CHECKER:
+    case SWR_PS_HASH:
+        need[0] = need[1] = true;
+        break;
+    case SWR_PS_RGB:
+    case SWR_PS_COVERAGE:
+    case SWR_PS_LOG:
+        need[0] = true;
+        break;
+    default:
+        break;
+    }
+    return 0;
+}
+
+int map_targets(swr_context *c, TargetMap &m)
+{
+    const swr_pixel_shader_desc &ps = c->ps;
+    if (ps.width < 0 || ps.height < 0 || ps.pitch < ps.width)
+        return fail(c, SWR_ERR_INVALID_ARG, "bad target size %dx%d pitch %d", ps.width, ps.height,
+                    ps.pitch);
+    bool need[3];
+    targets_needed(ps.program, need);
+    void *const t[3] = {ps.target0, ps.target1, ps.target2};
+    size_t bytes = (size_t)ps.pitch * (size_t)ps.height * 4;
+    for (int i = 0; i < 3; ++i) {
+        if (!need[i])
+            continue;
+        if (!t[i])
+            return fail(c, SWR_ERR_INVALID_ARG, "pixel program %d needs target%d", ps.program, i);
+        if (is_device_ptr(t[i])) {
+            m.dev[i] = t[i];
+        } else {
+            HIP_TRY(c, ensure(c, c->tgt[i], bytes));
+            HIP_TRY(c, hipMemcpyAsync(c->tgt[i].p, t[i], bytes, hipMemcpyHostToDevice, c->stream));
+            m.dev[i] = c->tgt[i].p;
+            m.host[i] = t[i];
+            m.bytes[i] = bytes;
+        }
+    }
+    if (ps.program == SWR_PS_LOG) {
+        if (!ps.log_count || (ps.log_capacity && !ps.log))
+            return fail(c, SWR_ERR_INVALID_ARG, "log program needs log and log_count");
+        size_t lb = (size_t)ps.log_capacity * SWR_LOG_WORDS * 4;
+        if (ps.log_capacity == 0 || is_device_ptr(ps.log)) {
+            m.log = ps.log;
+        } else {
+            HIP_TRY(c, ensure(c, c->log, lb));
+            m.log = (uint32_t *)c->log.p;
+            m.log_host = ps.log;
+            m.log_bytes = lb;
+        }
+        if (is_device_ptr(ps.log_count)) {
+            m.logcount = (unsigned long long *)ps.log_count;
+        } else {
+            HIP_TRY(c, ensure(c, c->logcount, 8));
+            HIP_TRY(c, hipMemcpyAsync(c->logcount.p, ps.log_count, 8, hipMemcpyHostToDevice,
+                                      c->stream));
+            m.logcount = (unsigned long long *)c->logcount.p;
+            m.logcount_host = (unsigned long long *)ps.log_count;
+        }
+    }
+    return SWR_OK;
+}
+
+int unmap_targets(swr_context *c, const TargetMap &m)
+{
+    for (int i = 0; i < 3; ++i)
+        if (m.host[i])
+            HIP_TRY(c, hipMemcpyAsync(m.host[i], m.dev[i], m.bytes[i], hipMemcpyDeviceToHost,
+                                      c->stream));
+    if (m.log_host)
+        HIP_TRY(c, hipMemcpyAsync(m.log_host, m.log, m.log_bytes, hipMemcpyDeviceToHost, c->stream));
+    if (m.logcount_host)
+        HIP_TRY(c, hipMemcpyAsync(m.logcount_host, m.logcount, 8, hipMemcpyDeviceToHost, c->stream));
+    return SWR_OK;
+}
+
+void record(swr_context *c, int i)
+{
+    if (c->profiling)
+        (void)hipEventRecord(c->ev[i], c->stream);
+}
+
+int check_status(swr_context *c)
+{
+    unsigned long long err = c->status_host[ST_ERROR];
+    if (err & 1ull)
+        return fail(c, SWR_ERR_INDEX_RANGE, "an index addresses outside the vertex data");
+    if (err & 2ull)
+        return fail(c, SWR_ERR_CLIP_OVERFLOW, "a clipped polygon exceeded %d vertices",
+                    SWR_MAX_CLIP_VERTS);
+    if (c->status_host[ST_OVERFLOW])
+        return fail(c, SWR_ERR_CAPACITY, "arena overflow (prims %llu / %llu, pairs %llu / %llu)",
+                    c->status_host[ST_PRIMS], (unsigned long long)c->prim_cap,
+                    c->status_host[ST_PAIRS], (unsigned long long)c->pair_cap);
+    return SWR_OK;
+}
+
+void collect_stats(swr_context *c)
+{
+    c->stats.raster_primitives = c->status_host[ST_PRIMS];
+    c->stats.tile_pairs = c->status_host[ST_PAIRS];
+    if (c->profiling) {
+        for (int i = 0; i < S_COUNT; ++i) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]) == hipSuccess)
+                c->stats.stage_ms[i] = ms;
+            else
+                c->stats.stage_ms[i] = 0.0f;
+        }
+    }
+}
+
+// The triangle draw: drawElements (window_input = false) or the IRasterizer seam.
+int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool window_input,
+                   const swr_rasterizer_vertex *wverts, size_t nverts, bool sync)
+{
+    const size_t T = count / 3;
+    std::memset(&c->stats, 0, sizeof c->stats);
+    c->stats.input_primitives = T;
+    c->stats.tiles = (uint64_t)((c->ps.width + TILE - 1) / TILE) * (uint64_t)((c->ps.height + TILE - 1) / TILE);
+    if (T == 0 || c->ps.program == SWR_PS_NULL)
+        return SWR_OK;  // nothing observable happens
+    if (T > (size_t)(1u << 31) / 8)
+        return fail(c, SWR_ERR_CAPACITY, "too many triangles (%zu)", T);
+    ParamLayout L;
+    if (!raster_layout(c->ps, L))
+        return fail(c, SWR_ERR_UNSUPPORTED,
+                    "pixel program %d does not support z=%d w=%d avars=%d pvars=%d",
+                    c->ps.program, c->ps.interpolate_z, c->ps.interpolate_w, c->ps.avar_count,
+                    c->ps.pvar_count);
+    const swr_pixel_shader_desc &ps = c->ps;
+    const int W = ps.width, H = ps.height;
+    const int tiles_x = (W + TILE - 1) / TILE, tiles_y = (H + TILE - 1) / TILE;
+    const uint32_t tiles = (uint32_t)tiles_x * (uint32_t)tiles_y;
+    if (tiles == 0)
+        return SWR_OK;
+    if (!sync) {
+        if (!is_device_ptr(indices))
+            return fail(c, SWR_ERR_INVALID_ARG, "async draws take device buffers only");
+    }
+
+    TargetMap tm;
+    int rc = map_targets(c, tm);
+    if (rc)
+        return rc;
+
+    // inputs
+    const void *didx = nullptr;
+    rc = resolve_in(c, indices, T * 3 * sizeof(int32_t), c->idx, &didx);
+    if (rc)
+        return rc;
+    const void *dattr = nullptr;
+    const void *dwv = nullptr;
+    uint64_t attr_bytes = 0;
+    const uint64_t need = vs_need_bytes(c->vs);
+    if (window_input) {
+        rc = resolve_in(c, wverts, nverts * sizeof(swr_rasterizer_vertex), c->wverts, &dwv);
+        if (rc)
+            return rc;
+    } else if (c->vs.program != SWR_VS_DUMMY) {
+        const auto &A = c->attr[0];
+        if (!A.ptr)
+            return fail(c, SWR_ERR_INVALID_ARG, "vertex attribute 0 is not set");
+        if (is_device_ptr(A.ptr)) {
+            dattr = A.ptr;
+            attr_bytes = A.bytes;
+        } else {
+            size_t bytes = A.bytes;
+            if (!bytes) {
+                if (is_device_ptr(indices))
+                    return fail(c, SWR_ERR_INVALID_ARG,
+                                "host vertex data with device indices needs buffer_bytes");
+                int32_t mx = 0;
+                for (size_t i = 0; i < T * 3; ++i) {
+                    if (indices[i] < 0)
+                        return fail(c, SWR_ERR_INDEX_RANGE, "negative index %d at %zu",
+                                    indices[i], i);
+                    mx = std::max(mx, indices[i]);
+                }
+                bytes = (size_t)mx * (size_t)A.stride + need;
+            }
+            rc = resolve_in(c, A.ptr, bytes, c->attr0, &dattr);
+            if (rc)
+                return rc;
+            attr_bytes = bytes;
+        }
+    }
+
+    const uint32_t batches = (uint32_t)((T + BATCH - 1) / BATCH);
+    const uint64_t ngrp = (uint64_t)batches * 2048;
+    HIP_TRY(c, ensure(c, c->app, T * 4));
+    HIP_TRY(c, ensure(c, c->app_ex, T * 4));
+    HIP_TRY(c, ensure(c, c->grp, ngrp * 8));
+    HIP_TRY(c, ensure(c, c->grp_ex, ngrp * 8));
+    HIP_TRY(c, ensure(c, c->scan_tmp, scan_tmp_elems(std::max<uint64_t>(T, ngrp)) * 8));
+    HIP_TRY(c, ensure(c, c->ranges, (size_t)tiles * 8));
+    HIP_TRY(c, ensure(c, c->status, ST_WORDS * 8));
+    c->prim_cap = std::max<uint64_t>(c->prim_cap, T + 256);
+    c->pair_cap = std::max<uint64_t>(c->pair_cap, 2 * T + 4096);
+
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        HIP_TRY(c, ensure(c, c->prims, c->prim_cap * (size_t)L.stride * 4));
+        HIP_TRY(c, ensure(c, c->pk_a, c->pair_cap * 4));
+        HIP_TRY(c, ensure(c, c->pv_a, c->pair_cap * 4));
+        HIP_TRY(c, ensure(c, c->pk_b, c->pair_cap * 4));
+        HIP_TRY(c, ensure(c, c->pv_b, c->pair_cap * 4));
+        uint32_t G = (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(1, c->pair_cap / 4096));
+        HIP_TRY(c, ensure(c, c->hist, (size_t)256 * G * 4));
+        HIP_TRY(c, ensure(c, c->hist_ex, (size_t)256 * G * 4));
+        HIP_TRY(c, ensure(c, c->hist_tmp, scan_tmp_elems((uint64_t)256 * G) * 4));
+
+        GeomArgs g;
+        std::memset(&g, 0, sizeof g);
+        g.indices = (const int32_t *)didx;
+        g.attr0 = (const uint8_t *)dattr;
+        g.stride0 = c->attr[0].stride;
+        g.attr0_bytes = attr_bytes;
+        g.attr0_need = need;
+        std::memcpy(g.mvp, c->vs.mvp, sizeof g.mvp);
+        g.vs_program = c->vs.program;
+        g.vs_na = c->vs.avar_count;
+        g.vs_np = c->vs.pvar_count;
+        g.tris = (uint32_t)T;
+        g.batches = batches;
+        g.raster_mode = c->raster_mode;
+        g.cull_mode = c->cull;
+        g.sc_minx = c->sc[0];
+        g.sc_miny = c->sc[1];
+        g.sc_maxx = c->sc[0] + c->sc[2];
+        g.sc_maxy = c->sc[1] + c->sc[3];
+        // VertexProcessor.cpp:50-53, 373 (host float arithmetic, one rounding per op)
+        volatile float px = (float)c->vp[2] / 2.0f, py = (float)c->vp[3] / 2.0f;
+        g.vp_px = px;
+        g.vp_py = py;
+        g.vp_ox = (float)c->vp[0] + px;
+        g.vp_oy = (float)c->vp[1] + py;
+        volatile float dzs = 0.5f * (c->df - c->dn), dzb = 0.5f * (c->dn + c->df);
+        g.dz_scale = dzs;
+        g.dz_bias = dzb;
+        g.width = W;
+        g.height = H;
+        g.tiles_x = tiles_x;
+        g.tiles_y = tiles_y;
+        g.lay = L;
+        g.window_input = window_input ? 1 : 0;
+        g.wverts = (const swr_rasterizer_vertex *)dwv;
+        g.wvert_count = nverts;
+        g.app = (uint32_t *)c->app.p;
+        g.app_ex = (const uint32_t *)c->app_ex.p;
+        g.grp = (uint2 *)c->grp.p;
+        g.grp_ex = (const uint2 *)c->grp_ex.p;
+        g.prims = (float *)c->prims.p;
+        g.prim_cap = c->prim_cap;
+        g.pair_keys = (uint32_t *)c->pk_a.p;
+        g.pair_vals = (uint32_t *)c->pv_a.p;
+        g.pair_cap = c->pair_cap;
+        g.status = (unsigned long long *)c->status.p;
+        g.ranges = (uint32_t *)c->ranges.p;
+        g.tiles = tiles;
+
+        record(c, 0);
+        HIP_TRY(c, launch_begin(g, c->stream));
+        HIP_TRY(c, launch_geom_count(g, c->stream));
+        record(c, 1);
+        if (!window_input)
+            HIP_TRY(c, scan_u32(g.app, (uint32_t *)c->app_ex.p, T, (uint32_t *)c->scan_tmp.p,
+                                c->stream));
+        HIP_TRY(c, scan_pairs(g.grp, (uint2 *)c->grp_ex.p, ngrp, (uint2 *)c->scan_tmp.p, g.status,
+                              c->prim_cap, c->pair_cap, c->stream));
+        record(c, 2);
+        HIP_TRY(c, launch_geom_emit(g, c->stream));
+        record(c, 3);
+        SortBuffers sb;
+        sb.keys_a = (uint32_t *)c->pk_a.p;
+        sb.vals_a = (uint32_t *)c->pv_a.p;
+        sb.keys_b = (uint32_t *)c->pk_b.p;
+        sb.vals_b = (uint32_t *)c->pv_b.p;
+        sb.hist = (uint32_t *)c->hist.p;
+        sb.hist_ex = (uint32_t *)c->hist_ex.p;
+        sb.scan_tmp = (uint32_t *)c->hist_tmp.p;
+        sb.ranges = (uint32_t *)c->ranges.p;
+        sb.blocks = G;
+        HIP_TRY(c, sort_pairs(sb, g.status, tiles, c->stream));
+        record(c, 4);
+        HIP_TRY(c, launch_ranges(sb, g.status, c->stream));
+        record(c, 5);
+
+        RasterArgs r;
+        std::memset(&r, 0, sizeof r);
+        r.prims = (const float *)c->prims.p;
+        r.stride = L.stride;
+        r.ranges = (const uint32_t *)c->ranges.p;
+        r.vals = sb.sorted_vals;
+        r.status = g.status;
+        r.tiles = (int)tiles;
+        r.tiles_x = tiles_x;
+        r.width = W;
+        r.height = H;
+        r.pitch = ps.pitch;
+        r.sc_minx = g.sc_minx;
+        r.sc_maxx = g.sc_maxx;
+        r.na_set = L.set_na;
+        r.np_set = L.set_np;
+        r.iz = ps.interpolate_z ? 1 : 0;
+        r.iw = ps.interpolate_w ? 1 : 0;
+        r.t0 = tm.dev[0];
+        r.t1 = tm.dev[1];
+        r.t2 = tm.dev[2];
+        r.log = tm.log;
+        r.log_cap = ps.log_capacity;
+        r.log_count = tm.logcount;
+        r.checker_cells = ps.checker_cells;
+        HIP_TRY(c, launch_raster(ps, L, r, c->stream));
+        record(c, 6);
+        HIP_TRY(c, hipMemcpyAsync(c->status_host, c->status.p, ST_WORDS * 8, hipMemcpyDeviceToHost,
+                                  c->stream));
+        if (!sync) {
+            c->async_pending = true;
+            return SWR_OK;
+        }
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        if (c->status_host[ST_OVERFLOW] && !(c->status_host[ST_ERROR])) {
+            if (c->status_host[ST_OVERFLOW] & 1ull)
+                c->prim_cap = c->status_host[ST_PRIMS] + c->status_host[ST_PRIMS] / 4 + 256;
+            if (c->status_host[ST_OVERFLOW] & 2ull)
+                c->pair_cap = c->status_host[ST_PAIRS] + c->status_host[ST_PAIRS] / 4 + 4096;
+            continue;  // nothing was rasterized: run the draw again with larger arenas
+        }
+        break;
+    }
+    rc = check_status(c);
+    if (rc)
+        return rc;
+    rc = unmap_targets(c, tm);
+    if (rc)
+        return rc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    collect_stats(c);
+    return SWR_OK;
+}
+
+}  // namespace
+
+// ---- C ABI ------------------------------------------------------------------------------
+extern "C" {
+
+int swr_create(swr_context **out, int device)
+{
+    if (!out)
+        return SWR_ERR_INVALID_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0)
+        return SWR_ERR_NO_DEVICE;
+    if (hipSetDevice(device) != hipSuccess)
+        return SWR_ERR_NO_DEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess)
+        return SWR_ERR_NO_DEVICE;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return SWR_ERR_NO_DEVICE;
+    swr_context *c = new swr_context();
+    c->device = device;
+    c->ps.program = SWR_PS_NULL;
+    c->ps.checker_cells = 8.0f;
+    c->vs.program = SWR_VS_DUMMY;
+    if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess ||
+        hipHostMalloc((void **)&c->status_host, ST_WORDS * 8, hipHostMallocDefault) != hipSuccess) {
+        delete c;
+        return SWR_ERR_HIP;
+    }
+    std::memset(c->status_host, 0, ST_WORDS * 8);
+    c->stream = c->own;
+    for (int i = 0; i <= S_COUNT; ++i)
+        (void)hipEventCreate(&c->ev[i]);
+    *out = c;
+    return SWR_OK;
+}
+
+int swr_destroy(swr_context *c)
+{
+    if (!c)
+        return SWR_ERR_INVALID_ARG;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    DevBuf *bufs[] = {&c->idx, &c->attr0, &c->wverts, &c->app, &c->app_ex, &c->grp, &c->grp_ex,
+                      &c->scan_tmp, &c->prims, &c->pk_a, &c->pv_a, &c->pk_b, &c->pv_b,
+                      &c->ranges, &c->hist, &c->hist_ex, &c->hist_tmp, &c->status, &c->tgt[0],
+                      &c->tgt[1], &c->tgt[2], &c->log, &c->logcount};
+    for (DevBuf *b : bufs)
+        if (b->p)
+            (void)hipFree(b->p);
+    for (int i = 0; i <= S_COUNT; ++i)
+        if (c->ev[i])
+            (void)hipEventDestroy(c->ev[i]);
+    if (c->status_host)
+        (void)hipHostFree(c->status_host);
+    if (c->own)
+        (void)hipStreamDestroy(c->own);
+    delete c;
+    return SWR_OK;
+}
+
+const char *swr_last_error(const swr_context *c) { return c ? c->err.c_str() : "null context"; }
+
+int swr_set_stream(swr_context *c, void *stream)
+{
+    if (!c)
+        return SWR_ERR_INVALID_ARG;
+    (void)hipSetDevice(c->device);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));  // work already queued stays ordered
+    c->stream = (hipStream_t)stream;
+    return SWR_OK;
+}
+
+int swr_synchronize(swr_context *c)
+{
+    if (!c)
+        return SWR_ERR_INVALID_ARG;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->async_pending) {
+        c->async_pending = false;
+        int rc = check_status(c);
+        if (rc)
+            return rc;
+        collect_stats(c);
+    }
+    return SWR_OK;
+}
+
+int swr_set_raster_mode(swr_context *c, int mode)
+{
+    if (!c || mode < SWR_RASTER_SPAN || mode > SWR_RASTER_ADAPTIVE)
+        return fail(c, SWR_ERR_INVALID_ARG, "bad raster mode %d", mode);
+    c->raster_mode = mode;
+    return SWR_OK;
+}
+
+int swr_set_scissor_rect(swr_context *c, int x, int y, int w, int h)
+{
+    if (!c)
+        return SWR_ERR_INVALID_ARG;
+    c->sc[0] = x;
+    c->sc[1] = y;
+    c->sc[2] = w;
+    c->sc[3] = h;
+    return SWR_OK;
+}
+
+int swr_set_pixel_shader(swr_context *c, const swr_pixel_shader_desc *d)
+{
+    if (!c || !d)
+        return fail(c, SWR_ERR_INVALID_ARG, "null pixel shader");
+    if (d->program < SWR_PS_NULL || d->program > SWR_PS_LOG)
+        return fail(c, SWR_ERR_UNSUPPORTED, "pixel program %d is not built", d->program);
+    ParamLayout L;
+    if (d->program != SWR_PS_NULL && !raster_layout(*d, L))
+        return fail(c, SWR_ERR_UNSUPPORTED, "pixel program %d cannot run with these counts",
+                    d->program);
+    c->ps = *d;
+    return SWR_OK;
+}
+
+int swr_set_viewport(swr_context *c, int x, int y, int w, int h)
+{
+    if (!c)
+        return SWR_ERR_INVALID_ARG;
+    c->vp[0] = x;
+    c->vp[1] = y;
+    c->vp[2] = w;
+    c->vp[3] = h;
+    return SWR_OK;
+}
+
+int swr_set_depth_range(swr_context *c, float n, float f)
+{
+    if (!c)
+        return SWR_ERR_INVALID_ARG;
+    c->dn = n;
+    c->df = f;
+    return SWR_OK;
+}
+
+int swr_set_cull_mode(swr_context *c, int mode)
+{
+    if (!c || mode < SWR_CULL_NONE || mode > SWR_CULL_CW)
+        return fail(c, SWR_ERR_INVALID_ARG, "bad cull mode %d", mode);
+    c->cull = mode;
+    return SWR_OK;
+}
+
+int swr_set_vertex_shader(swr_context *c, const swr_vertex_shader_desc *d)
+{
+    if (!c || !d)
+        return fail(c, SWR_ERR_INVALID_ARG, "null vertex shader");
+    if (d->program < SWR_VS_DUMMY || d->program > SWR_VS_MVP_POS_UV)
+        return fail(c, SWR_ERR_UNSUPPORTED, "vertex program %d is not built", d->program);
+    if (d->attrib_count < 0 || d->attrib_count > SWR_MAX_VERTEX_ATTRIBS || d->avar_count < 0 ||
+        d->avar_count > SWR_MAX_AVARS || d->pvar_count < 0 || d->pvar_count > SWR_MAX_PVARS)
+        return fail(c, SWR_ERR_INVALID_ARG, "vertex shader counts out of range");
+    c->vs = *d;
+    return SWR_OK;
+}
+
+int swr_set_vertex_attrib_pointer(swr_context *c, int index, int stride, const void *buffer,
+                                  size_t buffer_bytes)
+{
+    if (!c || index < 0 || index >= SWR_MAX_VERTEX_ATTRIBS || stride < 0)
+        return fail(c, SWR_ERR_INVALID_ARG, "bad vertex attribute %d", index);
+    c->attr[index].ptr = buffer;
+    c->attr[index].stride = stride;
+    c->attr[index].bytes = buffer_bytes;
+    return SWR_OK;
+}
+
+int swr_draw_elements(swr_context *c, int mode, size_t count, const int32_t *indices)
+{
+    if (!c || (count && !indices))
+        return fail(c, SWR_ERR_INVALID_ARG, "null indices");
+    if (mode != SWR_DRAW_TRIANGLE) {
+        if (mode == SWR_DRAW_POINT || mode == SWR_DRAW_LINE)
+            return fail(c, SWR_ERR_UNSUPPORTED, "point and line drawing is not built yet");
+        return fail(c, SWR_ERR_INVALID_ARG, "bad draw mode %d", mode);
+    }
+    (void)hipSetDevice(c->device);
+    return draw_triangles(c, indices, count, false, nullptr, 0, true);
+}
+
+int swr_draw_elements_async(swr_context *c, int mode, size_t count, const int32_t *indices)
+{
+    if (!c || (count && !indices))
+        return fail(c, SWR_ERR_INVALID_ARG, "null indices");
+    if (mode != SWR_DRAW_TRIANGLE)
+        return fail(c, SWR_ERR_UNSUPPORTED, "async draws take triangles");
+    (void)hipSetDevice(c->device);
+    return draw_triangles(c, indices, count, false, nullptr, 0, false);
+}
+
+int swr_reserve(swr_context *c, size_t primitives, size_t tile_pairs)
+{
+    if (!c)
+        return SWR_ERR_INVALID_ARG;
+    c->prim_cap = std::max<uint64_t>(c->prim_cap, primitives);
+    c->pair_cap = std::max<uint64_t>(c->pair_cap, tile_pairs);
+    return SWR_OK;
+}
+
+int swr_draw_triangle_list(swr_context *c, const swr_rasterizer_vertex *v, size_t nv,
+                           const int32_t *idx, size_t n)
+{
+    if (!c || (n && (!v || !idx)))
+        return fail(c, SWR_ERR_INVALID_ARG, "null vertices or indices");
+    (void)hipSetDevice(c->device);
+    return draw_triangles(c, idx, n, true, v, nv, true);
+}
+
+int swr_draw_line_list(swr_context *c, const swr_rasterizer_vertex *, size_t, const int32_t *,
+                       size_t)
+{
+    return fail(c, SWR_ERR_UNSUPPORTED, "line lists are not built yet");
+}
+
+int swr_draw_point_list(swr_context *c, const swr_rasterizer_vertex *, size_t, const int32_t *,
+                        size_t)
+{
+    return fail(c, SWR_ERR_UNSUPPORTED, "point lists are not built yet");
+}
+
+int swr_get_draw_stats(swr_context *c, swr_draw_stats *out)
+{
+    if (!c || !out)
+        return SWR_ERR_INVALID_ARG;
+    *out = c->stats;
+    return SWR_OK;
+}
+
+int swr_set_profiling(swr_context *c, int enable)
+{
+    if (!c)
+        return SWR_ERR_INVALID_ARG;
+    c->profiling = enable != 0;
+    return SWR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,737 @@
+// swr_geometry.hip — the geometry stage of the draw path on gfx950.
+//
+// One thread per input triangle.  A count pass and an emit pass run the same device code
+// (process_triangle<NV, EMIT>): vertex program, clip masks and Sutherland-Hodgman clipping
+// against the six frustum planes, perspective divide + viewport, facing cull, triangle
+// setup and the tile rectangle each surviving triangle touches.  Between the passes two
+// exclusive scans turn per-triangle counts into draw-order positions, so the emit pass
+// writes primitive records and (tile, primitive) pairs in the reference's draw order with
+// no sorting and no atomics:
+//   key order = (batch, phase, j, k): per 1024-triangle batch, first triangle j's fan
+//   triangle 0 (written in place, VertexProcessor.cpp:254-256), then the fan triangles
+//   k >= 1 of every triangle in j order (appended, VertexProcessor.cpp:257-261).
+//
+// The vertex cache of the reference (VertexCache.h) maps an index to an already shaded
+// output vertex within a batch.  Vertex programs are pure, so re-running the program for
+// every reference to a vertex yields the same bits; the cache is dropped.
+#include "swr_common.h"
+#include "swr_internal.h"
+
+namespace swr {
+
+constexpr int MAX_NODES = 32;
+
+struct Pos4 {
+    float x, y, z, w;
+};
+
+// ---- vertex programs (VertexProcessor.cpp:134-150 + the built-in shaders) ----------------
+__device__ inline void mvp_apply(const float *m, float x, float y, float z, Pos4 &o)
+{
+    float r[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        float s = 0.0f;  // vector_math.h:55-60 sequential dot product
+        s = s + m[4 * i + 0] * x;
+        s = s + m[4 * i + 1] * y;
+        s = s + m[4 * i + 2] * z;
+        s = s + m[4 * i + 3] * 1.0f;
+        r[i] = s;
+    }
+    o.x = r[0];
+    o.y = r[1];
+    o.z = r[2];
+    o.w = r[3];
+}
+
+// Carried varyings: var[0 .. lay.na) avar, var[lay.na .. lay.na+lay.np) pvar.
+template <int NV>
+__device__ inline void fetch_vertex(const GeomArgs &a, int32_t index, Pos4 &p, float *var)
+{
+    p.x = p.y = p.z = p.w = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+        var[k] = 0.0f;
+    if (a.vs_program == SWR_VS_DUMMY)
+        return;
+    const float *src = reinterpret_cast<const float *>(a.attr0 + (size_t)a.stride0 * (size_t)index);
+    const int na = a.lay.na, np = a.lay.np;
+    switch (a.vs_program) {
+    case SWR_VS_RAW: {
+        p.x = src[0];
+        p.y = src[1];
+        p.z = src[2];
+        p.w = src[3];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            if (k < na) {
+                if (k < a.vs_na)
+                    var[k] = src[4 + k];
+            } else if (k < na + np) {
+                int q = k - na;
+                if (q < a.vs_np)
+                    var[k] = src[4 + a.vs_na + q];
+            }
+        }
+        break;
+    }
+    case SWR_VS_POS_RGB:
+        p.x = src[0];
+        p.y = src[1];
+        p.z = src[2];
+        p.w = 1.0f;
+#pragma unroll
+        for (int k = 0; k < NV && k < 3; ++k)
+            if (k < na)
+                var[k] = src[3 + k];
+        break;
+    case SWR_VS_MVP_POS_RGB_UV:
+        mvp_apply(a.mvp, src[0], src[1], src[2], p);
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            if (k < na && k < 3)
+                var[k] = src[3 + k];
+            else if (k >= na && k < na + np && k - na < 2)
+                var[k] = src[6 + (k - na)];
+        }
+        break;
+    case SWR_VS_MVP_POS_UV:
+        mvp_apply(a.mvp, src[0], src[1], src[2], p);
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            if (k >= na && k < na + np && k - na < 2)
+                var[k] = src[6 + (k - na)];
+        break;
+    default:
+        break;
+    }
+}
+
+// VertexProcessor::clipMask, VertexProcessor.cpp:122-132
+__device__ inline int clip_mask(const Pos4 &v)
+{
+    int m = 0;
+    if (v.w - v.x < 0) m |= 0x01;
+    if (v.x + v.w < 0) m |= 0x02;
+    if (v.w - v.y < 0) m |= 0x04;
+    if (v.y + v.w < 0) m |= 0x08;
+    if (v.w - v.z < 0) m |= 0x10;
+    if (v.z + v.w < 0) m |= 0x20;
+    return m;
+}
+
+// a*x + b*y + c*z + d*w for the plane sequence of VertexProcessor.cpp:237-242
+__device__ inline float plane_dot(int p, const Pos4 &v)
+{
+    const float A[6] = {-1, 1, 0, 0, 0, 0};
+    const float B[6] = {0, 0, -1, 1, 0, 0};
+    const float Cc[6] = {0, 0, 0, 0, -1, 1};
+    return A[p] * v.x + B[p] * v.y + Cc[p] * v.z + 1.0f * v.w;
+}
+
+__device__ inline int sgnf(float v) { return (0.0f < v) - (v < 0.0f); }
+
+__device__ inline float lerp1(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+
+// Window-space vertex: VertexProcessor::transformVertices, VertexProcessor.cpp:364-373
+struct WinV {
+    float x, y, z, w;
+};
+
+__device__ inline WinV to_window(const GeomArgs &a, const Pos4 &p)
+{
+    WinV o;
+    float invW = 1.0f / p.w;
+    float x = p.x * invW, y = p.y * invW, z = p.z * invW;
+    o.x = (a.vp_px * x + a.vp_ox);
+    o.y = (a.vp_py * -y + a.vp_oy);
+    o.z = a.dz_scale * z + a.dz_bias;
+    o.w = p.w;
+    return o;
+}
+
+// ---- primitive setup ------------------------------------------------------------------
+struct Edge {
+    float a, b, c;
+    bool tie;
+};
+
+// EdgeEquation::init, EdgeEquation.h:37-43
+__device__ inline Edge edge_init(float x0, float y0, float x1, float y1)
+{
+    Edge e;
+    e.a = y0 - y1;
+    e.b = x1 - x0;
+    e.c = -(e.a * (x0 + x1) + e.b * (y0 + y1)) / 2.0f;
+    e.tie = e.a != 0.0f ? e.a > 0.0f : e.b > 0.0f;
+    return e;
+}
+
+struct SpanPart {
+    float ox, oy, is1, is2;
+    int lo, hi;
+};
+
+struct PrimGeo {
+    Edge e0, e1, e2;
+    float area2;
+    uint32_t flags;
+    int tx0, ty0, tx1, ty1;
+    SpanPart bot, top;
+};
+
+__device__ inline bool finitef(float x) { return __builtin_isfinite(x); }
+
+// x extent [xl, xr) of the span on scanline sy of one flat part (Rasterizer.h:370-380)
+__device__ inline void span_row(const GeomArgs &a, const SpanPart &p, int sy, int &xl, int &xr)
+{
+    float dy = ((float)sy - p.oy) + 0.5f;
+    float cx1 = p.ox + p.is1 * dy + 0.5f;
+    float cx2 = p.ox + p.is2 * dy + 0.5f;
+    int f1 = f2i(cx1), f2 = f2i(cx2);
+    xl = imax(a.sc_minx, f1);
+    xr = imin(a.sc_maxx, f2);
+    if (f1 == INT_MIN || f2 == INT_MIN) {  // non-monotonic sentinel: be conservative
+        xl = a.sc_minx;
+        xr = a.sc_maxx;
+    }
+}
+
+// Adds the tiles a flat part may cover to the running rectangle.
+__device__ inline void span_part_rect(const GeomArgs &a, const SpanPart &p, long long r0,
+                                      long long r1, int &tx0, int &ty0, int &tx1, int &ty1)
+{
+    if (r0 < 0) r0 = 0;
+    if (r1 > a.height - 1) r1 = a.height - 1;
+    if (r0 > r1)
+        return;
+    int xl0, xr0, xl1, xr1;
+    span_row(a, p, (int)r0, xl0, xr0);
+    span_row(a, p, (int)r1, xl1, xr1);
+    long long x0 = imin(xl0, xl1), x1 = (long long)imax(xr0, xr1) - 1;
+    if (x0 < 0) x0 = 0;
+    if (x1 > a.width - 1) x1 = a.width - 1;
+    if (x0 > x1)
+        return;
+    tx0 = imin(tx0, (int)(x0 >> 3));
+    tx1 = imax(tx1, (int)(x1 >> 3));
+    ty0 = imin(ty0, (int)(r0 >> 3));
+    ty1 = imax(ty1, (int)(r1 >> 3));
+}
+
+// Rasterizer::drawTriangleSpanTemplate, Rasterizer.h:318-357: vertex order and the two
+// flat parts (drawBottomFlatTriangle :360-385, drawTopFlatTriangle :387-411).
+__device__ inline void span_setup(const GeomArgs &a, const WinV &v0, const WinV &v1,
+                                  const WinV &v2, PrimGeo &g)
+{
+    float tx = v0.x, ty = v0.y, mx = v1.x, my = v1.y, bx = v2.x, by = v2.y, sx, sy;
+#define SWAP_V(P, Q) { sx = P##x; sy = P##y; P##x = Q##x; P##y = Q##y; Q##x = sx; Q##y = sy; }
+    if (ty > my) SWAP_V(t, m);
+    if (my > by) SWAP_V(m, b);
+    if (ty > my) SWAP_V(t, m);
+#undef SWAP_V
+    float dy = by - ty;
+    float iy = my - ty;
+    g.flags &= ~(FLAG_BOTTOM | FLAG_TOP);
+    auto bottom = [&](float x0, float y0, float x1, float y1, float x2, float y2) {
+        g.bot.is1 = (x1 - x0) / (y1 - y0);
+        g.bot.is2 = (x2 - x0) / (y2 - y0);
+        g.bot.ox = x0;
+        g.bot.oy = y0;
+        g.bot.lo = f2i(y0 + 0.5f);
+        g.bot.hi = f2i(y1 + 0.5f);
+        g.flags |= FLAG_BOTTOM;
+    };
+    auto top = [&](float x0, float y0, float x1, float y1, float x2, float y2) {
+        g.top.is1 = (x2 - x0) / (y2 - y0);
+        g.top.is2 = (x2 - x1) / (y2 - y1);
+        g.top.ox = x2;
+        g.top.oy = y2;
+        g.top.hi = f2i(y2 - 0.5f);
+        g.top.lo = f2i(y0 - 0.5f);
+        g.flags |= FLAG_TOP;
+    };
+    if (my == ty) {
+        float lx = mx, ly = my, rx = tx, ry = ty;
+        if (lx > rx) { float s0 = lx, s1 = ly; lx = rx; ly = ry; rx = s0; ry = s1; }
+        top(lx, ly, rx, ry, bx, by);
+    } else if (my == by) {
+        float lx = mx, ly = my, rx = bx, ry = by;
+        if (lx > rx) { float s0 = lx, s1 = ly; lx = rx; ly = ry; rx = s0; ry = s1; }
+        bottom(tx, ty, lx, ly, rx, ry);
+    } else {
+        float v4x = tx + ((bx - tx) / dy) * iy, v4y = my;  // split vertex (Rasterizer.h:344-346)
+        float lx = mx, ly = my, rx = v4x, ry = v4y;
+        if (lx > rx) { float s0 = lx, s1 = ly; lx = rx; ly = ry; rx = s0; ry = s1; }
+        bottom(tx, ty, lx, ly, rx, ry);
+        top(lx, ly, rx, ry, bx, by);
+    }
+    int tx0 = INT_MAX, ty0 = INT_MAX, tx1 = INT_MIN, ty1 = INT_MIN;
+    if (g.flags & FLAG_BOTTOM)
+        span_part_rect(a, g.bot, (long long)g.bot.lo, (long long)g.bot.hi - 1, tx0, ty0, tx1, ty1);
+    if (g.flags & FLAG_TOP)
+        span_part_rect(a, g.top, (long long)g.top.lo + 1, (long long)g.top.hi, tx0, ty0, tx1, ty1);
+    g.tx0 = tx0;
+    g.ty0 = ty0;
+    g.tx1 = tx1;
+    g.ty1 = ty1;
+}
+
+// Rasterizer::drawTriangleBlockTemplate, Rasterizer.h:234-258: the block grid as a tile rect.
+__device__ inline void block_setup(const GeomArgs &a, const WinV &v0, const WinV &v1,
+                                   const WinV &v2, PrimGeo &g)
+{
+    long long minX = f2i(ref_min(ref_min(v0.x, v1.x), v2.x));
+    long long maxX = f2i(ref_max(ref_max(v0.x, v1.x), v2.x));
+    long long minY = f2i(ref_min(ref_min(v0.y, v1.y), v2.y));
+    long long maxY = f2i(ref_max(ref_max(v0.y, v1.y), v2.y));
+    minX = minX > a.sc_minx ? minX : a.sc_minx;
+    maxX = maxX < a.sc_maxx ? maxX : a.sc_maxx;
+    minY = minY > a.sc_miny ? minY : a.sc_miny;
+    maxY = maxY < a.sc_maxy ? maxY : a.sc_maxy;
+    minX &= ~7LL;
+    maxX &= ~7LL;
+    minY &= ~7LL;
+    maxY &= ~7LL;
+    long long sx = (maxX - minX) / 8 + 1, sy = (maxY - minY) / 8 + 1;
+    long long bx0, bx1, by0, by1;
+    g.tx0 = INT_MAX;
+    g.tx1 = INT_MIN;
+    g.ty0 = INT_MAX;
+    g.ty1 = INT_MIN;
+    if (sx > 0 && sy > 0) {
+        bx0 = minX; bx1 = minX + 8 * (sx - 1);
+        by0 = minY; by1 = minY + 8 * (sy - 1);
+    } else if (sx < 0 && sy < 0) {  // both negative: positive block count (Rasterizer.h:254-258)
+        bx0 = minX; bx1 = minX + 8 * (-sx - 1);
+        by0 = minY - 8 * (-sy - 1); by1 = minY;
+    } else {
+        return;
+    }
+    long long t0 = bx0 >> 3, t1 = bx1 >> 3, u0 = by0 >> 3, u1 = by1 >> 3;
+    if (t0 < 0) t0 = 0;
+    if (u0 < 0) u0 = 0;
+    if (t1 > a.tiles_x - 1) t1 = a.tiles_x - 1;
+    if (u1 > a.tiles_y - 1) u1 = a.tiles_y - 1;
+    if (t0 > t1 || u0 > u1)
+        return;
+    g.tx0 = (int)t0;
+    g.tx1 = (int)t1;
+    g.ty0 = (int)u0;
+    g.ty1 = (int)u1;
+}
+
+// Setup shared by both passes.  Returns false when the triangle draws nothing.
+__device__ inline bool prim_geometry(const GeomArgs &a, const WinV &v0, const WinV &v1,
+                                     const WinV &v2, PrimGeo &g)
+{
+    if (!(finitef(v0.x) && finitef(v0.y) && finitef(v1.x) && finitef(v1.y) && finitef(v2.x) &&
+          finitef(v2.y)))
+        return false;
+    g.e0 = edge_init(v1.x, v1.y, v2.x, v2.y);
+    g.e1 = edge_init(v2.x, v2.y, v0.x, v0.y);
+    g.e2 = edge_init(v0.x, v0.y, v1.x, v1.y);
+    g.area2 = g.e0.c + g.e1.c + g.e2.c;
+    if (!(g.area2 > 0.0f) || !finitef(g.area2))  // TriangleEquations.h:53-56, Rasterizer.h:231
+        return false;
+    g.flags = (g.e0.tie ? FLAG_TIE0 : 0u) | (g.e1.tie ? FLAG_TIE1 : 0u) | (g.e2.tie ? FLAG_TIE2 : 0u);
+    bool block = a.raster_mode == SWR_RASTER_BLOCK;
+    if (a.raster_mode == SWR_RASTER_ADAPTIVE) {  // Rasterizer.h:413-428
+        float minX = ref_min(ref_min(v0.x, v1.x), v2.x);
+        float maxX = ref_max(ref_max(v0.x, v1.x), v2.x);
+        float minY = ref_min(ref_min(v0.y, v1.y), v2.y);
+        float maxY = ref_max(ref_max(v0.y, v1.y), v2.y);
+        float orient = (maxX - minX) / (maxY - minY);
+        block = (double)orient > 0.4 && (double)orient < 1.6;
+    }
+    if (block) {
+        g.flags |= FLAG_BLOCK;
+        block_setup(a, v0, v1, v2, g);
+    } else {
+        span_setup(a, v0, v1, v2, g);
+    }
+    return g.tx0 <= g.tx1 && g.ty0 <= g.ty1;
+}
+
+// ParameterEquation::init, ParameterEquation.h:36-48
+__device__ inline void param_write(float *rec, int slot, float p0, float p1, float p2,
+                                   const PrimGeo &g, float f)
+{
+    rec[REC_PARAM + 3 * slot + 0] = f * (p0 * g.e0.a + p1 * g.e1.a + p2 * g.e2.a);
+    rec[REC_PARAM + 3 * slot + 1] = f * (p0 * g.e0.b + p1 * g.e1.b + p2 * g.e2.b);
+    rec[REC_PARAM + 3 * slot + 2] = f * (p0 * g.e0.c + p1 * g.e1.c + p2 * g.e2.c);
+}
+
+template <int NV>
+__device__ void write_prim(const GeomArgs &a, uint64_t pos, uint32_t key, const PrimGeo &g,
+                           const WinV &v0, const WinV &v1, const WinV &v2, const float *var0,
+                           const float *var1, const float *var2, uint64_t pair_base)
+{
+    float *rec = a.prims + pos * (uint64_t)a.lay.stride;
+    uint32_t *ru = reinterpret_cast<uint32_t *>(rec);
+    ru[REC_KEY] = key;
+    ru[REC_FLAGS] = g.flags;
+    ru[REC_TILES + 0] = (uint32_t)g.tx0;
+    ru[REC_TILES + 1] = (uint32_t)g.ty0;
+    ru[REC_TILES + 2] = (uint32_t)g.tx1;
+    ru[REC_TILES + 3] = (uint32_t)g.ty1;
+    rec[REC_EDGE + 0] = g.e0.a;
+    rec[REC_EDGE + 1] = g.e0.b;
+    rec[REC_EDGE + 2] = g.e0.c;
+    rec[REC_EDGE + 3] = g.e1.a;
+    rec[REC_EDGE + 4] = g.e1.b;
+    rec[REC_EDGE + 5] = g.e1.c;
+    rec[REC_EDGE + 6] = g.e2.a;
+    rec[REC_EDGE + 7] = g.e2.b;
+    rec[REC_EDGE + 8] = g.e2.c;
+    if (!(g.flags & FLAG_BLOCK)) {
+        const SpanPart *parts[2] = {&g.bot, &g.top};
+        const uint32_t pf[2] = {FLAG_BOTTOM, FLAG_TOP};
+        const int base[2] = {REC_BOT, REC_TOP};
+        for (int q = 0; q < 2; ++q) {
+            if (!(g.flags & pf[q]))
+                continue;
+            rec[base[q] + 0] = parts[q]->ox;
+            rec[base[q] + 1] = parts[q]->oy;
+            rec[base[q] + 2] = parts[q]->is1;
+            rec[base[q] + 3] = parts[q]->is2;
+            ru[base[q] + 4] = (uint32_t)parts[q]->lo;
+            ru[base[q] + 5] = (uint32_t)parts[q]->hi;
+        }
+    }
+    // TriangleEquations.h:58-70
+    const ParamLayout &L = a.lay;
+    float f = 1.0f / g.area2;
+    int slot = 0;
+    if (L.nz)
+        param_write(rec, slot++, v0.z, v1.z, v2.z, g, f);
+    float iw0 = 1.0f / v0.w, iw1 = 1.0f / v1.w, iw2 = 1.0f / v2.w;
+    if (L.nw)
+        param_write(rec, slot++, iw0, iw1, iw2, g, f);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        if (k < L.na) {
+            if (k < L.set_na)
+                param_write(rec, slot + k, var0[k], var1[k], var2[k], g, f);
+            else
+                param_write(rec, slot + k, 0.0f, 0.0f, 0.0f, g, 0.0f);
+        } else if (k < L.na + L.np) {
+            if (k - L.na < L.set_np)
+                param_write(rec, slot + k, var0[k] * iw0, var1[k] * iw1, var2[k] * iw2, g, f);
+            else
+                param_write(rec, slot + k, 0.0f, 0.0f, 0.0f, g, 0.0f);
+        }
+    }
+    // (tile, primitive) pairs in row-major tile order
+    uint64_t n = 0;
+    for (int ty = g.ty0; ty <= g.ty1; ++ty)
+        for (int tx = g.tx0; tx <= g.tx1; ++tx) {
+            a.pair_keys[pair_base + n] = (uint32_t)(ty * a.tiles_x + tx);
+            a.pair_vals[pair_base + n] = (uint32_t)pos;
+            ++n;
+        }
+}
+
+__device__ inline uint64_t rect_tiles(const PrimGeo &g)
+{
+    return (uint64_t)(g.tx1 - g.tx0 + 1) * (uint64_t)(g.ty1 - g.ty0 + 1);
+}
+
+// Per-input-triangle bookkeeping shared by the fan triangles it produces.
+struct FanCtx {
+    uint32_t t, b, j;
+    uint32_t slot1;                 // batch-local slot of fan triangle 1
+    uint64_t pos0, pairs0, pos1, pairs1;
+    uint32_t cnt0, cntp0, cnt1, cntp1;
+};
+
+// One fan triangle (p0, p[k+1], p[k+2]): transform, cull, setup, count or emit.
+template <int NV, bool EMIT>
+__device__ inline void fan_triangle(const GeomArgs &a, FanCtx &c, int k, WinV w0, WinV w1,
+                                    WinV w2, const float *v0, const float *v1, const float *v2,
+                                    bool cull)
+{
+    if (cull) {
+        // VertexProcessor::cullTriangles, VertexProcessor.cpp:330-343
+        float facing = (w0.x - w1.x) * (w2.y - w1.y) - (w2.x - w1.x) * (w0.y - w1.y);
+        if (facing < 0) {
+            if (a.cull_mode == SWR_CULL_CW)
+                return;
+        } else {
+            if (a.cull_mode == SWR_CULL_CCW)
+                return;
+            WinV s = w0;  // indices i and i+2 swap (VertexProcessor.cpp:342)
+            w0 = w2;
+            w2 = s;
+            const float *sv = v0;
+            v0 = v2;
+            v2 = sv;
+        }
+    }
+    PrimGeo g;
+    if (!prim_geometry(a, w0, w1, w2, g))
+        return;
+    uint32_t nt = (uint32_t)rect_tiles(g);
+    if (!EMIT) {
+        if (k == 0) {
+            c.cnt0 += 1;
+            c.cntp0 += nt;
+        } else {
+            c.cnt1 += 1;
+            c.cntp1 += nt;
+        }
+        return;
+    }
+    uint32_t key = a.window_input ? c.t
+                                  : ((c.b << 15) + (k == 0 ? c.j : c.slot1 + (uint32_t)(k - 1)));
+    uint64_t pos, pb;
+    if (k == 0) {
+        pos = c.pos0;
+        pb = c.pairs0;
+    } else {
+        pos = c.pos1 + c.cnt1;
+        pb = c.pairs1 + c.cntp1;
+        c.cnt1 += 1;
+        c.cntp1 += nt;
+    }
+    if (pos >= a.prim_cap || pb + nt > a.pair_cap) {  // totals are checked; this guards wrap
+        atomicOr(&a.status[ST_OVERFLOW], 4ull);
+        return;
+    }
+    write_prim<NV>(a, pos, key, g, w0, w1, w2, v0, v1, v2, pb);
+}
+
+// Clipped triangles: polygon nodes live in per-thread arrays (rare path, kept out of line).
+template <int NV, bool EMIT>
+__device__ __noinline__ void clip_path(const GeomArgs &a, FanCtx &c, int cm, const Pos4 *in3,
+                                       const float (*var3)[NV])
+{
+    Pos4 P[MAX_NODES];
+    float var[EMIT ? MAX_NODES : 1][NV];
+    uint8_t poly_a[MAX_CLIP], poly_b[MAX_CLIP];
+    uint8_t *poly = poly_a, *other = poly_b;
+    for (int q = 0; q < 3; ++q) {
+        P[q] = in3[q];
+        if (EMIT)
+            for (int k = 0; k < NV; ++k)
+                var[q][k] = var3[q][k];
+        poly[q] = (uint8_t)q;
+    }
+    int n = 3, nn = 3;
+    // VertexProcessor.cpp:233-250 with PolyClipper::clipToPlane (PolyClipper.cpp:45-81)
+    for (int p = 0; p < 6; ++p) {
+        if (!(cm & (1 << p)) || n < 3)
+            continue;
+        int m = 0;
+        int iprev = poly[0];
+        float dprev = plane_dot(p, P[iprev]);
+        for (int i = 1; i <= n; ++i) {
+            int ic = (i == n) ? poly[0] : poly[i];
+            float dp = plane_dot(p, P[ic]);
+            if (dprev >= 0) {
+                if (m >= MAX_CLIP) { m = -1; break; }
+                other[m++] = (uint8_t)iprev;
+            }
+            if (sgnf(dp) != sgnf(dprev)) {
+                if (m >= MAX_CLIP || nn >= MAX_NODES) { m = -1; break; }
+                float tt = dp < 0 ? dprev / (dprev - dp) : -dprev / (dp - dprev);
+                const Pos4 A = P[iprev], B = P[ic];
+                P[nn].x = lerp1(A.x, B.x, tt);
+                P[nn].y = lerp1(A.y, B.y, tt);
+                P[nn].z = lerp1(A.z, B.z, tt);
+                P[nn].w = lerp1(A.w, B.w, tt);
+                if (EMIT) {
+                    // Helper::interpolateVertex lerps the vertex shader's counts only
+                    for (int k = 0; k < NV; ++k) {
+                        bool lerped = k < a.lay.na ? (k < a.vs_na) : ((k - a.lay.na) < a.vs_np);
+                        var[nn][k] = lerped ? lerp1(var[iprev][k], var[ic][k], tt) : 0.0f;
+                    }
+                }
+                other[m++] = (uint8_t)nn++;
+            }
+            iprev = ic;
+            dprev = dp;
+        }
+        if (m < 0) {
+            atomicOr(&a.status[ST_ERROR], 2ull);
+            n = 0;
+            break;
+        }
+        uint8_t *s = poly;
+        poly = other;
+        other = s;
+        n = m;
+    }
+    if (!EMIT)
+        a.app[c.t] = n > 3 ? (uint32_t)(n - 3) : 0u;
+    for (int k = 0; k + 2 < n; ++k) {
+        int ia = poly[0], ib = poly[k + 1], ic = poly[k + 2];
+        fan_triangle<NV, EMIT>(a, c, k, to_window(a, P[ia]), to_window(a, P[ib]),
+                               to_window(a, P[ic]), var[EMIT ? ia : 0], var[EMIT ? ib : 0],
+                               var[EMIT ? ic : 0], true);
+    }
+}
+
+template <int NV, bool EMIT>
+__device__ void process_triangle(const GeomArgs &a, uint32_t t)
+{
+    FanCtx c;
+    c.t = t;
+    c.b = t >> 10;
+    c.j = t & 1023u;
+    c.slot1 = 0;
+    c.pos0 = c.pairs0 = c.pos1 = c.pairs1 = 0;
+    c.cnt0 = c.cntp0 = c.cnt1 = c.cntp1 = 0;
+    const uint32_t nbt = (c.b + 1 < a.batches) ? (uint32_t)BATCH : a.tris - c.b * (uint32_t)BATCH;
+    if (EMIT) {
+        if (!a.window_input)
+            c.slot1 = nbt + (a.app_ex[t] - a.app_ex[c.b * BATCH]);
+        uint2 g0 = a.grp_ex[(size_t)c.b * 2048 + c.j];
+        uint2 g1 = a.grp_ex[(size_t)c.b * 2048 + 1024 + c.j];
+        c.pos0 = g0.x;
+        c.pairs0 = g0.y;
+        c.pos1 = g1.x;
+        c.pairs1 = g1.y;
+    }
+    constexpr int NVS = NV > 0 ? NV : 1;
+    float var3[3][NVS];
+
+    if (a.window_input) {
+        // IRasterizer seam: window-space vertices, no clip, transform or cull
+        int32_t i0 = a.indices[3 * t], i1 = a.indices[3 * t + 1], i2 = a.indices[3 * t + 2];
+        if (i0 != -1) {
+            if (i0 < 0 || i1 < 0 || i2 < 0 || (uint64_t)i0 >= a.wvert_count ||
+                (uint64_t)i1 >= a.wvert_count || (uint64_t)i2 >= a.wvert_count) {
+                atomicOr(&a.status[ST_ERROR], 1ull);
+            } else {
+                const int32_t ids[3] = {i0, i1, i2};
+                WinV w[3];
+                for (int q = 0; q < 3; ++q) {
+                    const swr_rasterizer_vertex &v = a.wverts[ids[q]];
+                    w[q].x = v.x;
+                    w[q].y = v.y;
+                    w[q].z = v.z;
+                    w[q].w = v.w;
+                    if (EMIT) {
+#pragma unroll
+                        for (int k = 0; k < NV; ++k) {
+                            int q2 = k - a.lay.na;
+                            var3[q][k] = k < a.lay.na ? v.avar[k & 15]
+                                                      : (q2 < a.lay.np ? v.pvar[q2 & 15] : 0.0f);
+                        }
+                    }
+                }
+                fan_triangle<NV, EMIT>(a, c, 0, w[0], w[1], w[2], var3[0], var3[1], var3[2],
+                                       false);
+            }
+        }
+    } else {
+        int32_t ids[3] = {a.indices[3 * t], a.indices[3 * t + 1], a.indices[3 * t + 2]};
+        bool bad = false;
+        for (int q = 0; q < 3; ++q)
+            if (ids[q] < 0 ||
+                (a.attr0_bytes && (uint64_t)ids[q] * (uint64_t)a.stride0 + a.attr0_need > a.attr0_bytes))
+                bad = true;
+        if (bad) {
+            atomicOr(&a.status[ST_ERROR], 1ull);
+            if (!EMIT)
+                a.app[t] = 0;
+        } else {
+            Pos4 P[3];
+            for (int q = 0; q < 3; ++q)
+                fetch_vertex<EMIT ? NV : 0>(a, ids[q], P[q], var3[q]);
+            int cm = clip_mask(P[0]) | clip_mask(P[1]) | clip_mask(P[2]);
+            if (cm == 0) {
+                if (!EMIT)
+                    a.app[t] = 0;
+                fan_triangle<NV, EMIT>(a, c, 0, to_window(a, P[0]), to_window(a, P[1]),
+                                       to_window(a, P[2]), var3[0], var3[1], var3[2], true);
+            } else {
+                clip_path<NVS, EMIT>(a, c, cm, P, var3);
+            }
+        }
+    }
+    if (!EMIT) {
+        a.grp[(size_t)c.b * 2048 + c.j] = make_uint2(c.cnt0, c.cntp0);
+        a.grp[(size_t)c.b * 2048 + 1024 + c.j] = make_uint2(c.cnt1, c.cntp1);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_begin(GeomArgs a)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ST_WORDS)
+        a.status[i] = 0;
+    for (uint32_t k = i; k < 2 * a.tiles; k += gridDim.x * blockDim.x)
+        a.ranges[k] = 0;
+    // the last batch's unused (batch, phase, j) slots count nothing
+    uint32_t tail = a.batches * 1024u - a.tris;
+    for (uint32_t k = i; k < tail; k += gridDim.x * blockDim.x) {
+        uint32_t tt = a.tris + k;
+        uint32_t bb = tt >> 10, jj = tt & 1023u;
+        a.grp[(size_t)bb * 2048 + jj] = make_uint2(0, 0);
+        a.grp[(size_t)bb * 2048 + 1024 + jj] = make_uint2(0, 0);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_geom_count(GeomArgs a)
+{
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < a.tris)
+        process_triangle<0, false>(a, t);
+}
+
+template <int NV>
+__global__ void __launch_bounds__(256) k_geom_emit(GeomArgs a)
+{
+    if (a.status[ST_OVERFLOW] | a.status[ST_ERROR])
+        return;
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < a.tris)
+        process_triangle<NV, true>(a, t);
+}
+
+// ---- host launchers -----------------------------------------------------------------------
+hipError_t launch_begin(const GeomArgs &a, hipStream_t s)
+{
+    uint32_t work = 2 * a.tiles > a.batches * 1024u ? 2 * a.tiles : a.batches * 1024u;
+    uint32_t blocks = (work + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_begin, dim3(blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_geom_count(const GeomArgs &a, hipStream_t s)
+{
+    if (!a.tris)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_geom_count, dim3((a.tris + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+int emit_class(int nvary)
+{
+    if (nvary <= 0) return 0;
+    if (nvary <= 3) return 3;
+    if (nvary <= 5) return 5;
+    if (nvary <= 8) return 8;
+    return 32;
+}
+
+hipError_t launch_geom_emit(const GeomArgs &a, hipStream_t s)
+{
+    if (!a.tris)
+        return hipSuccess;
+    dim3 g((a.tris + 255) / 256), blk(256);
+    switch (emit_class(a.lay.na + a.lay.np)) {
+    case 0: hipLaunchKernelGGL(k_geom_emit<0>, g, blk, 0, s, a); break;
+    case 3: hipLaunchKernelGGL(k_geom_emit<3>, g, blk, 0, s, a); break;
+    case 5: hipLaunchKernelGGL(k_geom_emit<5>, g, blk, 0, s, a); break;
+    case 8: hipLaunchKernelGGL(k_geom_emit<8>, g, blk, 0, s, a); break;
+    default: hipLaunchKernelGGL(k_geom_emit<32>, g, blk, 0, s, a); break;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace swr

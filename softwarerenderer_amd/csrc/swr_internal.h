@@ -1,0 +1,37 @@
+// swr_internal.h — launchers shared between the kernel files and the C-ABI layer.
+#pragma once
+
+#include "swr_common.h"
+
+namespace swr {
+
+struct SortBuffers {
+    uint32_t *keys_a, *vals_a, *keys_b, *vals_b;
+    uint32_t *hist, *hist_ex, *scan_tmp;
+    uint32_t *ranges;
+    uint32_t blocks;
+    uint32_t *sorted_keys, *sorted_vals;  // set by sort_pairs
+};
+
+// geometry (swr_geometry.hip)
+hipError_t launch_begin(const GeomArgs &a, hipStream_t s);
+hipError_t launch_geom_count(const GeomArgs &a, hipStream_t s);
+hipError_t launch_geom_emit(const GeomArgs &a, hipStream_t s);
+
+// scans and sort (swr_sort.hip)
+uint64_t scan_tmp_elems(uint64_t n);
+hipError_t scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *tmp, hipStream_t s);
+hipError_t scan_pairs(const uint2 *in, uint2 *out, uint64_t n, uint2 *tmp,
+                      unsigned long long *status, uint64_t cap_prims, uint64_t cap_pairs,
+                      hipStream_t s);
+int radix_passes(uint32_t tiles);
+hipError_t sort_pairs(SortBuffers &sb, const unsigned long long *status, uint32_t tiles,
+                      hipStream_t s);
+hipError_t launch_ranges(const SortBuffers &sb, const unsigned long long *status, hipStream_t s);
+
+// raster (swr_raster.hip)
+bool raster_layout(const swr_pixel_shader_desc &ps, ParamLayout &L);
+hipError_t launch_raster(const swr_pixel_shader_desc &ps, const ParamLayout &L,
+                         const RasterArgs &a, hipStream_t s);
+
+}  // namespace swr

@@ -1,0 +1,279 @@
+// swr_sort.hip — exclusive scans and the stable (tile, primitive) radix sort.
+//
+// Binning needs, per 8x8 tile, the primitives touching it in draw order.  The emit pass
+// writes (tile, primitive) pairs grouped by primitive in draw order; a stable LSD radix
+// sort on the tile id (8 bits per pass) regroups them by tile while keeping draw order
+// inside each tile.  Ranks inside a block are made stable with wave ballots: the lanes of
+// a wave holding the same digit are found with eight ballots, and a lane's rank is the
+// number of such lanes below it.
+#include "swr_common.h"
+#include "swr_internal.h"
+
+namespace swr {
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+__device__ inline uint32_t sadd(uint32_t a, uint32_t b) { return a + b; }
+__device__ inline uint2 sadd(uint2 a, uint2 b) { return make_uint2(a.x + b.x, a.y + b.y); }
+template <typename T> __device__ inline T szero();
+template <> __device__ inline uint32_t szero<uint32_t>() { return 0u; }
+template <> __device__ inline uint2 szero<uint2>() { return make_uint2(0u, 0u); }
+
+// Block-wide exclusive scan of one value per thread; returns the block total in *total.
+template <typename T>
+__device__ T block_exclusive(T v, T *total)
+{
+    __shared__ T s[SCAN_THREADS];
+    const int tid = threadIdx.x;
+    s[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < SCAN_THREADS; off <<= 1) {
+        T add = tid >= off ? s[tid - off] : szero<T>();
+        __syncthreads();
+        s[tid] = sadd(s[tid], add);
+        __syncthreads();
+    }
+    T incl = s[tid];
+    *total = s[SCAN_THREADS - 1];
+    __syncthreads();
+    T ex = tid ? s[tid - 1] : szero<T>();
+    __syncthreads();
+    (void)incl;
+    return ex;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce(const T *in, uint64_t n, T *sums)
+{
+    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    T acc = szero<T>();
+    for (int i = 0; i < SCAN_ITEMS; ++i)
+        if (base + i < n)
+            acc = sadd(acc, in[base + i]);
+    T total;
+    block_exclusive<T>(acc, &total);
+    if (threadIdx.x == 0)
+        sums[blockIdx.x] = total;
+}
+
+// One block scans the block sums in place (exclusive).
+template <typename T>
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_sums(T *sums, uint32_t nblocks)
+{
+    T carry = szero<T>();
+    for (uint32_t base = 0; base < nblocks; base += SCAN_THREADS) {
+        uint32_t i = base + threadIdx.x;
+        T v = i < nblocks ? sums[i] : szero<T>();
+        T total;
+        T ex = block_exclusive<T>(v, &total);
+        if (i < nblocks)
+            sums[i] = sadd(ex, carry);
+        carry = sadd(carry, total);
+        __syncthreads();
+    }
+}
+
+__device__ inline void publish_totals(uint32_t, unsigned long long *, uint64_t, uint64_t) {}
+__device__ inline void publish_totals(uint2 t, unsigned long long *status, uint64_t cap_x,
+                                      uint64_t cap_y)
+{
+    status[ST_PRIMS] = t.x;
+    status[ST_PAIRS] = t.y;
+    unsigned long long of = 0;
+    if (t.x > cap_x) of |= 1ull;
+    if (t.y > cap_y) of |= 2ull;
+    status[ST_OVERFLOW] = of;
+}
+
+// Writes the exclusive scan.  With `status` set (the (prims, pairs) scan) the thread that
+// owns the last element also publishes the totals and flags arena overflow.
+template <typename T>
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(const T *in, uint64_t n,
+                                                          const T *sums, T *out,
+                                                          unsigned long long *status,
+                                                          uint64_t cap_x, uint64_t cap_y)
+{
+    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    T vals[SCAN_ITEMS];
+    T acc = szero<T>();
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        vals[i] = base + i < n ? in[base + i] : szero<T>();
+        acc = sadd(acc, vals[i]);
+    }
+    T total;
+    T run = sadd(block_exclusive<T>(acc, &total), sums[blockIdx.x]);
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        if (base + i < n) {
+            out[base + i] = run;
+            if (status && base + i == n - 1)
+                publish_totals(sadd(run, vals[i]), status, cap_x, cap_y);
+        }
+        run = sadd(run, vals[i]);
+    }
+}
+
+template <typename T>
+hipError_t scan_impl(const T *in, T *out, uint64_t n, T *tmp, unsigned long long *status,
+                     uint64_t cap_x, uint64_t cap_y, hipStream_t s)
+{
+    if (!n)
+        return hipSuccess;
+    uint32_t nb = (uint32_t)((n + SCAN_TILE - 1) / SCAN_TILE);
+    hipLaunchKernelGGL(k_scan_reduce<T>, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, tmp);
+    hipLaunchKernelGGL(k_scan_sums<T>, dim3(1), dim3(SCAN_THREADS), 0, s, tmp, nb);
+    hipLaunchKernelGGL(k_scan_down<T>, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, tmp, out, status,
+                       cap_x, cap_y);
+    return hipGetLastError();
+}
+
+uint64_t scan_tmp_elems(uint64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
+
+hipError_t scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *tmp, hipStream_t s)
+{
+    return scan_impl<uint32_t>(in, out, n, tmp, nullptr, 0, 0, s);
+}
+
+hipError_t scan_pairs(const uint2 *in, uint2 *out, uint64_t n, uint2 *tmp,
+                      unsigned long long *status, uint64_t cap_prims, uint64_t cap_pairs,
+                      hipStream_t s)
+{
+    return scan_impl<uint2>(in, out, n, tmp, status, cap_prims, cap_pairs, s);
+}
+
+// ---- radix sort of (tile, primitive) pairs ---------------------------------------------
+constexpr int RADIX_THREADS = 256;
+
+__device__ inline void chunk_of(const unsigned long long *status, uint32_t G, uint32_t b,
+                                uint64_t &beg, uint64_t &end)
+{
+    uint64_t m = (status[ST_OVERFLOW] | status[ST_ERROR]) ? 0 : status[ST_PAIRS];
+    uint64_t ch = (m + G - 1) / G;
+    beg = (uint64_t)b * ch;
+    end = beg + ch;
+    if (beg > m) beg = m;
+    if (end > m) end = m;
+}
+
+__global__ void __launch_bounds__(RADIX_THREADS) k_radix_hist(const uint32_t *keys,
+                                                            const unsigned long long *status,
+                                                            int shift, uint32_t *hist)
+{
+    __shared__ uint32_t h[256];
+    const uint32_t G = gridDim.x, b = blockIdx.x;
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    uint64_t beg, end;
+    chunk_of(status, G, b, beg, end);
+    for (uint64_t i = beg + threadIdx.x; i < end; i += RADIX_THREADS)
+        atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    __syncthreads();
+    hist[(size_t)threadIdx.x * G + b] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(RADIX_THREADS) k_radix_scatter(
+    const uint32_t *keys, const uint32_t *vals, uint32_t *okeys, uint32_t *ovals,
+    const unsigned long long *status, int shift, const uint32_t *hist_ex)
+{
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[4][256];
+    const uint32_t G = gridDim.x, b = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    run[tid] = hist_ex[(size_t)tid * G + b];
+    for (int q = 0; q < 4; ++q)
+        wcnt[q][tid] = 0;
+    __syncthreads();
+    uint64_t beg, end;
+    chunk_of(status, G, b, beg, end);
+    const uint64_t m_total = (status[ST_OVERFLOW] | status[ST_ERROR]) ? 0 : status[ST_PAIRS];
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (uint64_t base = beg; base < end; base += RADIX_THREADS) {
+        uint64_t i = base + tid;
+        bool valid = i < end;
+        uint32_t k = valid ? keys[i] : 0u;
+        uint32_t v = valid ? vals[i] : 0u;
+        uint32_t d = (k >> shift) & 255u;
+        unsigned long long m = __ballot(valid);
+        for (int bit = 0; bit < 8; ++bit) {
+            unsigned long long bb = __ballot((d >> bit) & 1u);
+            m &= ((d >> bit) & 1u) ? bb : ~bb;
+        }
+        uint32_t rank = (uint32_t)__popcll(m & below);
+        if (valid && rank == 0)
+            wcnt[w][d] = (uint32_t)__popcll(m);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = run[d] + rank;
+            for (int q = 0; q < w; ++q)
+                pos += wcnt[q][d];
+            if (pos < m_total) {
+                okeys[pos] = k;
+                ovals[pos] = v;
+            }
+        }
+        __syncthreads();
+        run[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
+        for (int q = 0; q < 4; ++q)
+            wcnt[q][tid] = 0;
+        __syncthreads();
+    }
+}
+
+// Per-tile [begin, end) into the sorted pairs; tiles without pairs keep the zeros the
+// begin kernel wrote.
+__global__ void __launch_bounds__(256) k_ranges(const uint32_t *keys,
+                                              const unsigned long long *status, uint32_t *ranges)
+{
+    uint64_t m = (status[ST_OVERFLOW] | status[ST_ERROR]) ? 0 : status[ST_PAIRS];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t k = keys[i];
+        if (i == 0 || keys[i - 1] != k)
+            ranges[2 * (size_t)k] = (uint32_t)i;
+        if (i == m - 1 || keys[i + 1] != k)
+            ranges[2 * (size_t)k + 1] = (uint32_t)(i + 1);
+    }
+}
+
+int radix_passes(uint32_t tiles)
+{
+    int bits = 0;
+    while (bits < 32 && (1ull << bits) < (unsigned long long)tiles)
+        ++bits;
+    return bits == 0 ? 1 : (bits + 7) / 8;
+}
+
+hipError_t sort_pairs(SortBuffers &sb, const unsigned long long *status, uint32_t tiles,
+                      hipStream_t s)
+{
+    int passes = radix_passes(tiles);
+    uint32_t *ka = sb.keys_a, *va = sb.vals_a, *kb = sb.keys_b, *vb = sb.vals_b;
+    for (int p = 0; p < passes; ++p) {
+        int shift = 8 * p;
+        hipLaunchKernelGGL(k_radix_hist, dim3(sb.blocks), dim3(RADIX_THREADS), 0, s, ka, status,
+                           shift, sb.hist);
+        hipError_t e = scan_u32(sb.hist, sb.hist_ex, (uint64_t)256 * sb.blocks, sb.scan_tmp, s);
+        if (e != hipSuccess)
+            return e;
+        hipLaunchKernelGGL(k_radix_scatter, dim3(sb.blocks), dim3(RADIX_THREADS), 0, s, ka, va, kb,
+                           vb, status, shift, sb.hist_ex);
+        uint32_t *t = ka;
+        ka = kb;
+        kb = t;
+        t = va;
+        va = vb;
+        vb = t;
+    }
+    sb.sorted_keys = ka;
+    sb.sorted_vals = va;
+    return hipGetLastError();
+}
+
+hipError_t launch_ranges(const SortBuffers &sb, const unsigned long long *status, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ranges, dim3(1024), dim3(256), 0, s, sb.sorted_keys, status, sb.ranges);
+    return hipGetLastError();
+}
+
+}  // namespace swr
