@@ -236,7 +236,7 @@ struct PsEntry { int a, p; set_ps_fn fn[4]; };
                              set_ps<true, false, A, P>, set_ps<true, true, A, P>}}
 const PsEntry k_ps_table[] = {
     PS_ROW(0, 0), PS_ROW(3, 0), PS_ROW(0, 2), PS_ROW(3, 2), PS_ROW(1, 1), PS_ROW(2, 3),
-    PS_ROW(4, 4), PS_ROW(5, 0), PS_ROW(0, 5), PS_ROW(16, 16),
+    PS_ROW(4, 4), PS_ROW(5, 0), PS_ROW(0, 5), PS_ROW(16, 16), PS_ROW(4, 3), PS_ROW(2, 2),
 };
 
 bool bind_pixel_shader(swr::Rasterizer &r, const swr_pixel_shader_desc &ps)

@@ -61,7 +61,7 @@ struct swr_context {
 
     // arenas
     DevBuf idx, attr0, wverts, app, app_ex, grp, grp_ex, scan_tmp, prims, pk_a, pv_a, pk_b, pv_b,
-        ranges, hist, hist_ex, hist_tmp, status, tgt[3], log, logcount;
+        ranges, hist, hist_ex, hist_tmp, status, tgt[3], log, logcount, bigq;
     uint64_t prim_cap = 0, pair_cap = 0;
     unsigned long long *status_host = nullptr;
 
@@ -374,6 +374,7 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
 
     for (int attempt = 0; attempt < 4; ++attempt) {
         HIP_TRY(c, ensure(c, c->prims, c->prim_cap * (size_t)L.stride * 4));
+        HIP_TRY(c, ensure(c, c->bigq, c->prim_cap * 4));
         HIP_TRY(c, ensure(c, c->pk_a, c->pair_cap * 4));
         HIP_TRY(c, ensure(c, c->pv_a, c->pair_cap * 4));
         HIP_TRY(c, ensure(c, c->pk_b, c->pair_cap * 4));
@@ -431,6 +432,8 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
         g.status = (unsigned long long *)c->status.p;
         g.ranges = (uint32_t *)c->ranges.p;
         g.tiles = tiles;
+        g.sentinel = pair_sentinel(tiles);
+        g.bigq = (uint32_t *)c->bigq.p;
 
         record(c, 0);
         HIP_TRY(c, launch_begin(g, c->stream));
@@ -443,6 +446,7 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
                               c->prim_cap, c->pair_cap, c->stream));
         record(c, 2);
         HIP_TRY(c, launch_geom_emit(g, c->stream));
+        HIP_TRY(c, launch_expand_big(g, c->stream));
         record(c, 3);
         SortBuffers sb;
         sb.keys_a = (uint32_t *)c->pk_a.p;
@@ -456,7 +460,7 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
         sb.blocks = G;
         HIP_TRY(c, sort_pairs(sb, g.status, tiles, c->stream));
         record(c, 4);
-        HIP_TRY(c, launch_ranges(sb, g.status, c->stream));
+        HIP_TRY(c, launch_ranges(sb, g.status, tiles, c->stream));
         record(c, 5);
 
         RasterArgs r;
@@ -560,7 +564,7 @@ int swr_destroy(swr_context *c)
     DevBuf *bufs[] = {&c->idx, &c->attr0, &c->wverts, &c->app, &c->app_ex, &c->grp, &c->grp_ex,
                       &c->scan_tmp, &c->prims, &c->pk_a, &c->pv_a, &c->pk_b, &c->pv_b,
                       &c->ranges, &c->hist, &c->hist_ex, &c->hist_tmp, &c->status, &c->tgt[0],
-                      &c->tgt[1], &c->tgt[2], &c->log, &c->logcount};
+                      &c->tgt[1], &c->tgt[2], &c->log, &c->logcount, &c->bigq};
     for (DevBuf *b : bufs)
         if (b->p)
             (void)hipFree(b->p);

@@ -46,7 +46,13 @@ constexpr int ST_PRIMS = 0;        // primitives emitted
 constexpr int ST_PAIRS = 1;        // (primitive, tile) pairs emitted
 constexpr int ST_OVERFLOW = 2;     // bit 0: prim arena, bit 1: pair arena
 constexpr int ST_ERROR = 3;        // bit 0: index range, bit 1: clip overflow
+constexpr int ST_BIG = 4;          // primitives queued for cooperative pair expansion
 constexpr int ST_WORDS = 8;
+
+// record word 6: index of the primitive's first (tile, primitive) pair
+constexpr int REC_PAIRBASE = 6;
+// primitives with more candidate tiles than this are expanded by a whole workgroup
+constexpr uint32_t BIG_PRIM_TILES = 16;
 
 // Which plane equations the rasterizer interpolates (chosen by the pixel program).
 struct ParamLayout {
@@ -69,6 +75,44 @@ __host__ __device__ inline float ref_min(float a, float b) { return (b < a) ? b 
 __host__ __device__ inline float ref_max(float a, float b) { return (a < b) ? b : a; }
 __host__ __device__ inline int imin(int a, int b) { return (b < a) ? b : a; }
 __host__ __device__ inline int imax(int a, int b) { return (a < b) ? b : a; }
+
+__device__ inline bool edge_pass(float v, bool tie) { return v > 0.0f || (v == 0.0f && tie); }
+
+// Block classification from the four corners, Rasterizer.h:270-304: 0 skip,
+// 1 drawBlock<true> (per-pixel edge tests), 2 drawBlock<false> (all 64 pixels).
+// e[9] = e0.abc e1.abc e2.abc; corners use v + a*7 / v + b*7 (EdgeData::stepX/Y(eqn, s)).
+__device__ inline int classify_block(const float *e, uint32_t flags, int x, int y)
+{
+    const float xf = (float)x + 0.5f, yf = (float)y + 0.5f, s = 7.0f;
+    bool p[4][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float a = e[3 * k], b = e[3 * k + 1], c = e[3 * k + 2];
+        const bool tie = (flags >> (1 + k)) & 1u;
+        float v00 = a * xf + b * yf + c;
+        float v01 = v00 + b * s;
+        float v10 = v00 + a * s;
+        float v11 = v01 + a * s;
+        p[0][k] = edge_pass(v00, tie);
+        p[1][k] = edge_pass(v01, tie);
+        p[2][k] = edge_pass(v10, tie);
+        p[3][k] = edge_pass(v11, tie);
+    }
+    int result = 0;
+    bool same = true;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        result += (p[q][0] && p[q][1] && p[q][2]) ? 1 : 0;
+        // `a == b == c` parses as (a == b) == c: true iff an odd number of edges pass
+        same = same && ((p[q][0] == p[q][1]) == p[q][2]);
+    }
+    if (result == 0)
+        return same ? 0 : 1;
+    return result == 4 ? 2 : 1;
+}
+
+// pair value: primitive index | classification << 30 (Block primitives)
+constexpr uint32_t PAIR_PRIM_MASK = 0x3FFFFFFFu;
 
 // ---- launch arguments ---------------------------------------------------------------
 struct GeomArgs {
@@ -106,6 +150,8 @@ struct GeomArgs {
     unsigned long long *status;
     uint32_t *ranges;           // [tiles*2] cleared by the count pass
     uint32_t tiles;
+    uint32_t sentinel;          // pair key of a block the corner test rejects (sorts last)
+    uint32_t *bigq;             // primitives whose pairs the expansion kernel writes
 };
 
 struct RasterArgs {

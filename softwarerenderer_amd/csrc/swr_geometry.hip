@@ -422,14 +422,61 @@ __device__ void write_prim(const GeomArgs &a, uint64_t pos, uint32_t key, const 
                 param_write(rec, slot + k, 0.0f, 0.0f, 0.0f, g, 0.0f);
         }
     }
-    // (tile, primitive) pairs in row-major tile order
+    ru[REC_PAIRBASE] = (uint32_t)pair_base;
+    // (tile, primitive) pairs in row-major tile order; large primitives are expanded by
+    // k_expand_big with a whole workgroup
+    const uint64_t nt = (uint64_t)(g.tx1 - g.tx0 + 1) * (uint64_t)(g.ty1 - g.ty0 + 1);
+    if (nt > BIG_PRIM_TILES) {
+        unsigned long long q = atomicAdd(&a.status[ST_BIG], 1ull);
+        a.bigq[q] = (uint32_t)pos;
+        return;
+    }
     uint64_t n = 0;
     for (int ty = g.ty0; ty <= g.ty1; ++ty)
         for (int tx = g.tx0; tx <= g.tx1; ++tx) {
-            a.pair_keys[pair_base + n] = (uint32_t)(ty * a.tiles_x + tx);
-            a.pair_vals[pair_base + n] = (uint32_t)pos;
+            uint32_t key = (uint32_t)(ty * a.tiles_x + tx), val = (uint32_t)pos;
+            if (g.flags & FLAG_BLOCK) {
+                int cls = classify_block(rec + REC_EDGE, g.flags, tx * TILE, ty * TILE);
+                key = cls ? key : a.sentinel;
+                val |= (uint32_t)cls << 30;
+            }
+            a.pair_keys[pair_base + n] = key;
+            a.pair_vals[pair_base + n] = val;
             ++n;
         }
+}
+
+// Pairs of the queued large primitives: one workgroup per primitive, threads over tiles.
+__global__ void __launch_bounds__(256) k_expand_big(GeomArgs a)
+{
+    if (a.status[ST_OVERFLOW] | a.status[ST_ERROR])
+        return;
+    const uint64_t nbig = a.status[ST_BIG];
+    for (uint64_t q = blockIdx.x; q < nbig; q += gridDim.x) {
+        const uint32_t pos = a.bigq[q];
+        const float *rec = a.prims + (uint64_t)pos * (uint64_t)a.lay.stride;
+        const uint32_t *ru = reinterpret_cast<const uint32_t *>(rec);
+        const uint32_t flags = ru[REC_FLAGS];
+        const int tx0 = (int)ru[REC_TILES + 0], ty0 = (int)ru[REC_TILES + 1];
+        const int tx1 = (int)ru[REC_TILES + 2], ty1 = (int)ru[REC_TILES + 3];
+        const uint32_t w = (uint32_t)(tx1 - tx0 + 1);
+        const uint64_t nt = (uint64_t)w * (uint64_t)(ty1 - ty0 + 1);
+        const uint64_t pb = ru[REC_PAIRBASE];
+        float e[9];
+        for (int k = 0; k < 9; ++k)
+            e[k] = rec[REC_EDGE + k];
+        for (uint64_t k = threadIdx.x; k < nt; k += blockDim.x) {
+            int tx = tx0 + (int)(k % w), ty = ty0 + (int)(k / w);
+            uint32_t key = (uint32_t)(ty * a.tiles_x + tx), val = pos;
+            if (flags & FLAG_BLOCK) {
+                int cls = classify_block(e, flags, tx * TILE, ty * TILE);
+                key = cls ? key : a.sentinel;
+                val |= (uint32_t)cls << 30;
+            }
+            a.pair_keys[pb + k] = key;
+            a.pair_vals[pb + k] = val;
+        }
+    }
 }
 
 __device__ inline uint64_t rect_tiles(const PrimGeo &g)
@@ -503,7 +550,7 @@ __device__ inline void fan_triangle(const GeomArgs &a, FanCtx &c, int k, WinV w0
 
 // Clipped triangles: polygon nodes live in per-thread arrays (rare path, kept out of line).
 template <int NV, bool EMIT>
-__device__ __noinline__ void clip_path(const GeomArgs &a, FanCtx &c, int cm, const Pos4 *in3,
+__device__ inline void clip_path(const GeomArgs &a, FanCtx &c, int cm, const Pos4 *in3,
                                        const float (*var3)[NV])
 {
     Pos4 P[MAX_NODES];
@@ -707,6 +754,12 @@ hipError_t launch_geom_count(const GeomArgs &a, hipStream_t s)
     if (!a.tris)
         return hipSuccess;
     hipLaunchKernelGGL(k_geom_count, dim3((a.tris + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_expand_big(const GeomArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_expand_big, dim3(1024), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

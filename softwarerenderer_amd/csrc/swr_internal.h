@@ -17,6 +17,7 @@ struct SortBuffers {
 hipError_t launch_begin(const GeomArgs &a, hipStream_t s);
 hipError_t launch_geom_count(const GeomArgs &a, hipStream_t s);
 hipError_t launch_geom_emit(const GeomArgs &a, hipStream_t s);
+hipError_t launch_expand_big(const GeomArgs &a, hipStream_t s);
 
 // scans and sort (swr_sort.hip)
 uint64_t scan_tmp_elems(uint64_t n);
@@ -27,7 +28,9 @@ hipError_t scan_pairs(const uint2 *in, uint2 *out, uint64_t n, uint2 *tmp,
 int radix_passes(uint32_t tiles);
 hipError_t sort_pairs(SortBuffers &sb, const unsigned long long *status, uint32_t tiles,
                       hipStream_t s);
-hipError_t launch_ranges(const SortBuffers &sb, const unsigned long long *status, hipStream_t s);
+hipError_t launch_ranges(const SortBuffers &sb, const unsigned long long *status,
+                         uint32_t tiles, hipStream_t s);
+uint32_t pair_sentinel(uint32_t tiles);
 
 // raster (swr_raster.hip)
 bool raster_layout(const swr_pixel_shader_desc &ps, ParamLayout &L);

@@ -1,29 +1,34 @@
-// swr_raster.hip — the raster stage: one wave64 per 8x8 tile, one lane per pixel.
+// swr_raster.hip — the raster stage: one wave64 per 8x8 tile.
 //
 // A tile is exactly one of the reference's 8x8 blocks (Rasterizer.h:247-250 aligns blocks
-// to multiples of 8 in window space).  The wave walks the tile's primitives in draw order;
-// each lane owns one pixel, so every pixel sees its pixel-shader invocations in the
-// reference's order.  The pixel's target values (colour/depth, hash/count, ...) stay in
-// registers for the whole walk and are written back once, coalesced.
+// to multiples of 8 in window space).  The wave walks the tile's primitives in draw order
+// and every pixel's target values (colour/depth, hash/count, ...) stay in registers for the
+// whole walk, written back once, coalesced.
 //
-// Interpolants are reproduced bit for bit by replaying the reference's incremental chains
-// instead of evaluating plane equations directly:
-//   Block (PixelShaderBase.h:55-94): value(i, j) = init(x+.5, y+.5), +b repeated j times
-//     (PixelData::stepY per row), then +a repeated i times (stepX per pixel); edges alike;
-//     the block is classified from its four corners with v + a*7 (Rasterizer.h:270-304).
-//   Span (PixelShaderBase.h:96-112): value(x) = init(xl+.5, y+.5), +a repeated (x - xl) times,
-//     with [xl, xr) from the flat-part edge walk (Rasterizer.h:370-405).
+// Interpolants are reproduced bit for bit by replaying the reference's incremental chains:
+//   Block (PixelShaderBase.h:55-94): row start R_j = init(x+.5, y+.5) with +b repeated j
+//     times (PixelData::stepY per row), then +a per pixel along the row (stepX); edge
+//     values alike.  The corner classification (Rasterizer.h:270-304) was done at binning
+//     and rides in the pair value.
+//   Span (PixelShaderBase.h:96-112): value(x) = init(xl+.5, y+.5) then +a per pixel from
+//     xl, with [xl, xr) from the flat-part edge walk (Rasterizer.h:370-405).
 //   w = 1/invw and pvar = pvarTemp * w at the pixel (PixelData.h:84-103).
+//
+// Work split.  The chains are independent of draw order, only shading is not.  So the
+// wave takes GS primitives at a time.  Phase A: lane = (slot, row) walks one 8-pixel row
+// of one primitive — 7 adds per parameter produce the row — and the rows go through LDS
+// into pixel-per-lane layout.  Phase B: lane = pixel shades the GS primitives in draw
+// order.
 #include "swr_common.h"
 #include "swr_internal.h"
 
 namespace swr {
 
-template <int NA, int NP>
-struct Pix {
-    float z, w, invw;
-    float avar[NA > 0 ? NA : 1];
-    float pvar[NP > 0 ? NP : 1];
+template <int NZ_, int NW_, int NA_, int NP_>
+struct Layout {
+    static constexpr int NZ = NZ_, NW = NW_, NA = NA_, NP = NP_;
+    static constexpr int N = NZ + NW + NA + NP;
+    static constexpr int Z = 0, IW = NZ, A = NZ + NW, P = NZ + NW + NA;
 };
 
 __device__ inline uint32_t fbits(float f) { return __float_as_uint(f); }
@@ -31,11 +36,12 @@ __device__ inline uint32_t mix32(uint32_t h, uint32_t x) { return (h ^ x) * 1677
 __device__ inline float clamp01(float x) { return x > 0.0f ? (x < 1.0f ? x : 1.0f) : 0.0f; }
 
 // ---- pixel programs (include/swr_hip.h) ---------------------------------------------------
+// v[] holds one pixel's chained values in the Layout order: z, invw, avar..., pvarTemp...
+template <class L>
 struct ProgCoverage {
     struct State { bool hit; };
     __device__ static void load(State &s, const RasterArgs &, size_t) { s.hit = false; }
-    template <class P>
-    __device__ static void shade(State &s, const RasterArgs &, const P &, uint32_t, int, int)
+    __device__ static void shade(State &s, const RasterArgs &, const float *, uint32_t, int, int)
     {
         s.hit = true;
     }
@@ -47,15 +53,15 @@ struct ProgCoverage {
 };
 
 // RasterizerTest.cpp:38-48
+template <class L>
 struct ProgRGB {
     struct State { uint32_t c; bool hit; };
     __device__ static void load(State &s, const RasterArgs &, size_t) { s.hit = false; s.c = 0; }
-    template <class P>
-    __device__ static void shade(State &s, const RasterArgs &, const P &p, uint32_t, int, int)
+    __device__ static void shade(State &s, const RasterArgs &, const float *v, uint32_t, int, int)
     {
-        uint32_t r = (uint32_t)f2i(p.avar[0] * 255.0f);
-        uint32_t g = (uint32_t)f2i(p.avar[1] * 255.0f);
-        uint32_t b = (uint32_t)f2i(p.avar[2] * 255.0f);
+        uint32_t r = (uint32_t)f2i(v[L::A + 0] * 255.0f);
+        uint32_t g = (uint32_t)f2i(v[L::A + 1] * 255.0f);
+        uint32_t b = (uint32_t)f2i(v[L::A + 2] * 255.0f);
         s.c = (r << 16) | (g << 8) | b;
         s.hit = true;
     }
@@ -66,6 +72,7 @@ struct ProgRGB {
     }
 };
 
+template <class L>
 struct ProgDepthChecker {
     struct State { float depth; uint32_t color; bool hit; };
     __device__ static void load(State &s, const RasterArgs &a, size_t o)
@@ -74,17 +81,19 @@ struct ProgDepthChecker {
         s.color = 0;
         s.hit = false;
     }
-    template <class P>
-    __device__ static void shade(State &s, const RasterArgs &a, const P &p, uint32_t, int, int)
+    __device__ static void shade(State &s, const RasterArgs &a, const float *v, uint32_t, int, int)
     {
-        if (p.z < s.depth) {
-            s.depth = p.z;
-            int cu = f2i(floorf(p.pvar[0] * a.checker_cells));
-            int cv = f2i(floorf(p.pvar[1] * a.checker_cells));
+        const float z = v[L::Z];
+        if (z < s.depth) {
+            s.depth = z;
+            const float w = 1.0f / v[L::IW];  // only the depth-passing pixels read pvar
+            const float u = v[L::P + 0] * w, t = v[L::P + 1] * w;
+            int cu = f2i(floorf(u * a.checker_cells));
+            int cv = f2i(floorf(t * a.checker_cells));
             float k = (((uint32_t)cu + (uint32_t)cv) & 1u) ? 1.0f : 0.5f;
-            uint32_t r = (uint32_t)(clamp01(p.avar[0] * k) * 255.0f);
-            uint32_t g = (uint32_t)(clamp01(p.avar[1] * k) * 255.0f);
-            uint32_t b = (uint32_t)(clamp01(p.avar[2] * k) * 255.0f);
+            uint32_t r = (uint32_t)(clamp01(v[L::A + 0] * k) * 255.0f);
+            uint32_t g = (uint32_t)(clamp01(v[L::A + 1] * k) * 255.0f);
+            uint32_t b = (uint32_t)(clamp01(v[L::A + 2] * k) * 255.0f);
             s.color = r | (g << 8) | (b << 16) | 0xFF000000u;
             s.hit = true;
         }
@@ -98,7 +107,7 @@ struct ProgDepthChecker {
     }
 };
 
-template <int NA, int NP>
+template <class L>
 struct ProgHash {
     struct State { uint32_t h, n, n0; };
     __device__ static void load(State &s, const RasterArgs &a, size_t o)
@@ -106,24 +115,25 @@ struct ProgHash {
         s.h = static_cast<const uint32_t *>(a.t0)[o];
         s.n = s.n0 = static_cast<const uint32_t *>(a.t1)[o];
     }
-    template <class P>
-    __device__ static void shade(State &s, const RasterArgs &a, const P &p, uint32_t key, int, int)
+    __device__ static void shade(State &s, const RasterArgs &a, const float *v, uint32_t key, int,
+                                 int)
     {
+        const float w = 1.0f / v[L::IW];
         uint32_t h = mix32(s.h, key);
         if (a.iz)
-            h = mix32(h, fbits(p.z));
+            h = mix32(h, fbits(v[L::Z]));
         if (a.iw) {
-            h = mix32(h, fbits(p.w));
-            h = mix32(h, fbits(p.invw));
+            h = mix32(h, fbits(w));
+            h = mix32(h, fbits(v[L::IW]));
         }
 #pragma unroll
-        for (int k = 0; k < NA; ++k)
+        for (int k = 0; k < L::NA; ++k)
             if (k < a.na_set)
-                h = mix32(h, fbits(p.avar[k]));
+                h = mix32(h, fbits(v[L::A + k]));
 #pragma unroll
-        for (int k = 0; k < NP; ++k)
+        for (int k = 0; k < L::NP; ++k)
             if (k < a.np_set)
-                h = mix32(h, fbits(p.pvar[k]));
+                h = mix32(h, fbits(v[L::P + k] * w));
         s.h = h;
         s.n += 1;
     }
@@ -136,17 +146,17 @@ struct ProgHash {
     }
 };
 
-template <int NA, int NP>
+template <class L>
 struct ProgLog {
     struct State { uint32_t n, n0; };
     __device__ static void load(State &s, const RasterArgs &a, size_t o)
     {
         s.n = s.n0 = static_cast<const uint32_t *>(a.t0)[o];
     }
-    template <class P>
-    __device__ static void shade(State &s, const RasterArgs &a, const P &p, uint32_t key, int x,
+    __device__ static void shade(State &s, const RasterArgs &a, const float *v, uint32_t key, int x,
                                  int y)
     {
+        const float w = 1.0f / v[L::IW];
         uint32_t seq = s.n++;
         unsigned long long slot = atomicAdd(a.log_count, 1ull);
         if (slot < a.log_cap) {
@@ -155,13 +165,16 @@ struct ProgLog {
             r[1] = (uint32_t)y;
             r[2] = key;
             r[3] = seq;
-            r[4] = a.iz ? fbits(p.z) : 0u;
-            r[5] = a.iw ? fbits(p.w) : 0u;
-            r[6] = a.iw ? fbits(p.invw) : 0u;
+            r[4] = a.iz ? fbits(v[L::Z]) : 0u;
+            r[5] = a.iw ? fbits(w) : 0u;
+            r[6] = a.iw ? fbits(v[L::IW]) : 0u;
+#pragma unroll
             for (int k = 0; k < SWR_MAX_AVARS; ++k)
-                r[7 + k] = (k < NA && k < a.na_set) ? fbits(p.avar[k < NA ? k : 0]) : 0u;
+                r[7 + k] = (k < L::NA && k < a.na_set) ? fbits(v[L::A + (k < L::NA ? k : 0)]) : 0u;
+#pragma unroll
             for (int k = 0; k < SWR_MAX_PVARS; ++k)
-                r[7 + SWR_MAX_AVARS + k] = (k < NP && k < a.np_set) ? fbits(p.pvar[k < NP ? k : 0]) : 0u;
+                r[7 + SWR_MAX_AVARS + k] =
+                    (k < L::NP && k < a.np_set) ? fbits(v[L::P + (k < L::NP ? k : 0)] * w) : 0u;
         }
     }
     __device__ static void store(const State &s, const RasterArgs &a, size_t o)
@@ -172,15 +185,29 @@ struct ProgLog {
 };
 
 // ---- the tile kernel ----------------------------------------------------------------------
-__device__ inline bool edge_pass(float v, bool tie) { return v > 0.0f || (v == 0.0f && tie); }
+__device__ inline void lds_sync()
+{
+    // LDS operations of one wave complete in issue order; keep the compiler from moving
+    // them across this point and wait for the wave's own LDS traffic.
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+    __builtin_amdgcn_wave_barrier();
+    __asm__ volatile("" ::: "memory");
+}
 
-template <class Prog, int NZ, int NW, int NA, int NP>
+template <template <class> class ProgT, int NZ, int NW, int NA, int NP>
 __global__ void __launch_bounds__(256) k_raster(RasterArgs a)
 {
-    constexpr int N = NZ + NW + NA + NP;
+    using L = Layout<NZ, NW, NA, NP>;
+    using Prog = ProgT<L>;
+    constexpr int N = L::N;
     constexpr int NS = N > 0 ? N : 1;
-    const int lane = threadIdx.x & 63;
-    const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    constexpr int GS = N <= 8 ? 8 : (N <= 16 ? 4 : (N <= 32 ? 2 : 1));  // primitives per group
+    __shared__ float s_rows[4][GS * 64];
+    __shared__ unsigned long long s_cov[4][GS];
+
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * 4 + wv;
     if (tile >= a.tiles)
         return;
     if (a.status[ST_OVERFLOW] | a.status[ST_ERROR])
@@ -190,141 +217,152 @@ __global__ void __launch_bounds__(256) k_raster(RasterArgs a)
         return;
     const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
     const int bx = tx * TILE, by = ty * TILE;
-    const int li = lane & 7, lj = lane >> 3;
-    const int px = bx + li, py = by + lj;
+    // phase B: lane = pixel (pi, pj)
+    const int pi = lane & 7, pj = lane >> 3;
+    const int px = bx + pi, py = by + pj;
     const bool inside = px < a.width && py < a.height;
     const size_t o = (size_t)py * (size_t)a.pitch + (size_t)px;
+    // phase A: lane = (slot, row)
+    const int slot = lane >> 3, row = lane & 7;
+    const int ry = by + row;
+    const float ryf = (float)ry;
+    const float xf = (float)bx + 0.5f, yf = (float)by + 0.5f;
+    float *rows = s_rows[wv];
+    unsigned char *covb = reinterpret_cast<unsigned char *>(s_cov[wv]);
+
     typename Prog::State st;
     if (inside)
         Prog::load(st, a, o);
-    const float xf = (float)bx + 0.5f, yf = (float)by + 0.5f;
-    const float pyf = (float)py;
 
-    for (uint32_t e = beg; e < end; ++e) {
-        const uint32_t pid = __builtin_amdgcn_readfirstlane(a.vals[e]);
-        const float *__restrict__ rec = a.prims + (size_t)pid * (size_t)a.stride;
-        const uint32_t flags = __float_as_uint(rec[REC_FLAGS]);
-        const uint32_t key = __float_as_uint(rec[REC_KEY]);
-        float val[NS];
-        bool cov;
-        if (flags & FLAG_BLOCK) {
-            const float e0a = rec[REC_EDGE + 0], e0b = rec[REC_EDGE + 1], e0c = rec[REC_EDGE + 2];
-            const float e1a = rec[REC_EDGE + 3], e1b = rec[REC_EDGE + 4], e1c = rec[REC_EDGE + 5];
-            const float e2a = rec[REC_EDGE + 6], e2b = rec[REC_EDGE + 7], e2c = rec[REC_EDGE + 8];
-            const bool t0 = flags & FLAG_TIE0, t1 = flags & FLAG_TIE1, t2 = flags & FLAG_TIE2;
-            // corner classification, Rasterizer.h:270-304
-            const float s7 = 7.0f;
-            float c00 = e0a * xf + e0b * yf + e0c, d00 = e1a * xf + e1b * yf + e1c,
-                  f00 = e2a * xf + e2b * yf + e2c;
-            float c01 = c00 + e0b * s7, d01 = d00 + e1b * s7, f01 = f00 + e2b * s7;
-            float c10 = c00 + e0a * s7, d10 = d00 + e1a * s7, f10 = f00 + e2a * s7;
-            float c11 = c01 + e0a * s7, d11 = d01 + e1a * s7, f11 = f01 + e2a * s7;
-            bool p00[3] = {edge_pass(c00, t0), edge_pass(d00, t1), edge_pass(f00, t2)};
-            bool p01[3] = {edge_pass(c01, t0), edge_pass(d01, t1), edge_pass(f01, t2)};
-            bool p10[3] = {edge_pass(c10, t0), edge_pass(d10, t1), edge_pass(f10, t2)};
-            bool p11[3] = {edge_pass(c11, t0), edge_pass(d11, t1), edge_pass(f11, t2)};
-            int result = (p00[0] && p00[1] && p00[2]) + (p01[0] && p01[1] && p01[2]) +
-                         (p10[0] && p10[1] && p10[2]) + (p11[0] && p11[1] && p11[2]);
-            bool test_edges = true;
-            if (result == 0) {
-                // `a == b == c` is (a == b) == c: true iff an odd number of edges pass
-                bool same = ((p00[0] == p00[1]) == p00[2]) && ((p01[0] == p01[1]) == p01[2]) &&
-                            ((p10[0] == p10[1]) == p10[2]) && ((p11[0] == p11[1]) == p11[2]);
-                if (same)
-                    continue;
-            } else if (result == 4) {
-                test_edges = false;
-            }
-#pragma unroll
-            for (int p = 0; p < N; ++p) {
-                const float pa = rec[REC_PARAM + 3 * p], pb = rec[REC_PARAM + 3 * p + 1],
-                            pc = rec[REC_PARAM + 3 * p + 2];
-                float v = pa * xf + pb * yf + pc;
-                for (int k = 0; k < lj; ++k)
-                    v = v + pb;
-                for (int k = 0; k < li; ++k)
-                    v = v + pa;
-                val[p] = v;
-            }
-            cov = true;
-            if (test_edges) {
-                float v0 = c00, v1 = d00, v2 = f00;
-                for (int k = 0; k < lj; ++k) {
-                    v0 = v0 + e0b;
-                    v1 = v1 + e1b;
-                    v2 = v2 + e2b;
-                }
-                for (int k = 0; k < li; ++k) {
-                    v0 = v0 + e0a;
-                    v1 = v1 + e1a;
-                    v2 = v2 + e2a;
-                }
-                cov = edge_pass(v0, t0) && edge_pass(v1, t1) && edge_pass(v2, t2);
-            }
-        } else {
-            // Span: which flat part owns this scanline, and its [xl, xr)
+    for (uint32_t g0 = beg; g0 < end; g0 += GS) {
+        const int cnt = (int)((end - g0) < (uint32_t)GS ? (end - g0) : (uint32_t)GS);
+        const bool act = slot < cnt;
+        const uint32_t pv = act ? a.vals[g0 + slot] : 0u;
+        const float *__restrict__ rec = a.prims + (size_t)(pv & PAIR_PRIM_MASK) * (size_t)a.stride;
+        const int cls = (int)(pv >> 30);
+        const uint32_t flags = act ? __float_as_uint(rec[REC_FLAGS]) : 0u;
+        const uint32_t key = act ? __float_as_uint(rec[REC_KEY]) : 0u;
+        const bool blk = (flags & FLAG_BLOCK) != 0;
+
+        // Span: this row's part and [xl, xr) (Rasterizer.h:370-380, 397-407)
+        int xl = 0, xr = 0;
+        bool rowv = false;
+        if (act && !blk) {
             const int blo = (int)__float_as_uint(rec[REC_BOT + 4]);
             const int bhi = (int)__float_as_uint(rec[REC_BOT + 5]);
             const int tlo = (int)__float_as_uint(rec[REC_TOP + 4]);
             const int thi = (int)__float_as_uint(rec[REC_TOP + 5]);
-            const bool inb = (flags & FLAG_BOTTOM) && blo <= py && py < bhi;
-            const bool intop = (flags & FLAG_TOP) && tlo < py && py <= thi;
+            const bool inb = (flags & FLAG_BOTTOM) && blo <= ry && ry < bhi;
+            const bool intop = (flags & FLAG_TOP) && tlo < ry && ry <= thi;
             const int base = inb ? REC_BOT : REC_TOP;
-            const float ox = inb ? rec[REC_BOT + 0] : rec[REC_TOP + 0];
-            const float oy = inb ? rec[REC_BOT + 1] : rec[REC_TOP + 1];
-            const float is1 = inb ? rec[REC_BOT + 2] : rec[REC_TOP + 2];
-            const float is2 = inb ? rec[REC_BOT + 3] : rec[REC_TOP + 3];
-            (void)base;
-            float dy = (pyf - oy) + 0.5f;
-            float cx1 = ox + is1 * dy + 0.5f;
-            float cx2 = ox + is2 * dy + 0.5f;
-            int xl = imax(a.sc_minx, f2i(cx1));
-            int xr = imin(a.sc_maxx, f2i(cx2));
-            cov = (inb || intop) && xl <= px && px < xr;
-            const int steps = cov ? px - xl : 0;
-            const float xs = (float)xl + 0.5f, ys = pyf + 0.5f;
+            const float ox = rec[base + 0], oy = rec[base + 1];
+            const float is1 = rec[base + 2], is2 = rec[base + 3];
+            const float dy = (ryf - oy) + 0.5f;
+            const float cx1 = ox + is1 * dy + 0.5f;
+            const float cx2 = ox + is2 * dy + 0.5f;
+            xl = imax(a.sc_minx, f2i(cx1));
+            xr = imin(a.sc_maxx, f2i(cx2));
+            rowv = inb || intop;
+        }
+        const float xs = (float)xl + 0.5f, ys = ryf + 0.5f;
+        const int pre = bx - xl;  // Span: chain steps from xl to the tile's first column
+
+        // coverage of this lane's row (8 bits, bit i = column i)
+        uint32_t rowmask = 0;
+        if (act) {
+            if (blk) {
+                if (cls == 2) {
+                    rowmask = 0xFFu;
+                } else {
+                    rowmask = 0xFFu;
 #pragma unroll
-            for (int p = 0; p < N; ++p) {
+                    for (int e = 0; e < 3; ++e) {
+                        const float ea = rec[REC_EDGE + 3 * e], eb = rec[REC_EDGE + 3 * e + 1],
+                                    ec = rec[REC_EDGE + 3 * e + 2];
+                        const bool tie = (flags >> (1 + e)) & 1u;
+                        float v = ea * xf + eb * yf + ec;
+                        for (int k = 0; k < row; ++k)
+                            v = v + eb;
+                        uint32_t m = 0;
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) {
+                            m |= edge_pass(v, tie) ? (1u << i) : 0u;
+                            v = v + ea;
+                        }
+                        rowmask &= m;
+                    }
+                }
+            } else if (rowv) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    rowmask |= (xl <= bx + i && bx + i < xr) ? (1u << i) : 0u;
+            }
+            covb[slot * 8 + row] = (unsigned char)rowmask;
+        }
+
+        // chained values: phase A row walk, then transpose to pixel-per-lane
+        float val[GS][NS];
+#pragma unroll
+        for (int p = 0; p < N; ++p) {
+            float out[8];
+            if (act) {
                 const float pa = rec[REC_PARAM + 3 * p], pb = rec[REC_PARAM + 3 * p + 1],
                             pc = rec[REC_PARAM + 3 * p + 2];
-                float v = pa * xs + pb * ys + pc;
-                for (int k = 0; k < steps; ++k)
-                    v = v + pa;
-                val[p] = v;
+                if (blk) {
+                    float r0 = pa * xf + pb * yf + pc;
+                    for (int k = 0; k < row; ++k)
+                        r0 = r0 + pb;
+                    out[0] = r0;
+#pragma unroll
+                    for (int i = 1; i < 8; ++i)
+                        out[i] = out[i - 1] + pa;
+                } else {
+                    float v = pa * xs + pb * ys + pc;
+                    for (int k = 0; k < pre; ++k)
+                        v = v + pa;
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        out[i] = v;
+                        if (bx + i >= xl)
+                            v = v + pa;
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    rows[slot * 64 + row * 8 + i] = out[i];
+            }
+            lds_sync();
+#pragma unroll
+            for (int s2 = 0; s2 < GS; ++s2)
+                val[s2][p] = rows[s2 * 64 + pj * 8 + pi];
+            lds_sync();
+        }
+        if (N == 0)
+            lds_sync();
+
+        // phase B: shade the group's primitives in draw order
+#pragma unroll
+        for (int s2 = 0; s2 < GS; ++s2) {
+            if (s2 < cnt) {
+                const unsigned long long m = s_cov[wv][s2];
+                const uint32_t k2 = __builtin_amdgcn_readlane(key, s2 * 8);
+                if (((m >> lane) & 1ull) && inside)
+                    Prog::shade(st, a, val[s2], k2, px, py);
             }
         }
-        if (cov && inside) {
-            Pix<NA, NP> pix;
-            int q = 0;
-            pix.z = NZ ? val[q] : 0.0f;
-            q += NZ;
-            if (NW) {
-                pix.invw = val[q];
-                pix.w = 1.0f / pix.invw;
-            } else {
-                pix.invw = pix.w = 0.0f;
-            }
-            q += NW;
-#pragma unroll
-            for (int k = 0; k < NA; ++k)
-                pix.avar[k] = val[q + k];
-#pragma unroll
-            for (int k = 0; k < NP; ++k)
-                pix.pvar[k] = val[q + NA + k] * pix.w;
-            Prog::shade(st, a, pix, key, px, py);
-        }
+        lds_sync();
     }
     if (inside)
         Prog::store(st, a, o);
 }
 
-template <class Prog, int NZ, int NW, int NA, int NP>
+template <template <class> class ProgT, int NZ, int NW, int NA, int NP>
 static hipError_t launch(const RasterArgs &a, hipStream_t s)
 {
     uint32_t blocks = (uint32_t)((a.tiles + 3) / 4);
     if (!blocks)
         return hipSuccess;
-    hipLaunchKernelGGL((k_raster<Prog, NZ, NW, NA, NP>), dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_raster<ProgT, NZ, NW, NA, NP>), dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -381,11 +419,9 @@ hipError_t launch_raster(const swr_pixel_shader_desc &ps, const ParamLayout &L,
     case SWR_PS_DEPTH_CHECKER:
         return launch<ProgDepthChecker, 1, 1, 3, 2>(a, s);
     case SWR_PS_HASH:
-        return L.na == 4 ? launch<ProgHash<4, 4>, 1, 1, 4, 4>(a, s)
-                         : launch<ProgHash<16, 16>, 1, 1, 16, 16>(a, s);
+        return L.na == 4 ? launch<ProgHash, 1, 1, 4, 4>(a, s) : launch<ProgHash, 1, 1, 16, 16>(a, s);
     case SWR_PS_LOG:
-        return L.na == 4 ? launch<ProgLog<4, 4>, 1, 1, 4, 4>(a, s)
-                         : launch<ProgLog<16, 16>, 1, 1, 16, 16>(a, s);
+        return L.na == 4 ? launch<ProgLog, 1, 1, 4, 4>(a, s) : launch<ProgLog, 1, 1, 16, 16>(a, s);
     default:
         return hipSuccess;
     }

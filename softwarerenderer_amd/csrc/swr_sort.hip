@@ -221,14 +221,17 @@ __global__ void __launch_bounds__(RADIX_THREADS) k_radix_scatter(
 }
 
 // Per-tile [begin, end) into the sorted pairs; tiles without pairs keep the zeros the
-// begin kernel wrote.
+// begin kernel wrote, and rejected blocks (sentinel keys, sorted last) are skipped.
 __global__ void __launch_bounds__(256) k_ranges(const uint32_t *keys,
-                                              const unsigned long long *status, uint32_t *ranges)
+                                              const unsigned long long *status, uint32_t *ranges,
+                                              uint32_t tiles)
 {
     uint64_t m = (status[ST_OVERFLOW] | status[ST_ERROR]) ? 0 : status[ST_PAIRS];
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
          i += (uint64_t)gridDim.x * blockDim.x) {
         uint32_t k = keys[i];
+        if (k >= tiles)
+            continue;
         if (i == 0 || keys[i - 1] != k)
             ranges[2 * (size_t)k] = (uint32_t)i;
         if (i == m - 1 || keys[i + 1] != k)
@@ -244,10 +247,18 @@ int radix_passes(uint32_t tiles)
     return bits == 0 ? 1 : (bits + 7) / 8;
 }
 
+// The sort covers every bit a tile id or the sentinel can set: tile ids < tiles, sentinel
+// = all ones in the sorted digits, which is > tiles.
+uint32_t pair_sentinel(uint32_t tiles)
+{
+    int passes = radix_passes(tiles + 2);
+    return passes >= 4 ? 0xFFFFFFFFu : (uint32_t)((1ull << (8 * passes)) - 1);
+}
+
 hipError_t sort_pairs(SortBuffers &sb, const unsigned long long *status, uint32_t tiles,
                       hipStream_t s)
 {
-    int passes = radix_passes(tiles);
+    int passes = radix_passes(tiles + 2);
     uint32_t *ka = sb.keys_a, *va = sb.vals_a, *kb = sb.keys_b, *vb = sb.vals_b;
     for (int p = 0; p < passes; ++p) {
         int shift = 8 * p;
@@ -270,9 +281,11 @@ hipError_t sort_pairs(SortBuffers &sb, const unsigned long long *status, uint32_
     return hipGetLastError();
 }
 
-hipError_t launch_ranges(const SortBuffers &sb, const unsigned long long *status, hipStream_t s)
+hipError_t launch_ranges(const SortBuffers &sb, const unsigned long long *status,
+                         uint32_t tiles, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_ranges, dim3(1024), dim3(256), 0, s, sb.sorted_keys, status, sb.ranges);
+    hipLaunchKernelGGL(k_ranges, dim3(1024), dim3(256), 0, s, sb.sorted_keys, status, sb.ranges,
+                       tiles);
     return hipGetLastError();
 }
 

@@ -236,17 +236,19 @@ def benchmark_scene(ntri: int = 4096 * 10, width: int = 640, height: int = 480) 
 
 
 def clip_scene(ntri: int = 2000, seed: int = 3, width: int = 256, height: int = 192,
-               navar: int = 3, npvar: int = 2, spread: float = 1.6) -> Scene:
+               navar: int = 3, npvar: int = 2, spread: float = 1.6,
+               shrink=(0.05, 1.0)) -> Scene:
     """Clip-space triangles {x,y,z,w, avar[navar], pvar[npvar]} for the raw vertex program.
-    Vertices are spread up to `spread` times the frustum so most triangles cross planes."""
+    Vertices are spread up to `spread` times the frustum so most triangles cross planes;
+    each triangle is scaled about its first vertex by a factor drawn from `shrink`."""
     rng = np.random.default_rng(seed)
     n = 3 * ntri
     w = rng.uniform(0.2, 2.0, n)
     xyz = rng.uniform(-spread, spread, (n, 3)) * w[:, None]
     # keep triangles reasonably sized: pull each triangle's vertices toward its first vertex
     xyz = xyz.reshape(ntri, 3, 3)
-    shrink = rng.uniform(0.05, 1.0, (ntri, 1, 1))
-    xyz = xyz[:, :1, :] + (xyz - xyz[:, :1, :]) * shrink
+    f = rng.uniform(shrink[0], shrink[1], (ntri, 1, 1))
+    xyz = xyz[:, :1, :] + (xyz - xyz[:, :1, :]) * f
     xyz = xyz.reshape(n, 3)
     var = rng.random((n, navar + npvar))
     verts = np.concatenate([xyz, w[:, None], var], axis=1).astype(np.float32)
