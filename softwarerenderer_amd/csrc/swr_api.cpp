@@ -272,7 +272,7 @@ int check_status(swr_context *c)
 void collect_stats(swr_context *c)
 {
     c->stats.raster_primitives = c->status_host[ST_PRIMS];
-    c->stats.tile_pairs = c->status_host[ST_PAIRS];
+    c->stats.tile_pairs = c->status_host[ST_KEPT];
     if (c->profiling) {
         for (int i = 0; i < S_COUNT; ++i) {
             float ms = 0.0f;

@@ -47,6 +47,7 @@ constexpr int ST_PAIRS = 1;        // (primitive, tile) pairs emitted
 constexpr int ST_OVERFLOW = 2;     // bit 0: prim arena, bit 1: pair arena
 constexpr int ST_ERROR = 3;        // bit 0: index range, bit 1: clip overflow
 constexpr int ST_BIG = 4;          // primitives queued for cooperative pair expansion
+constexpr int ST_KEPT = 5;         // pairs left after dropping rejected blocks / tiles
 constexpr int ST_WORDS = 8;
 
 // record word 6: index of the primitive's first (tile, primitive) pair
@@ -109,6 +110,51 @@ __device__ inline int classify_block(const float *e, uint32_t flags, int x, int 
     if (result == 0)
         return same ? 0 : 1;
     return result == 4 ? 2 : 1;
+}
+
+// True unless some edge is negative at every pixel centre of the block by more than the
+// rounding the per-pixel chain can accumulate, i.e. unless no pixel of drawBlock<true>
+// (PixelShaderBase.h:55-94) can pass all three edge tests.  The chained value of pixel
+// (i, j) is init = (a*xf + b*yf) + c followed by j additions of b and i of a: at most 18
+// roundings, each at most 2^-24 of a partial sum bounded by M = |a xf| + |b yf| + |c| +
+// 7|a| + 7|b|.  A block is dropped only when an edge's exact maximum over the block (a
+// corner: the function is linear) is below -2^-19 M, more than twice that error bound.
+__device__ inline bool block_may_cover(const float *e, int x, int y)
+{
+    const double xf = (double)x + 0.5, yf = (double)y + 0.5;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double a = e[3 * k], b = e[3 * k + 1], c = e[3 * k + 2];
+        if (!__builtin_isfinite(a) || !__builtin_isfinite(b) || !__builtin_isfinite(c))
+            return true;
+        const double m = fabs(a * xf) + fabs(b * yf) + fabs(c) + 7.0 * (fabs(a) + fabs(b));
+        const double mx = a * xf + b * yf + c + fmax(0.0, 7.0 * a) + fmax(0.0, 7.0 * b);
+        if (mx < -m * 1.9073486328125e-06)  // 2^-19
+            return false;
+    }
+    return true;
+}
+
+// Span: the flat part owning scanline y and its pixel range [xl, xr)
+// (Rasterizer.h:370-380 bottom-flat, :397-407 top-flat).  Returns false when no part owns y.
+__device__ inline bool span_row_extent(const float *rec, uint32_t flags, int y, int sc_minx,
+                                       int sc_maxx, int &xl, int &xr)
+{
+    const int blo = (int)__float_as_uint(rec[REC_BOT + 4]);
+    const int bhi = (int)__float_as_uint(rec[REC_BOT + 5]);
+    const int tlo = (int)__float_as_uint(rec[REC_TOP + 4]);
+    const int thi = (int)__float_as_uint(rec[REC_TOP + 5]);
+    const bool inb = (flags & FLAG_BOTTOM) && blo <= y && y < bhi;
+    const bool intop = (flags & FLAG_TOP) && tlo < y && y <= thi;
+    const int base = inb ? REC_BOT : REC_TOP;
+    const float ox = rec[base + 0], oy = rec[base + 1];
+    const float is1 = rec[base + 2], is2 = rec[base + 3];
+    const float dy = ((float)y - oy) + 0.5f;
+    const float cx1 = ox + is1 * dy + 0.5f;
+    const float cx2 = ox + is2 * dy + 0.5f;
+    xl = imax(sc_minx, f2i(cx1));
+    xr = imin(sc_maxx, f2i(cx2));
+    return inb || intop;
 }
 
 // pair value: primitive index | classification << 30 (Block primitives)

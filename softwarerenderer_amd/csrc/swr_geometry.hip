@@ -353,6 +353,33 @@ __device__ inline bool prim_geometry(const GeomArgs &a, const WinV &v0, const Wi
     return g.tx0 <= g.tx1 && g.ty0 <= g.ty1;
 }
 
+// The (key, value) pair of one candidate tile.  Tiles in which the primitive cannot shade a
+// pixel get the sentinel key and are dropped before the sort:
+//   Block: the reference's corner classification (0 = skip), then block_may_cover;
+//   Span: exact, the tile's eight scanline ranges are those the rasterizer computes.
+__device__ inline void pair_for_tile(const GeomArgs &a, const float *rec, uint32_t flags,
+                                     uint32_t pos, int tx, int ty, uint32_t &key, uint32_t &val)
+{
+    key = (uint32_t)(ty * a.tiles_x + tx);
+    val = pos;
+    const int bx = tx * TILE, by = ty * TILE;
+    if (flags & FLAG_BLOCK) {
+        int cls = classify_block(rec + REC_EDGE, flags, bx, by);
+        if (cls == 1 && !block_may_cover(rec + REC_EDGE, bx, by))
+            cls = 0;
+        key = cls ? key : a.sentinel;
+        val |= (uint32_t)cls << 30;
+    } else {
+        bool any = false;
+        for (int r = 0; r < TILE && !any; ++r) {
+            int xl, xr;
+            bool rowv = span_row_extent(rec, flags, by + r, a.sc_minx, a.sc_maxx, xl, xr);
+            any = rowv && imax(xl, bx) < imin(xr, bx + TILE);
+        }
+        key = any ? key : a.sentinel;
+    }
+}
+
 // ParameterEquation::init, ParameterEquation.h:36-48
 __device__ inline void param_write(float *rec, int slot, float p0, float p1, float p2,
                                    const PrimGeo &g, float f)
@@ -434,12 +461,8 @@ __device__ void write_prim(const GeomArgs &a, uint64_t pos, uint32_t key, const 
     uint64_t n = 0;
     for (int ty = g.ty0; ty <= g.ty1; ++ty)
         for (int tx = g.tx0; tx <= g.tx1; ++tx) {
-            uint32_t key = (uint32_t)(ty * a.tiles_x + tx), val = (uint32_t)pos;
-            if (g.flags & FLAG_BLOCK) {
-                int cls = classify_block(rec + REC_EDGE, g.flags, tx * TILE, ty * TILE);
-                key = cls ? key : a.sentinel;
-                val |= (uint32_t)cls << 30;
-            }
+            uint32_t key, val;
+            pair_for_tile(a, rec, g.flags, (uint32_t)pos, tx, ty, key, val);
             a.pair_keys[pair_base + n] = key;
             a.pair_vals[pair_base + n] = val;
             ++n;
@@ -462,17 +485,10 @@ __global__ void __launch_bounds__(256) k_expand_big(GeomArgs a)
         const uint32_t w = (uint32_t)(tx1 - tx0 + 1);
         const uint64_t nt = (uint64_t)w * (uint64_t)(ty1 - ty0 + 1);
         const uint64_t pb = ru[REC_PAIRBASE];
-        float e[9];
-        for (int k = 0; k < 9; ++k)
-            e[k] = rec[REC_EDGE + k];
         for (uint64_t k = threadIdx.x; k < nt; k += blockDim.x) {
             int tx = tx0 + (int)(k % w), ty = ty0 + (int)(k / w);
-            uint32_t key = (uint32_t)(ty * a.tiles_x + tx), val = pos;
-            if (flags & FLAG_BLOCK) {
-                int cls = classify_block(e, flags, tx * TILE, ty * TILE);
-                key = cls ? key : a.sentinel;
-                val |= (uint32_t)cls << 30;
-            }
+            uint32_t key, val;
+            pair_for_tile(a, rec, flags, pos, tx, ty, key, val);
             a.pair_keys[pb + k] = key;
             a.pair_vals[pb + k] = val;
         }

@@ -247,23 +247,8 @@ __global__ void __launch_bounds__(256) k_raster(RasterArgs a)
         // Span: this row's part and [xl, xr) (Rasterizer.h:370-380, 397-407)
         int xl = 0, xr = 0;
         bool rowv = false;
-        if (act && !blk) {
-            const int blo = (int)__float_as_uint(rec[REC_BOT + 4]);
-            const int bhi = (int)__float_as_uint(rec[REC_BOT + 5]);
-            const int tlo = (int)__float_as_uint(rec[REC_TOP + 4]);
-            const int thi = (int)__float_as_uint(rec[REC_TOP + 5]);
-            const bool inb = (flags & FLAG_BOTTOM) && blo <= ry && ry < bhi;
-            const bool intop = (flags & FLAG_TOP) && tlo < ry && ry <= thi;
-            const int base = inb ? REC_BOT : REC_TOP;
-            const float ox = rec[base + 0], oy = rec[base + 1];
-            const float is1 = rec[base + 2], is2 = rec[base + 3];
-            const float dy = (ryf - oy) + 0.5f;
-            const float cx1 = ox + is1 * dy + 0.5f;
-            const float cx2 = ox + is2 * dy + 0.5f;
-            xl = imax(a.sc_minx, f2i(cx1));
-            xr = imin(a.sc_maxx, f2i(cx2));
-            rowv = inb || intop;
-        }
+        if (act && !blk)
+            rowv = span_row_extent(rec, flags, ry, a.sc_minx, a.sc_maxx, xl, xr);
         const float xs = (float)xl + 0.5f, ys = ryf + 0.5f;
         const int pre = bx - xl;  // Span: chain steps from xl to the tile's first column
 

@@ -145,10 +145,17 @@ hipError_t scan_pairs(const uint2 *in, uint2 *out, uint64_t n, uint2 *tmp,
 // ---- radix sort of (tile, primitive) pairs ---------------------------------------------
 constexpr int RADIX_THREADS = 256;
 
-__device__ inline void chunk_of(const unsigned long long *status, uint32_t G, uint32_t b,
+// Pass 0 reads all emitted pairs (ST_PAIRS) and drops the rejected ones (keys >= tiles);
+// later passes and the ranges read the kept ones (ST_KEPT).
+__device__ inline uint64_t pair_count(const unsigned long long *status, int word)
+{
+    return (status[ST_OVERFLOW] | status[ST_ERROR]) ? 0 : status[word];
+}
+
+__device__ inline void chunk_of(const unsigned long long *status, int word, uint32_t G, uint32_t b,
                                 uint64_t &beg, uint64_t &end)
 {
-    uint64_t m = (status[ST_OVERFLOW] | status[ST_ERROR]) ? 0 : status[ST_PAIRS];
+    uint64_t m = pair_count(status, word);
     uint64_t ch = (m + G - 1) / G;
     beg = (uint64_t)b * ch;
     end = beg + ch;
@@ -158,23 +165,35 @@ __device__ inline void chunk_of(const unsigned long long *status, uint32_t G, ui
 
 __global__ void __launch_bounds__(RADIX_THREADS) k_radix_hist(const uint32_t *keys,
                                                             const unsigned long long *status,
-                                                            int shift, uint32_t *hist)
+                                                            int word, int shift, uint32_t tiles,
+                                                            uint32_t *hist)
 {
     __shared__ uint32_t h[256];
     const uint32_t G = gridDim.x, b = blockIdx.x;
     h[threadIdx.x] = 0;
     __syncthreads();
     uint64_t beg, end;
-    chunk_of(status, G, b, beg, end);
-    for (uint64_t i = beg + threadIdx.x; i < end; i += RADIX_THREADS)
-        atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    chunk_of(status, word, G, b, beg, end);
+    for (uint64_t i = beg + threadIdx.x; i < end; i += RADIX_THREADS) {
+        uint32_t k = keys[i];
+        if (k < tiles)
+            atomicAdd(&h[(k >> shift) & 255u], 1u);
+    }
     __syncthreads();
     hist[(size_t)threadIdx.x * G + b] = h[threadIdx.x];
 }
 
+// Number of kept pairs: the exclusive scan of the pass-0 histogram plus its last count.
+__global__ void k_kept_total(const uint32_t *hist, const uint32_t *hist_ex, uint32_t n,
+                             unsigned long long *status)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        status[ST_KEPT] = (unsigned long long)hist_ex[n - 1] + hist[n - 1];
+}
+
 __global__ void __launch_bounds__(RADIX_THREADS) k_radix_scatter(
     const uint32_t *keys, const uint32_t *vals, uint32_t *okeys, uint32_t *ovals,
-    const unsigned long long *status, int shift, const uint32_t *hist_ex)
+    const unsigned long long *status, int word, int shift, uint32_t tiles, const uint32_t *hist_ex)
 {
     __shared__ uint32_t run[256];
     __shared__ uint32_t wcnt[4][256];
@@ -185,13 +204,13 @@ __global__ void __launch_bounds__(RADIX_THREADS) k_radix_scatter(
         wcnt[q][tid] = 0;
     __syncthreads();
     uint64_t beg, end;
-    chunk_of(status, G, b, beg, end);
-    const uint64_t m_total = (status[ST_OVERFLOW] | status[ST_ERROR]) ? 0 : status[ST_PAIRS];
+    chunk_of(status, word, G, b, beg, end);
+    const uint64_t cap = pair_count(status, ST_PAIRS);
     const unsigned long long below = (1ull << lane) - 1ull;
     for (uint64_t base = beg; base < end; base += RADIX_THREADS) {
         uint64_t i = base + tid;
-        bool valid = i < end;
-        uint32_t k = valid ? keys[i] : 0u;
+        uint32_t k = i < end ? keys[i] : 0u;
+        bool valid = i < end && k < tiles;
         uint32_t v = valid ? vals[i] : 0u;
         uint32_t d = (k >> shift) & 255u;
         unsigned long long m = __ballot(valid);
@@ -207,7 +226,7 @@ __global__ void __launch_bounds__(RADIX_THREADS) k_radix_scatter(
             uint32_t pos = run[d] + rank;
             for (int q = 0; q < w; ++q)
                 pos += wcnt[q][d];
-            if (pos < m_total) {
+            if (pos < cap) {
                 okeys[pos] = k;
                 ovals[pos] = v;
             }
@@ -220,18 +239,15 @@ __global__ void __launch_bounds__(RADIX_THREADS) k_radix_scatter(
     }
 }
 
-// Per-tile [begin, end) into the sorted pairs; tiles without pairs keep the zeros the
-// begin kernel wrote, and rejected blocks (sentinel keys, sorted last) are skipped.
+// Per-tile [begin, end) into the sorted kept pairs; tiles without pairs keep the zeros the
+// begin kernel wrote.
 __global__ void __launch_bounds__(256) k_ranges(const uint32_t *keys,
-                                              const unsigned long long *status, uint32_t *ranges,
-                                              uint32_t tiles)
+                                              const unsigned long long *status, uint32_t *ranges)
 {
-    uint64_t m = (status[ST_OVERFLOW] | status[ST_ERROR]) ? 0 : status[ST_PAIRS];
+    uint64_t m = pair_count(status, ST_KEPT);
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
          i += (uint64_t)gridDim.x * blockDim.x) {
         uint32_t k = keys[i];
-        if (k >= tiles)
-            continue;
         if (i == 0 || keys[i - 1] != k)
             ranges[2 * (size_t)k] = (uint32_t)i;
         if (i == m - 1 || keys[i + 1] != k)
@@ -247,28 +263,28 @@ int radix_passes(uint32_t tiles)
     return bits == 0 ? 1 : (bits + 7) / 8;
 }
 
-// The sort covers every bit a tile id or the sentinel can set: tile ids < tiles, sentinel
-// = all ones in the sorted digits, which is > tiles.
-uint32_t pair_sentinel(uint32_t tiles)
-{
-    int passes = radix_passes(tiles + 2);
-    return passes >= 4 ? 0xFFFFFFFFu : (uint32_t)((1ull << (8 * passes)) - 1);
-}
+// Rejected pairs carry a key >= tiles; every key below `tiles` sorts within the passes.
+uint32_t pair_sentinel(uint32_t) { return 0xFFFFFFFFu; }
 
 hipError_t sort_pairs(SortBuffers &sb, const unsigned long long *status, uint32_t tiles,
                       hipStream_t s)
 {
-    int passes = radix_passes(tiles + 2);
+    int passes = radix_passes(tiles);
     uint32_t *ka = sb.keys_a, *va = sb.vals_a, *kb = sb.keys_b, *vb = sb.vals_b;
+    const uint32_t nh = 256 * sb.blocks;
     for (int p = 0; p < passes; ++p) {
-        int shift = 8 * p;
+        const int shift = 8 * p;
+        const int word = p == 0 ? ST_PAIRS : ST_KEPT;
         hipLaunchKernelGGL(k_radix_hist, dim3(sb.blocks), dim3(RADIX_THREADS), 0, s, ka, status,
-                           shift, sb.hist);
-        hipError_t e = scan_u32(sb.hist, sb.hist_ex, (uint64_t)256 * sb.blocks, sb.scan_tmp, s);
+                           word, shift, tiles, sb.hist);
+        hipError_t e = scan_u32(sb.hist, sb.hist_ex, nh, sb.scan_tmp, s);
         if (e != hipSuccess)
             return e;
+        if (p == 0)
+            hipLaunchKernelGGL(k_kept_total, dim3(1), dim3(64), 0, s, sb.hist, sb.hist_ex, nh,
+                               (unsigned long long *)status);
         hipLaunchKernelGGL(k_radix_scatter, dim3(sb.blocks), dim3(RADIX_THREADS), 0, s, ka, va, kb,
-                           vb, status, shift, sb.hist_ex);
+                           vb, status, word, shift, tiles, sb.hist_ex);
         uint32_t *t = ka;
         ka = kb;
         kb = t;
@@ -284,8 +300,8 @@ hipError_t sort_pairs(SortBuffers &sb, const unsigned long long *status, uint32_
 hipError_t launch_ranges(const SortBuffers &sb, const unsigned long long *status,
                          uint32_t tiles, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_ranges, dim3(1024), dim3(256), 0, s, sb.sorted_keys, status, sb.ranges,
-                       tiles);
+    (void)tiles;
+    hipLaunchKernelGGL(k_ranges, dim3(1024), dim3(256), 0, s, sb.sorted_keys, status, sb.ranges);
     return hipGetLastError();
 }
 
