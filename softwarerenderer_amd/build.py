@@ -56,5 +56,19 @@ def build_library(verbose: bool = False) -> str:
     return LIB
 
 
+def build_cpp_example() -> str:
+    """tests/cpp/render_scene: the C++ host API (include/swr/Renderer.h) driving the library."""
+    src = os.path.join(ROOT, "tests", "cpp", "render_scene.cpp")
+    exe = os.path.join(ROOT, "tests", "cpp", "render_scene")
+    r = subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"),
+                        src, "-L" + HERE, "-lswr_hip",
+                        "-Wl,-rpath,$ORIGIN/../../softwarerenderer_amd", "-o", exe],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("render_scene build failed:\n" + r.stderr)
+    return exe
+
+
 if __name__ == "__main__":
     build_library(verbose=True)
+    build_cpp_example()

@@ -61,7 +61,7 @@ struct swr_context {
 
     // arenas
     DevBuf idx, attr0, wverts, app, app_ex, grp, grp_ex, scan_tmp, prims, pk_a, pv_a, pk_b, pv_b,
-        ranges, hist, hist_ex, hist_tmp, status, tgt[3], log, logcount, bigq;
+        ranges, hist, hist_ex, hist_tmp, status, tgt[3], log, logcount, bigq;  // bigq: per-primitive first pair
     uint64_t prim_cap = 0, pair_cap = 0;
     unsigned long long *status_host = nullptr;
 
@@ -433,7 +433,7 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
         g.ranges = (uint32_t *)c->ranges.p;
         g.tiles = tiles;
         g.sentinel = pair_sentinel(tiles);
-        g.bigq = (uint32_t *)c->bigq.p;
+        g.prim_pb = (uint32_t *)c->bigq.p;
 
         record(c, 0);
         HIP_TRY(c, launch_begin(g, c->stream));
@@ -446,7 +446,7 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
                               c->prim_cap, c->pair_cap, c->stream));
         record(c, 2);
         HIP_TRY(c, launch_geom_emit(g, c->stream));
-        HIP_TRY(c, launch_expand_big(g, c->stream));
+        HIP_TRY(c, launch_expand(g, c->stream));
         record(c, 3);
         SortBuffers sb;
         sb.keys_a = (uint32_t *)c->pk_a.p;

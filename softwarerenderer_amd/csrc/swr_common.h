@@ -52,8 +52,6 @@ constexpr int ST_WORDS = 8;
 
 // record word 6: index of the primitive's first (tile, primitive) pair
 constexpr int REC_PAIRBASE = 6;
-// primitives with more candidate tiles than this are expanded by a whole workgroup
-constexpr uint32_t BIG_PRIM_TILES = 16;
 
 // Which plane equations the rasterizer interpolates (chosen by the pixel program).
 struct ParamLayout {
@@ -196,8 +194,8 @@ struct GeomArgs {
     unsigned long long *status;
     uint32_t *ranges;           // [tiles*2] cleared by the count pass
     uint32_t tiles;
-    uint32_t sentinel;          // pair key of a block the corner test rejects (sorts last)
-    uint32_t *bigq;             // primitives whose pairs the expansion kernel writes
+    uint32_t sentinel;          // pair key of a tile the primitive cannot shade (dropped)
+    uint32_t *prim_pb;          // [prim_cap] index of each primitive's first pair
 };
 
 struct RasterArgs {

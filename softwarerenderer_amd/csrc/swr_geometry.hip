@@ -450,48 +450,68 @@ __device__ void write_prim(const GeomArgs &a, uint64_t pos, uint32_t key, const 
         }
     }
     ru[REC_PAIRBASE] = (uint32_t)pair_base;
-    // (tile, primitive) pairs in row-major tile order; large primitives are expanded by
-    // k_expand_big with a whole workgroup
-    const uint64_t nt = (uint64_t)(g.tx1 - g.tx0 + 1) * (uint64_t)(g.ty1 - g.ty0 + 1);
-    if (nt > BIG_PRIM_TILES) {
-        unsigned long long q = atomicAdd(&a.status[ST_BIG], 1ull);
-        a.bigq[q] = (uint32_t)pos;
-        return;
-    }
-    uint64_t n = 0;
-    for (int ty = g.ty0; ty <= g.ty1; ++ty)
-        for (int tx = g.tx0; tx <= g.tx1; ++tx) {
-            uint32_t key, val;
-            pair_for_tile(a, rec, g.flags, (uint32_t)pos, tx, ty, key, val);
-            a.pair_keys[pair_base + n] = key;
-            a.pair_vals[pair_base + n] = val;
-            ++n;
-        }
+    a.prim_pb[pos] = (uint32_t)pair_base;  // the expansion kernel searches these
 }
 
-// Pairs of the queued large primitives: one workgroup per primitive, threads over tiles.
-__global__ void __launch_bounds__(256) k_expand_big(GeomArgs a)
+// (tile, primitive) pairs, load-balanced: each workgroup owns EXPAND_CHUNK consecutive pair
+// slots.  Pair bases are increasing in draw order, so the primitives owning a chunk are a
+// contiguous run found by one binary search; each thread then searches that run (in LDS)
+// for the primitive of its slot and writes the slot's tile in row-major rect order.
+constexpr int EXPAND_CHUNK = 1024;
+
+__global__ void __launch_bounds__(256) k_expand(GeomArgs a)
 {
+    __shared__ uint32_t s_pb[EXPAND_CHUNK + 1];
+    __shared__ uint32_t s_p0, s_np;
     if (a.status[ST_OVERFLOW] | a.status[ST_ERROR])
         return;
-    const uint64_t nbig = a.status[ST_BIG];
-    for (uint64_t q = blockIdx.x; q < nbig; q += gridDim.x) {
-        const uint32_t pos = a.bigq[q];
+    const uint64_t M = a.status[ST_PAIRS], P = a.status[ST_PRIMS];
+    const uint64_t c0 = (uint64_t)blockIdx.x * EXPAND_CHUNK;
+    if (c0 >= M)
+        return;
+    const uint64_t c1 = c0 + EXPAND_CHUNK < M ? c0 + EXPAND_CHUNK : M;
+    if (threadIdx.x == 0) {
+        // last primitive whose first pair is <= target
+        auto owner = [&](uint64_t target) {
+            uint64_t lo = 0, hi = P;  // invariant: pb[lo] <= target, answer in [lo, hi)
+            while (hi - lo > 1) {
+                uint64_t mid = (lo + hi) >> 1;
+                if (a.prim_pb[mid] <= target)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            return lo;
+        };
+        uint64_t p0 = owner(c0), p1 = owner(c1 - 1);
+        s_p0 = (uint32_t)p0;
+        s_np = (uint32_t)(p1 - p0 + 1);
+    }
+    __syncthreads();
+    const uint32_t p0 = s_p0, np = s_np;
+    for (uint32_t i = threadIdx.x; i <= np; i += blockDim.x)
+        s_pb[i] = (p0 + i < P) ? a.prim_pb[p0 + i] : (uint32_t)M;
+    __syncthreads();
+    for (uint64_t q = c0 + threadIdx.x; q < c1; q += blockDim.x) {
+        uint32_t lo = 0, hi = np;
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (s_pb[mid] <= (uint32_t)q)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const uint32_t pos = p0 + lo;
+        const uint32_t local = (uint32_t)q - s_pb[lo];
         const float *rec = a.prims + (uint64_t)pos * (uint64_t)a.lay.stride;
         const uint32_t *ru = reinterpret_cast<const uint32_t *>(rec);
-        const uint32_t flags = ru[REC_FLAGS];
         const int tx0 = (int)ru[REC_TILES + 0], ty0 = (int)ru[REC_TILES + 1];
-        const int tx1 = (int)ru[REC_TILES + 2], ty1 = (int)ru[REC_TILES + 3];
-        const uint32_t w = (uint32_t)(tx1 - tx0 + 1);
-        const uint64_t nt = (uint64_t)w * (uint64_t)(ty1 - ty0 + 1);
-        const uint64_t pb = ru[REC_PAIRBASE];
-        for (uint64_t k = threadIdx.x; k < nt; k += blockDim.x) {
-            int tx = tx0 + (int)(k % w), ty = ty0 + (int)(k / w);
-            uint32_t key, val;
-            pair_for_tile(a, rec, flags, pos, tx, ty, key, val);
-            a.pair_keys[pb + k] = key;
-            a.pair_vals[pb + k] = val;
-        }
+        const uint32_t w = ru[REC_TILES + 2] - ru[REC_TILES + 0] + 1u;
+        const int tx = tx0 + (int)(local % w), ty = ty0 + (int)(local / w);
+        uint32_t key, val;
+        pair_for_tile(a, rec, ru[REC_FLAGS], pos, tx, ty, key, val);
+        a.pair_keys[q] = key;
+        a.pair_vals[q] = val;
     }
 }
 
@@ -773,9 +793,12 @@ hipError_t launch_geom_count(const GeomArgs &a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_expand_big(const GeomArgs &a, hipStream_t s)
+hipError_t launch_expand(const GeomArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_expand_big, dim3(1024), dim3(256), 0, s, a);
+    uint64_t blocks = (a.pair_cap + EXPAND_CHUNK - 1) / EXPAND_CHUNK;
+    if (blocks < 1)
+        blocks = 1;
+    hipLaunchKernelGGL(k_expand, dim3((uint32_t)blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

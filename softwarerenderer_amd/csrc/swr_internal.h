@@ -17,7 +17,7 @@ struct SortBuffers {
 hipError_t launch_begin(const GeomArgs &a, hipStream_t s);
 hipError_t launch_geom_count(const GeomArgs &a, hipStream_t s);
 hipError_t launch_geom_emit(const GeomArgs &a, hipStream_t s);
-hipError_t launch_expand_big(const GeomArgs &a, hipStream_t s);
+hipError_t launch_expand(const GeomArgs &a, hipStream_t s);
 
 // scans and sort (swr_sort.hip)
 uint64_t scan_tmp_elems(uint64_t n);

@@ -265,9 +265,12 @@ __global__ void __launch_bounds__(256) k_raster(RasterArgs a)
                         const float ea = rec[REC_EDGE + 3 * e], eb = rec[REC_EDGE + 3 * e + 1],
                                     ec = rec[REC_EDGE + 3 * e + 2];
                         const bool tie = (flags >> (1 + e)) & 1u;
-                        float v = ea * xf + eb * yf + ec;
-                        for (int k = 0; k < row; ++k)
-                            v = v + eb;
+                        float r = ea * xf + eb * yf + ec, v = r;
+#pragma unroll
+                        for (int k = 1; k < 8; ++k) {
+                            r = r + eb;
+                            v = k <= row ? r : v;
+                        }
                         uint32_t m = 0;
 #pragma unroll
                         for (int i = 0; i < 8; ++i) {
@@ -294,9 +297,13 @@ __global__ void __launch_bounds__(256) k_raster(RasterArgs a)
                 const float pa = rec[REC_PARAM + 3 * p], pb = rec[REC_PARAM + 3 * p + 1],
                             pc = rec[REC_PARAM + 3 * p + 2];
                 if (blk) {
-                    float r0 = pa * xf + pb * yf + pc;
-                    for (int k = 0; k < row; ++k)
-                        r0 = r0 + pb;
+                    // row start: the same chain for every row, kept at this lane's row
+                    float r = pa * xf + pb * yf + pc, r0 = r;
+#pragma unroll
+                    for (int k = 1; k < 8; ++k) {
+                        r = r + pb;
+                        r0 = k <= row ? r : r0;
+                    }
                     out[0] = r0;
 #pragma unroll
                     for (int i = 1; i < 8; ++i)
