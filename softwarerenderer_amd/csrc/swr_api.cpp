@@ -61,7 +61,10 @@ struct swr_context {
 
     // arenas
     DevBuf idx, attr0, wverts, app, app_ex, grp, grp_ex, scan_tmp, prims, pk_a, pv_a, pk_b, pv_b,
-        ranges, hist, hist_ex, hist_tmp, status, tgt[3], log, logcount, bigq;  // bigq: per-primitive first pair
+        ranges, hist, hist_ex, hist_tmp, status, tgt[3], log, logcount,
+        bigq,       // per-primitive first pair index
+        clipq,      // triangles queued for the clipper
+        clipq_cnt;  // its per-region counters
     uint64_t prim_cap = 0, pair_cap = 0;
     unsigned long long *status_host = nullptr;
 
@@ -363,6 +366,11 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
     const uint32_t batches = (uint32_t)((T + BATCH - 1) / BATCH);
     const uint64_t ngrp = (uint64_t)batches * 2048;
     HIP_TRY(c, ensure(c, c->app, T * 4));
+    {
+        const size_t blocks = (T + 255) / 256, per_region = ((blocks + 63) / 64) * 256;
+        HIP_TRY(c, ensure(c, c->clipq, 64 * per_region * 4));
+        HIP_TRY(c, ensure(c, c->clipq_cnt, 64 * 16 * 4));
+    }
     HIP_TRY(c, ensure(c, c->app_ex, T * 4));
     HIP_TRY(c, ensure(c, c->grp, ngrp * 8));
     HIP_TRY(c, ensure(c, c->grp_ex, ngrp * 8));
@@ -434,6 +442,8 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
         g.tiles = tiles;
         g.sentinel = pair_sentinel(tiles);
         g.prim_pb = (uint32_t *)c->bigq.p;
+        g.clipq = (uint32_t *)c->clipq.p;
+        g.clipq_cnt = (uint32_t *)c->clipq_cnt.p;
 
         record(c, 0);
         HIP_TRY(c, launch_begin(g, c->stream));
@@ -564,7 +574,8 @@ int swr_destroy(swr_context *c)
     DevBuf *bufs[] = {&c->idx, &c->attr0, &c->wverts, &c->app, &c->app_ex, &c->grp, &c->grp_ex,
                       &c->scan_tmp, &c->prims, &c->pk_a, &c->pv_a, &c->pk_b, &c->pv_b,
                       &c->ranges, &c->hist, &c->hist_ex, &c->hist_tmp, &c->status, &c->tgt[0],
-                      &c->tgt[1], &c->tgt[2], &c->log, &c->logcount, &c->bigq};
+                      &c->tgt[1], &c->tgt[2], &c->log, &c->logcount, &c->bigq, &c->clipq,
+                      &c->clipq_cnt};
     for (DevBuf *b : bufs)
         if (b->p)
             (void)hipFree(b->p);

@@ -48,6 +48,7 @@ constexpr int ST_OVERFLOW = 2;     // bit 0: prim arena, bit 1: pair arena
 constexpr int ST_ERROR = 3;        // bit 0: index range, bit 1: clip overflow
 constexpr int ST_BIG = 4;          // primitives queued for cooperative pair expansion
 constexpr int ST_KEPT = 5;         // pairs left after dropping rejected blocks / tiles
+constexpr int ST_CLIPQ = 6;        // triangles queued for the clipper
 constexpr int ST_WORDS = 8;
 
 // record word 6: index of the primitive's first (tile, primitive) pair
@@ -196,6 +197,8 @@ struct GeomArgs {
     uint32_t tiles;
     uint32_t sentinel;          // pair key of a tile the primitive cannot shade (dropped)
     uint32_t *prim_pb;          // [prim_cap] index of each primitive's first pair
+    uint32_t *clipq;            // [tris + 64*256] triangles that straddle a clip plane
+    uint32_t *clipq_cnt;        // [64 * 16] per-region queue counters
 };
 
 struct RasterArgs {

@@ -123,10 +123,10 @@ __device__ inline int clip_mask(const Pos4 &v)
 // a*x + b*y + c*z + d*w for the plane sequence of VertexProcessor.cpp:237-242
 __device__ inline float plane_dot(int p, const Pos4 &v)
 {
-    const float A[6] = {-1, 1, 0, 0, 0, 0};
-    const float B[6] = {0, 0, -1, 1, 0, 0};
-    const float Cc[6] = {0, 0, 0, 0, -1, 1};
-    return A[p] * v.x + B[p] * v.y + Cc[p] * v.z + 1.0f * v.w;
+    const float a = p == 0 ? -1.0f : (p == 1 ? 1.0f : 0.0f);
+    const float b = p == 2 ? -1.0f : (p == 3 ? 1.0f : 0.0f);
+    const float c = p == 4 ? -1.0f : (p == 5 ? 1.0f : 0.0f);
+    return a * v.x + b * v.y + c * v.z + 1.0f * v.w;
 }
 
 __device__ inline int sgnf(float v) { return (0.0f < v) - (v < 0.0f); }
@@ -390,7 +390,7 @@ __device__ inline void param_write(float *rec, int slot, float p0, float p1, flo
 }
 
 template <int NV>
-__device__ void write_prim(const GeomArgs &a, uint64_t pos, uint32_t key, const PrimGeo &g,
+__device__ __forceinline__ void write_prim(const GeomArgs &a, uint64_t pos, uint32_t key, const PrimGeo &g,
                            const WinV &v0, const WinV &v1, const WinV &v2, const float *var0,
                            const float *var1, const float *var2, uint64_t pair_base)
 {
@@ -530,10 +530,11 @@ struct FanCtx {
 
 // One fan triangle (p0, p[k+1], p[k+2]): transform, cull, setup, count or emit.
 template <int NV, bool EMIT>
-__device__ inline void fan_triangle(const GeomArgs &a, FanCtx &c, int k, WinV w0, WinV w1,
+__device__ __forceinline__ void fan_triangle(const GeomArgs &a, FanCtx &c, int k, WinV w0, WinV w1,
                                     WinV w2, const float *v0, const float *v1, const float *v2,
                                     bool cull)
 {
+    bool swap = false;
     if (cull) {
         // VertexProcessor::cullTriangles, VertexProcessor.cpp:330-343
         float facing = (w0.x - w1.x) * (w2.y - w1.y) - (w2.x - w1.x) * (w0.y - w1.y);
@@ -543,13 +544,13 @@ __device__ inline void fan_triangle(const GeomArgs &a, FanCtx &c, int k, WinV w0
         } else {
             if (a.cull_mode == SWR_CULL_CCW)
                 return;
-            WinV s = w0;  // indices i and i+2 swap (VertexProcessor.cpp:342)
-            w0 = w2;
-            w2 = s;
-            const float *sv = v0;
-            v0 = v2;
-            v2 = sv;
+            swap = true;  // indices i and i+2 swap (VertexProcessor.cpp:342)
         }
+    }
+    {
+        const WinV s0 = w0, s2 = w2;
+        w0 = swap ? s2 : s0;
+        w2 = swap ? s0 : s2;
     }
     PrimGeo g;
     if (!prim_geometry(a, w0, w1, w2, g))
@@ -581,7 +582,15 @@ __device__ inline void fan_triangle(const GeomArgs &a, FanCtx &c, int k, WinV w0
         atomicOr(&a.status[ST_OVERFLOW], 4ull);
         return;
     }
-    write_prim<NV>(a, pos, key, g, w0, w1, w2, v0, v1, v2, pb);
+    // swap by value: selecting between the two arrays by pointer would force them to memory
+    constexpr int NVS = NV > 0 ? NV : 1;
+    float x0[NVS], x2[NVS];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        x0[q] = swap ? v2[q] : v0[q];
+        x2[q] = swap ? v0[q] : v2[q];
+    }
+    write_prim<NV>(a, pos, key, g, w0, w1, w2, x0, v1, x2, pb);
 }
 
 // Clipped triangles: polygon nodes live in per-thread arrays (rare path, kept out of line).
@@ -655,7 +664,168 @@ __device__ inline void clip_path(const GeomArgs &a, FanCtx &c, int cm, const Pos
     }
 }
 
+// All three vertices are outside the first plane the clipper applies (the lowest set bit of
+// the OR of the clip masks, VertexProcessor.cpp:237-242): that pass keeps no vertex and
+// creates none (PolyClipper.cpp:64-74), so the triangle is fully clipped.  For finite
+// coordinates the plane value a*x + b*y + c*z + d*w equals the clip-mask difference
+// (w - x, x + w, ...) exactly, so the test is exact.
+//
+// Beyond the first plane: every vertex the clipper can create is a lerp v0*(1-t) + v1*t
+// (t in [0,1]) of earlier polygon vertices, so a convex combination of the three input
+// vertices up to rounding: at most 5 lerp levels of ~4 units in the last place plus the
+// plane evaluation, under 1.5e-6 of M = max_i (|x|+|y|+|z|+|w|).  If all three vertices lie
+// outside a plane by more than 1e-5 M, every polygon reaching that plane lies outside it,
+// that pass empties the polygon, and the triangle is fully clipped.
+__device__ inline bool fully_clipped_early(const Pos4 *P, int cm)
+{
+    for (int q = 0; q < 3; ++q)
+        if (!(finitef(P[q].x) && finitef(P[q].y) && finitef(P[q].z) && finitef(P[q].w)))
+            return false;
+    const int all_out = clip_mask(P[0]) & clip_mask(P[1]) & clip_mask(P[2]);
+    if (all_out & (cm & -cm))
+        return true;
+    if (!all_out)
+        return false;
+    float m = 0.0f;
+    for (int q = 0; q < 3; ++q)
+        m = fmaxf(m, fabsf(P[q].x) + fabsf(P[q].y) + fabsf(P[q].z) + fabsf(P[q].w));
+    const float margin = -1e-5f * m;
+    for (int p = 0; p < 6; ++p) {
+        if (!(all_out & (1 << p)))
+            continue;
+        bool far_out = true;
+        for (int q = 0; q < 3; ++q)
+            far_out = far_out && plane_dot(p, P[q]) < margin;
+        if (far_out)
+            return true;
+    }
+    return false;
+}
+
+// Exactly one plane selected, finite coordinates, no vertex on that plane (dp != 0): the
+// register-only clip_one_plane applies.  Fills dp with the plane values.
+__device__ __forceinline__ bool one_plane_case(const Pos4 *P, int cm, float *dp)
+{
+    if (cm & (cm - 1))
+        return false;
+    const int p = __builtin_ctz(cm);
+    bool ok = true;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        ok = ok && finitef(P[q].x) && finitef(P[q].y) && finitef(P[q].z) && finitef(P[q].w);
+        dp[q] = plane_dot(p, P[q]);
+        ok = ok && dp[q] != 0.0f;
+    }
+    return ok;
+}
+
+// The clip queue is split over CLIPQ_REGIONS counters in separate cache lines (block b uses
+// region b % CLIPQ_REGIONS) so that enqueueing does not serialise on one address.
+constexpr uint32_t CLIPQ_REGIONS = 64;
+constexpr uint32_t CLIPQ_STRIDE = 16;  // u32 words between counters
+
+__device__ inline uint32_t clipq_region_cap(uint32_t tris)
+{
+    uint32_t blocks = (tris + 255) / 256;
+    return ((blocks + CLIPQ_REGIONS - 1) / CLIPQ_REGIONS) * 256;
+}
+
+// One clip plane, no vertex on it: the Sutherland-Hodgman pass (PolyClipper.cpp:52-78) over
+// (v0,v1), (v1,v2), (v2,v0) keeps each inside vertex and inserts I_ab = lerp(a, b, t) where
+// the sign changes, so with s_i = vertex i inside the polygon is one of six fixed sequences:
+//   (1,1,0) v0 v1 I12 I20   (1,0,1) v0 I01 I12 v2   (0,1,1) I01 v1 v2 I20
+//   (1,0,0) v0 I01 I20      (0,1,0) I01 v1 I12      (0,0,1) I12 v2 I20
+// Fan triangles (p0, p1, p2) and (p0, p2, p3) follow (VertexProcessor.cpp:254-261).
+template <int NV>
+struct CVert {
+    Pos4 p;
+    float v[NV];
+};
+
 template <int NV, bool EMIT>
+__device__ __forceinline__ CVert<NV> intersect(const GeomArgs &a, const Pos4 *P, const float (*var)[NV],
+                                      const float *dp, int ia, int ib)
+{
+    const float dprev = dp[ia], dcur = dp[ib];
+    const float t = dcur < 0 ? dprev / (dprev - dcur) : -dprev / (dcur - dprev);
+    CVert<NV> r;
+    r.p.x = lerp1(P[ia].x, P[ib].x, t);
+    r.p.y = lerp1(P[ia].y, P[ib].y, t);
+    r.p.z = lerp1(P[ia].z, P[ib].z, t);
+    r.p.w = lerp1(P[ia].w, P[ib].w, t);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        bool lerped = k < a.lay.na ? (k < a.vs_na) : ((k - a.lay.na) < a.vs_np);
+        r.v[k] = (EMIT && lerped) ? lerp1(var[ia][k], var[ib][k], t) : 0.0f;
+    }
+    return r;
+}
+
+template <int NV>
+__device__ __forceinline__ CVert<NV> pick(bool c1, const CVert<NV> &x1, bool c2, const CVert<NV> &x2,
+                                 const CVert<NV> &x3)
+{
+    CVert<NV> r;
+    r.p.x = c1 ? x1.p.x : (c2 ? x2.p.x : x3.p.x);
+    r.p.y = c1 ? x1.p.y : (c2 ? x2.p.y : x3.p.y);
+    r.p.z = c1 ? x1.p.z : (c2 ? x2.p.z : x3.p.z);
+    r.p.w = c1 ? x1.p.w : (c2 ? x2.p.w : x3.p.w);
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+        r.v[k] = c1 ? x1.v[k] : (c2 ? x2.v[k] : x3.v[k]);
+    return r;
+}
+
+template <int NV>
+__device__ __forceinline__ void put(WinV &w, float *v, const GeomArgs &a, const CVert<NV> &x)
+{
+    w = to_window(a, x.p);
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+        v[k] = x.v[k];
+}
+
+// Writes the clipped polygon to W / V (3 or 4 vertices) and returns its fan count.
+template <int NV, bool EMIT>
+__device__ __forceinline__ int clip_one_plane(const GeomArgs &a, const Pos4 *P,
+                                              const float (*var)[NV], const float *dp, WinV *W,
+                                              float (*V)[NV])
+{
+    CVert<NV> v0, v1, v2;
+    v0.p = P[0];
+    v1.p = P[1];
+    v2.p = P[2];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        v0.v[k] = var[0][k];
+        v1.v[k] = var[1][k];
+        v2.v[k] = var[2][k];
+    }
+    const CVert<NV> i01 = intersect<NV, EMIT>(a, P, var, dp, 0, 1);
+    const CVert<NV> i12 = intersect<NV, EMIT>(a, P, var, dp, 1, 2);
+    const CVert<NV> i20 = intersect<NV, EMIT>(a, P, var, dp, 2, 0);
+    const int cs = (dp[0] > 0 ? 1 : 0) | (dp[1] > 0 ? 2 : 0) | (dp[2] > 0 ? 4 : 0);
+    const bool quad = cs == 3 || cs == 5 || cs == 6;
+    // o0: v0 (1xx), I01 (010, 011), I12 (001)
+    const CVert<NV> o0 = pick(cs & 1, v0, cs == 2 || cs == 6, i01, i12);
+    // o1: v1 (110, 011, 010), I01 (101, 100), v2 (001)
+    const CVert<NV> o1 = pick(cs == 3 || cs == 6 || cs == 2, v1, cs == 5 || cs == 1, i01, v2);
+    // o2: I12 (110, 101, 010), v2 (011), I20 (100, 001)
+    const CVert<NV> o2 = pick(cs == 3 || cs == 5 || cs == 2, i12, cs == 6, v2, i20);
+    // o3: I20 (110, 011), v2 (101)
+    const CVert<NV> o3 = pick(cs == 5, v2, false, v2, i20);
+    put(W[0], V[0], a, o0);
+    put(W[1], V[1], a, o1);
+    put(W[2], V[2], a, o2);
+    if (quad)
+        put(W[3], V[3], a, o3);
+    return quad ? 2 : 1;
+}
+
+// CLIPQ = false: the main pass over all input triangles; triangles that need the clipper
+// go to a queue (count pass) or are left to the queue kernel (emit pass).
+// CLIPQ = true: one queued triangle, clipped.
+template <int NV, bool EMIT, bool CLIPQ = false>
 __device__ void process_triangle(const GeomArgs &a, uint32_t t)
 {
     FanCtx c;
@@ -678,6 +848,11 @@ __device__ void process_triangle(const GeomArgs &a, uint32_t t)
     }
     constexpr int NVS = NV > 0 ? NV : 1;
     float var3[3][NVS];
+    // the polygon handed to the fan loop below: 3 or 4 window-space vertices
+    WinV W[4];
+    float V[4][NVS];
+    int nfan = 0;
+    bool cull = true;
 
     if (a.window_input) {
         // IRasterizer seam: window-space vertices, no clip, transform or cull
@@ -688,24 +863,24 @@ __device__ void process_triangle(const GeomArgs &a, uint32_t t)
                 atomicOr(&a.status[ST_ERROR], 1ull);
             } else {
                 const int32_t ids[3] = {i0, i1, i2};
-                WinV w[3];
+#pragma unroll
                 for (int q = 0; q < 3; ++q) {
                     const swr_rasterizer_vertex &v = a.wverts[ids[q]];
-                    w[q].x = v.x;
-                    w[q].y = v.y;
-                    w[q].z = v.z;
-                    w[q].w = v.w;
+                    W[q].x = v.x;
+                    W[q].y = v.y;
+                    W[q].z = v.z;
+                    W[q].w = v.w;
                     if (EMIT) {
 #pragma unroll
                         for (int k = 0; k < NV; ++k) {
                             int q2 = k - a.lay.na;
-                            var3[q][k] = k < a.lay.na ? v.avar[k & 15]
-                                                      : (q2 < a.lay.np ? v.pvar[q2 & 15] : 0.0f);
+                            V[q][k] = k < a.lay.na ? v.avar[k & 15]
+                                                   : (q2 < a.lay.np ? v.pvar[q2 & 15] : 0.0f);
                         }
                     }
                 }
-                fan_triangle<NV, EMIT>(a, c, 0, w[0], w[1], w[2], var3[0], var3[1], var3[2],
-                                       false);
+                nfan = 1;
+                cull = false;
             }
         }
     } else {
@@ -721,18 +896,51 @@ __device__ void process_triangle(const GeomArgs &a, uint32_t t)
                 a.app[t] = 0;
         } else {
             Pos4 P[3];
+#pragma unroll
             for (int q = 0; q < 3; ++q)
                 fetch_vertex<EMIT ? NV : 0>(a, ids[q], P[q], var3[q]);
             int cm = clip_mask(P[0]) | clip_mask(P[1]) | clip_mask(P[2]);
-            if (cm == 0) {
-                if (!EMIT)
-                    a.app[t] = 0;
-                fan_triangle<NV, EMIT>(a, c, 0, to_window(a, P[0]), to_window(a, P[1]),
-                                       to_window(a, P[2]), var3[0], var3[1], var3[2], true);
-            } else {
+            float dp1[3];
+            if (CLIPQ) {
                 clip_path<NVS, EMIT>(a, c, cm, P, var3);
+            } else if (cm == 0) {
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    W[q] = to_window(a, P[q]);
+#pragma unroll
+                    for (int k = 0; k < NVS; ++k)
+                        V[q][k] = var3[q][k];
+                }
+                nfan = 1;
+            } else if (fully_clipped_early(P, cm)) {
+                nfan = 0;
+            } else if (one_plane_case(P, cm, dp1)) {
+                nfan = clip_one_plane<NVS, EMIT>(a, P, var3, dp1, W, V);
+            } else {
+                if (!EMIT) {  // the queue kernels clip it and write its counts
+                    const uint32_t r = blockIdx.x % CLIPQ_REGIONS;
+                    uint32_t q = atomicAdd(&a.clipq_cnt[r * CLIPQ_STRIDE], 1u);
+                    a.clipq[r * clipq_region_cap(a.tris) + q] = t;
+                }
+                return;
             }
+            if (!EMIT && !CLIPQ)
+                a.app[t] = nfan == 2 ? 1u : 0u;
         }
+    }
+    // fan triangles (p0, p1, p2) and (p0, p2, p3), VertexProcessor.cpp:254-261; one call
+    // site so the vertex arrays stay in registers
+#pragma unroll 1
+    for (int k = 0; k < nfan; ++k) {
+        const WinV wb = k ? W[2] : W[1];
+        const WinV wc = k ? W[3] : W[2];
+        float vb[NVS], vc[NVS];
+#pragma unroll
+        for (int q = 0; q < NVS; ++q) {
+            vb[q] = k ? V[2][q] : V[1][q];
+            vc[q] = k ? V[3][q] : V[2][q];
+        }
+        fan_triangle<NV, EMIT>(a, c, k, W[0], wb, wc, V[0], vb, vc, cull);
     }
     if (!EMIT) {
         a.grp[(size_t)c.b * 2048 + c.j] = make_uint2(c.cnt0, c.cntp0);
@@ -745,6 +953,8 @@ __global__ void __launch_bounds__(256) k_begin(GeomArgs a)
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < ST_WORDS)
         a.status[i] = 0;
+    if (i < CLIPQ_REGIONS)
+        a.clipq_cnt[i * CLIPQ_STRIDE] = 0;
     for (uint32_t k = i; k < 2 * a.tiles; k += gridDim.x * blockDim.x)
         a.ranges[k] = 0;
     // the last batch's unused (batch, phase, j) slots count nothing
@@ -774,7 +984,52 @@ __global__ void __launch_bounds__(256) k_geom_emit(GeomArgs a)
         process_triangle<NV, true>(a, t);
 }
 
+// The queued triangles that straddle a clip plane, densely packed (order does not matter:
+// each writes its own (batch, phase, j) counts and draw-order positions).  Each block
+// prefix-sums the region counts, then maps a dense index to (region, slot).
+template <int NV, bool EMIT>
+__device__ void run_clip_queue(const GeomArgs &a)
+{
+    __shared__ uint32_t s_off[CLIPQ_REGIONS + 1];
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (uint32_t r = 0; r < CLIPQ_REGIONS; ++r) {
+            s_off[r] = acc;
+            acc += a.clipq_cnt[r * CLIPQ_STRIDE];
+        }
+        s_off[CLIPQ_REGIONS] = acc;
+    }
+    __syncthreads();
+    const uint32_t n = s_off[CLIPQ_REGIONS], cap = clipq_region_cap(a.tris);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = CLIPQ_REGIONS;  // last region with s_off <= i
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (s_off[mid] <= i)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        process_triangle<NV, EMIT, true>(a, a.clipq[lo * cap + (i - s_off[lo])]);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_clip_count(GeomArgs a)
+{
+    run_clip_queue<0, false>(a);
+}
+
+template <int NV>
+__global__ void __launch_bounds__(256) k_clip_emit(GeomArgs a)
+{
+    if (a.status[ST_OVERFLOW] | a.status[ST_ERROR])
+        return;
+    run_clip_queue<NV, true>(a);
+}
+
 // ---- host launchers -----------------------------------------------------------------------
+// queue kernels: enough threads for a tenth of the triangles, at least one block
+#define CLIP_BLOCKS(tris) ((uint32_t)(((tris) / 10 + 255) / 256 + 1))
 hipError_t launch_begin(const GeomArgs &a, hipStream_t s)
 {
     uint32_t work = 2 * a.tiles > a.batches * 1024u ? 2 * a.tiles : a.batches * 1024u;
@@ -790,6 +1045,8 @@ hipError_t launch_geom_count(const GeomArgs &a, hipStream_t s)
     if (!a.tris)
         return hipSuccess;
     hipLaunchKernelGGL(k_geom_count, dim3((a.tris + 255) / 256), dim3(256), 0, s, a);
+    if (!a.window_input)
+        hipLaunchKernelGGL(k_clip_count, dim3(CLIP_BLOCKS(a.tris)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -815,13 +1072,29 @@ hipError_t launch_geom_emit(const GeomArgs &a, hipStream_t s)
 {
     if (!a.tris)
         return hipSuccess;
-    dim3 g((a.tris + 255) / 256), blk(256);
+    dim3 g((a.tris + 255) / 256), blk(256), gq(CLIP_BLOCKS(a.tris));
+    const bool q = !a.window_input;
     switch (emit_class(a.lay.na + a.lay.np)) {
-    case 0: hipLaunchKernelGGL(k_geom_emit<0>, g, blk, 0, s, a); break;
-    case 3: hipLaunchKernelGGL(k_geom_emit<3>, g, blk, 0, s, a); break;
-    case 5: hipLaunchKernelGGL(k_geom_emit<5>, g, blk, 0, s, a); break;
-    case 8: hipLaunchKernelGGL(k_geom_emit<8>, g, blk, 0, s, a); break;
-    default: hipLaunchKernelGGL(k_geom_emit<32>, g, blk, 0, s, a); break;
+    case 0:
+        hipLaunchKernelGGL(k_geom_emit<0>, g, blk, 0, s, a);
+        if (q) hipLaunchKernelGGL(k_clip_emit<1>, gq, blk, 0, s, a);
+        break;
+    case 3:
+        hipLaunchKernelGGL(k_geom_emit<3>, g, blk, 0, s, a);
+        if (q) hipLaunchKernelGGL(k_clip_emit<3>, gq, blk, 0, s, a);
+        break;
+    case 5:
+        hipLaunchKernelGGL(k_geom_emit<5>, g, blk, 0, s, a);
+        if (q) hipLaunchKernelGGL(k_clip_emit<5>, gq, blk, 0, s, a);
+        break;
+    case 8:
+        hipLaunchKernelGGL(k_geom_emit<8>, g, blk, 0, s, a);
+        if (q) hipLaunchKernelGGL(k_clip_emit<8>, gq, blk, 0, s, a);
+        break;
+    default:
+        hipLaunchKernelGGL(k_geom_emit<32>, g, blk, 0, s, a);
+        if (q) hipLaunchKernelGGL(k_clip_emit<32>, gq, blk, 0, s, a);
+        break;
     }
     return hipGetLastError();
 }
