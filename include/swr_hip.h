@@ -116,8 +116,10 @@ typedef struct swr_vertex_shader_desc {
  *                         log (SWR_LOG_WORDS u32 words each) at an atomic slot:
  *                         {x, y, key, seq, z, w, invw, avar[16], pvar[16]} (floats as bits);
  *                         *log_count receives the number of invocations even past capacity.
- *   SWR_PS_VISIBILITY     target0 u32 primitive key+1 (0 = none), target1 f32 depth,
- *                         target2 float2 barycentrics: depth-tested visibility buffer
+ *   SWR_PS_VISIBILITY     target0 u32 source triangle + 1 (0 = none), target1 f32 depth:
+ *                         if (z < depth) { depth = z; id = triangle + 1 }, where triangle is
+ *                         the input triangle (index triple) the primitive was clipped from;
+ *                         the visibility buffer the differentiable path reads
  * "key" identifies the primitive in draw order: (batch << 15) + slot, where slot is the
  * triangle's position in the batch's index list after clipping (VertexProcessor.cpp:254-261);
  * for swr_draw_triangle_list it is the triangle's position in the list.
@@ -197,6 +199,77 @@ int  swr_draw_elements(swr_context *ctx, int mode, size_t count, const int32_t *
  * returns SWR_ERR_CAPACITY. */
 int  swr_draw_elements_async(swr_context *ctx, int mode, size_t count, const int32_t *indices);
 int  swr_reserve(swr_context *ctx, size_t primitives, size_t tile_pairs);
+
+/* ---- batched draws --------------------------------------------------------------------
+ * No counterpart in the reference: a batch stands for a sequence of independent
+ * drawElements(Triangle) calls (VertexProcessor.cpp:78-120), each preceded by its own
+ * setViewport / setDepthRange / setCullMode (VertexProcessor.cpp:43-65), setRasterMode /
+ * setScissorRect (Rasterizer.h:75-87), setVertexAttribPointer(0, ...) (VertexProcessor.cpp:
+ * 67-76) and render targets, all run as one pass of the pipeline (one set of launches for
+ * the whole batch).  The vertex program, its counts, the pixel program and its counts are
+ * the context's at draw time and shared by every scene.  Every scene's result is the one
+ * its own draw would give (bit for bit).  Device buffers only; SWR_PS_LOG is not batched. */
+typedef struct swr_scene {
+    const int32_t *indices;        /* device; index_count / 3 triangles                   */
+    uint64_t index_count;
+    const void *attrib0;           /* device vertex data of attribute 0                  */
+    int32_t  attrib0_stride;
+    int32_t  cull_mode;            /* VertexProcessor::setCullMode                        */
+    uint64_t attrib0_bytes;        /* 0: unchecked (as swr_set_vertex_attrib_pointer)     */
+    float    mvp[16];              /* the MVP programs' uniform for this scene            */
+    int32_t  viewport[4];          /* x, y, w, h (VertexProcessor::setViewport)           */
+    float    depth_near, depth_far;/* VertexProcessor::setDepthRange                      */
+    int32_t  raster_mode;          /* Rasterizer::setRasterMode                           */
+    int32_t  scissor[4];           /* x, y, w, h (Rasterizer::setScissorRect)             */
+    int32_t  width, height, pitch; /* render targets, as in swr_pixel_shader_desc         */
+    void    *target0;
+    void    *target1;
+    void    *target2;
+} swr_scene;
+
+typedef struct swr_batch swr_batch;
+/* Validates the scenes and uploads their table to the device (synchronous).  The scene
+ * array is copied; the buffers it points to must stay valid while the batch is drawn. */
+int  swr_batch_create(swr_context *ctx, const swr_scene *scenes, size_t count, swr_batch **out);
+/* Draws every scene.  async = 0: returns when all targets are final (arenas grow as needed).
+ * async = 1: only enqueues kernels and one status copy on the context stream, so the call
+ * can be captured into a HIP graph; arenas must already be large enough (a synchronous draw
+ * of the same batch sizes them), else swr_synchronize reports SWR_ERR_CAPACITY. */
+int  swr_batch_draw(swr_context *ctx, const swr_batch *batch, int async);
+int  swr_batch_destroy(swr_batch *batch);
+
+/* ---- differentiable shading over a visibility buffer ---------------------------------
+ * No counterpart in the reference.  For V views of one triangle mesh (clip-space positions
+ * per view, attributes shared), after a SWR_PS_VISIBILITY draw of every view (viewport
+ * (0, 0, width, height), depth cleared to 1, ids to 0):
+ *   forward:  bary  = perspective-correct barycentrics (b0, b1) of the visible triangle at
+ *                     each pixel centre; color = sum_i b_i * attribute_i (0 on background);
+ *             image = color plus the analytic edge-antialiasing blend across silhouette
+ *                     edges between 4-neighbour pixels.
+ *   backward: given d(loss)/d(image), adds d(loss)/d(clip x, y, w) per view and vertex to
+ *             grad_clip ([views][vertex_count][4]; z is left alone) and d(loss)/d(attribute)
+ *             summed over views to grad_attributes ([vertex_count][channels]).
+ * All buffers are device memory; both calls only enqueue kernels on the context stream
+ * (they can be captured into a HIP graph).  The backward pass recomputes everything else
+ * from ids, depth, bary and color. */
+typedef struct swr_diff_desc {
+    int32_t views, width, height;
+    int32_t vertex_count, triangle_count;
+    int32_t channels;              /* 1..4 */
+    const float    *clip;          /* [views][vertex_count][4] clip-space x, y, z, w       */
+    const int32_t  *triangles;     /* [triangle_count][3]                                  */
+    const int32_t  *adjacency;     /* [triangle_count][3]: the triangle across the edge
+                                      opposite vertex e, -1 if none (silhouette test)      */
+    const float    *attributes;    /* [vertex_count][channels]                             */
+    const uint32_t *ids;           /* [views][height][width]: triangle + 1, 0 = background */
+    const float    *depth;         /* [views][height][width]                               */
+    float *bary;                   /* [views][height][width][2]: forward out, backward in  */
+    float *color;                  /* [views][height][width][channels], before antialiasing */
+    float *image;                  /* [views][height][width][channels]: forward out        */
+} swr_diff_desc;
+int  swr_diff_forward(swr_context *ctx, const swr_diff_desc *d);
+int  swr_diff_backward(swr_context *ctx, const swr_diff_desc *d, const float *grad_image,
+                       float *grad_clip, float *grad_attributes);
 
 /* ---- statistics of the last completed draw ------------------------------------------- */
 typedef struct swr_draw_stats {

@@ -89,6 +89,9 @@ class Targets:
             self.t1 = np.full(shape, depth_clear, np.float32)
         elif program in (PS_RGB, PS_COVERAGE):
             self.t0 = np.zeros(shape, np.uint32)
+        elif program == PS_VISIBILITY:
+            self.t0 = np.zeros(shape, np.uint32)
+            self.t1 = np.full(shape, depth_clear, np.float32)
         elif program == PS_HASH:
             self.t0 = np.full(shape, HASH_SEED, np.uint32)
             self.t1 = np.zeros(shape, np.uint32)
@@ -211,3 +214,36 @@ class CpuRenderer:
         if n < 0:
             raise RuntimeError(f"clip failed {n}")
         return out[: min(n, max_out)], n
+
+
+def diff_reference(clip, tris, adj, attr, ids, depth, grad_image=None):
+    """FP64 reference of swr_diff_forward / swr_diff_backward (oracle/swr_diff_ref.c).
+
+    clip [V,N,4] f32, tris [T,3] i32, adj [T,3] i32, attr [N,C] f32, ids [V,H,W] u32,
+    depth [V,H,W] f32, grad_image [V,H,W,C] f32 or None.  Returns (image [V,H,W,C] f64,
+    grad_clip [V,N,4] f64 or None, grad_attr [N,C] f64 or None)."""
+    lib = C.CDLL(ORACLE_SO)
+    f = lib.swr_oracle_diff
+    f.restype = C.c_int
+    f.argtypes = [C.c_int] * 6 + [C.c_void_p] * 10
+    clip = np.ascontiguousarray(clip, np.float32)
+    tris = np.ascontiguousarray(tris, np.int32)
+    adj = np.ascontiguousarray(adj, np.int32)
+    attr = np.ascontiguousarray(attr, np.float32)
+    ids = np.ascontiguousarray(ids, np.uint32)
+    depth = np.ascontiguousarray(depth, np.float32)
+    V, H, W = ids.shape
+    N, Cc = attr.shape
+    image = np.zeros((V, H, W, Cc), np.float64)
+    gclip = gattr = gi = None
+    if grad_image is not None:
+        gi = np.ascontiguousarray(grad_image, np.float32)
+        gclip = np.zeros((V, N, 4), np.float64)
+        gattr = np.zeros((N, Cc), np.float64)
+    rc = f(V, W, H, N, tris.shape[0], Cc, clip.ctypes.data, tris.ctypes.data, adj.ctypes.data,
+           attr.ctypes.data, ids.ctypes.data, depth.ctypes.data, image.ctypes.data,
+           None if gi is None else gi.ctypes.data, None if gclip is None else gclip.ctypes.data,
+           None if gattr is None else gattr.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"swr_oracle_diff failed: {rc}")
+    return image, gclip, gattr

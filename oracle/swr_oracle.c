@@ -114,6 +114,7 @@ typedef struct {
     int minX, maxX, minY, maxY;      /* Rasterizer.h:81-87 */
     int na, np, iz, iw;              /* pixel shader statics */
     uint32_t key;                    /* primitive in draw order */
+    uint32_t src;                    /* input triangle the primitive comes from */
 } rctx;
 
 static void rctx_init(rctx *r, const swr_pipeline_state *st)
@@ -128,6 +129,7 @@ static void rctx_init(rctx *r, const swr_pipeline_state *st)
     r->iz = st->ps.interpolate_z != 0;
     r->iw = st->ps.interpolate_w != 0;
     r->key = 0;
+    r->src = 0;
 }
 
 /* PixelData, PixelData.h:38-125 */
@@ -215,6 +217,14 @@ static void draw_pixel(const rctx *r, const pix_t *p)
     case SWR_PS_COVERAGE:
         ((uint32_t *)ps->target0)[o] = 1u;
         break;
+    case SWR_PS_VISIBILITY: {
+        float *depth = (float *)ps->target1;
+        if (p->z < depth[o]) {
+            depth[o] = p->z;
+            ((uint32_t *)ps->target0)[o] = r->src + 1u;
+        }
+        break;
+    }
     case SWR_PS_HASH: {
         uint32_t *hp = (uint32_t *)ps->target0;
         uint32_t h = mix32(hp[o], r->key);
@@ -591,13 +601,16 @@ static void draw_line(const rctx *r, const vtx *v0, const vtx *v1)
 
 static uint32_t make_key(uint32_t batch, size_t slot) { return (batch << 15) + (uint32_t)slot; }
 
-/* Rasterizer::drawTriangleList, Rasterizer.h:134-141 */
-static void draw_triangle_list(rctx *r, uint32_t batch, const vtx *v, const int32_t *idx, size_t n)
+/* Rasterizer::drawTriangleList, Rasterizer.h:134-141.  src: batch-local input triangle of
+ * each slot (NULL: the slot itself). */
+static void draw_triangle_list(rctx *r, uint32_t batch, const vtx *v, const int32_t *idx, size_t n,
+                               const uint32_t *src)
 {
     for (size_t i = 0; i + 3 <= n; i += 3) {
         if (idx[i] == -1)
             continue;
         r->key = make_key(batch, i / 3);
+        r->src = src ? batch * (uint32_t)SWR_BATCH_PRIMITIVES + src[i / 3] : (uint32_t)(i / 3);
         draw_triangle(r, &v[idx[i]], &v[idx[i + 1]], &v[idx[i + 2]]);
     }
 }
@@ -643,7 +656,7 @@ int swr_oracle_draw_triangle_list(const swr_pipeline_state *st, const vtx *v, si
         return rc;
     rctx r;
     rctx_init(&r, st);
-    draw_triangle_list(&r, 0, v, idx, n);
+    draw_triangle_list(&r, 0, v, idx, n, NULL);
     return SWR_OK;
 }
 
@@ -747,6 +760,7 @@ typedef struct {
     size_t nv, cv;
     int32_t *idx;
     size_t ni, ci;
+    uint32_t *src;   /* after clipping: the batch-local input triangle of each slot */
 } batch_t;
 
 static int push_vertex(batch_t *b, const vtx *v)
@@ -861,6 +875,17 @@ static int clip_triangles(batch_t *b, int na, int np)
         mask[i] = clip_mask(&b->v[i]);
     size_t n = b->ni;
     int32_t pa[POLY_MAX], pb[POLY_MAX];
+    /* source triangle of every slot: slot j < n/3 is triangle j; fans append theirs */
+    size_t cap_src = n / 3 + 64;
+    uint32_t *src = (uint32_t *)realloc(b->src, cap_src * sizeof(uint32_t));
+    if (!src) {
+        free(mask);
+        return SWR_ERR_CAPACITY;
+    }
+    b->src = src;
+    for (size_t j = 0; j < n / 3; ++j)
+        b->src[j] = (uint32_t)j;
+    size_t nsrc = n / 3;
     for (size_t i = 0; i < n; i += 3) {
         int32_t i0 = b->idx[i], i1 = b->idx[i + 1], i2 = b->idx[i + 2];
         int cm = mask[i0] | mask[i1] | mask[i2];
@@ -896,6 +921,16 @@ static int clip_triangles(batch_t *b, int na, int np)
                 free(mask);
                 return SWR_ERR_CAPACITY;
             }
+            if (nsrc == cap_src) {
+                cap_src *= 2;
+                uint32_t *q = (uint32_t *)realloc(b->src, cap_src * sizeof(uint32_t));
+                if (!q) {
+                    free(mask);
+                    return SWR_ERR_CAPACITY;
+                }
+                b->src = q;
+            }
+            b->src[nsrc++] = (uint32_t)(i / 3);
         }
     }
     free(mask);
@@ -1047,7 +1082,7 @@ static int process_primitives(batch_t *b, rctx *r, int mode, uint32_t batch)
         return rc;
     if (mode == SWR_DRAW_TRIANGLE) {
         cull_triangles(b, st->cull_mode);
-        draw_triangle_list(r, batch, b->v, b->idx, b->ni);
+        draw_triangle_list(r, batch, b->v, b->idx, b->ni, b->src);
     } else if (mode == SWR_DRAW_LINE) {
         draw_line_list(r, batch, b->v, b->idx, b->ni);
     } else {
@@ -1108,6 +1143,7 @@ int swr_oracle_draw_elements(const swr_pipeline_state *st, int mode, size_t coun
         rc = process_primitives(&b, &r, mode, batch);
     free(b.v);
     free(b.idx);
+    free(b.src);
     return rc;
 }
 

@@ -6,4 +6,5 @@ include/swr_hip.h; the kernels live in csrc/ and build into libswr_hip.so in-tre
 from .api import (CullMode, DrawMode, RasterMode, Rasterizer, VertexProcessor,  # noqa: F401
                   PixelShaderBase, VertexShaderBase, NullPixelShader, DummyVertexShader,
                   DepthCheckerShader, RGBShader, CoverageShader, HashShader, LogShader,
-                  MvpVertexShader, PosRGBVertexShader, RawVertexShader, RASTERIZER_VERTEX)
+                  MvpVertexShader, PosRGBVertexShader, RawVertexShader, RASTERIZER_VERTEX,
+                  VisibilityShader, Batch)

@@ -45,6 +45,28 @@ class DrawStats(C.Structure):
                 ("stage_ms", C.c_float * 8), ("stage_launches", C.c_uint32 * 8)]
 
 
+class Scene(C.Structure):
+    """swr_scene: one scene of a batched draw."""
+    _fields_ = [("indices", C.c_void_p), ("index_count", C.c_uint64),
+                ("attrib0", C.c_void_p), ("attrib0_stride", C.c_int32),
+                ("cull_mode", C.c_int32), ("attrib0_bytes", C.c_uint64),
+                ("mvp", C.c_float * 16), ("viewport", C.c_int32 * 4),
+                ("depth_near", C.c_float), ("depth_far", C.c_float),
+                ("raster_mode", C.c_int32), ("scissor", C.c_int32 * 4),
+                ("width", C.c_int32), ("height", C.c_int32), ("pitch", C.c_int32),
+                ("target0", C.c_void_p), ("target1", C.c_void_p), ("target2", C.c_void_p)]
+
+
+class DiffDesc(C.Structure):
+    """swr_diff_desc: differentiable shading over a visibility buffer."""
+    _fields_ = [("views", C.c_int32), ("width", C.c_int32), ("height", C.c_int32),
+                ("vertex_count", C.c_int32), ("triangle_count", C.c_int32),
+                ("channels", C.c_int32), ("clip", C.c_void_p), ("triangles", C.c_void_p),
+                ("adjacency", C.c_void_p), ("attributes", C.c_void_p), ("ids", C.c_void_p),
+                ("depth", C.c_void_p), ("bary", C.c_void_p), ("color", C.c_void_p),
+                ("image", C.c_void_p)]
+
+
 # name -> (restype, argtypes): every entry point include/swr_hip.h declares
 _P, _I, _S, _F = C.c_void_p, C.c_int, C.c_size_t, C.c_float
 SIGNATURES = {
@@ -69,6 +91,11 @@ SIGNATURES = {
     "swr_reserve": (_I, [_P, _S, _S]),
     "swr_get_draw_stats": (_I, [_P, C.POINTER(DrawStats)]),
     "swr_set_profiling": (_I, [_P, _I]),
+    "swr_batch_create": (_I, [_P, C.POINTER(Scene), _S, C.POINTER(C.c_void_p)]),
+    "swr_batch_draw": (_I, [_P, _P, _I]),
+    "swr_batch_destroy": (_I, [_P]),
+    "swr_diff_forward": (_I, [_P, C.POINTER(DiffDesc)]),
+    "swr_diff_backward": (_I, [_P, C.POINTER(DiffDesc), _P, _P, _P]),
 }
 
 _lib = None

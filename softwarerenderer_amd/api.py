@@ -168,6 +168,21 @@ class LogShader(PixelShaderBase):
         return (self.count_img, None, None) + _hw(self.count_img)
 
 
+class VisibilityShader(PixelShaderBase):
+    """Visibility buffer: id = source triangle + 1 (0 = none) under `z < depth`."""
+
+    InterpolateZ = True
+    Program = PS_VISIBILITY
+
+    def __init__(self, ids=None, depth=None):
+        self.ids, self.depth = ids, depth
+
+    def targets(self):
+        if self.ids is None:
+            return None, None, None, 0, 0, 0
+        return (self.ids, self.depth, None) + _hw(self.ids)
+
+
 class VertexShaderBase:
     """VertexShaderBase<T> (VertexShaderBase.h:33-51)."""
 
@@ -361,3 +376,63 @@ class VertexProcessor:
         n = ibytes // 4 if count is None else count
         self._r._check(self._r._lib.swr_draw_elements_async(self._r._ctx, mode, n, ip),
                        "drawElementsAsync")
+
+
+class Batch:
+    """Many independent drawElements(Triangle) calls as one pass (swr_batch_create).
+
+    Each scene is a dict with keys: indices, vertices (attribute 0), stride, mvp (16 floats,
+    row-major), viewport (x, y, w, h), depth_range (n, f), cull_mode, raster_mode, scissor
+    (x, y, w, h) and targets (t0, t1, t2) of width x height (pitch = width).  Buffers are
+    contiguous device tensors.  The vertex and pixel programs are the rasterizer's at draw
+    time (set them with setVertexShader / setPixelShader first)."""
+
+    def __init__(self, rasterizer: Rasterizer, scenes: list):
+        self._r = rasterizer
+        self._keep = []
+        arr = (_abi.Scene * max(1, len(scenes)))()
+        for i, sc in enumerate(scenes):
+            d = arr[i]
+            ip, ib, ik = _buf(sc["indices"])
+            vp, vb, vk = _buf(sc["vertices"])
+            self._keep += [ik, vk]
+            d.indices, d.index_count = ip, ib // 4
+            d.attrib0, d.attrib0_bytes, d.attrib0_stride = vp, vb, int(sc["stride"])
+            d.cull_mode = int(sc.get("cull_mode", CullMode.CW))
+            m = np.asarray(sc.get("mvp", np.eye(4)), np.float32).reshape(16)
+            for k in range(16):
+                d.mvp[k] = float(m[k])
+            for k, v in enumerate(sc["viewport"]):
+                d.viewport[k] = int(v)
+            d.depth_near, d.depth_far = sc.get("depth_range", (0.0, 1.0))
+            d.raster_mode = int(sc.get("raster_mode", RasterMode.Span))
+            for k, v in enumerate(sc["scissor"]):
+                d.scissor[k] = int(v)
+            t = list(sc["targets"]) + [None] * (3 - len(sc["targets"]))
+            ptrs = []
+            for x in t:
+                p, _, kx = _buf(x)
+                self._keep.append(kx)
+                ptrs.append(p)
+            d.target0, d.target1, d.target2 = ptrs
+            h, w = sc["size"]
+            d.width, d.height, d.pitch = int(w), int(h), int(w)
+        self._h = C.c_void_p()
+        rasterizer._check(rasterizer._lib.swr_batch_create(rasterizer._ctx, arr, len(scenes),
+                                                            C.byref(self._h)), "Batch")
+
+    def draw(self, async_: bool = False):
+        r = self._r
+        r._apply_shader([])
+        r._check(r._lib.swr_batch_draw(r._ctx, self._h, int(async_)), "Batch.draw")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._r._lib.swr_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

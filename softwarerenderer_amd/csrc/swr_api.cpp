@@ -74,6 +74,18 @@ struct swr_context {
     bool async_pending = false;
 };
 
+// A batched draw's device tables (include/swr_hip.h, swr_batch_create).
+struct swr_batch {
+    swr_context *ctx = nullptr;
+    DevBuf scenes;               // SceneDev[nscenes]
+    DevBuf bmap;                 // u32 scene of each virtual 1024-triangle batch
+    uint32_t nscenes = 0, batches = 0;
+    uint64_t vtris = 0;          // virtual triangles: every scene padded to whole batches
+    uint64_t tris = 0;           // input triangles
+    uint64_t tiles = 0;          // sum of the scenes' tiles
+    unsigned tmask = 7;          // bit i: every scene provides target i
+};
+
 namespace {
 
 int fail(swr_context *c, int code, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
@@ -175,6 +187,7 @@ int targets_needed(int program, bool need[3])
     switch (program) {
     case SWR_PS_DEPTH_CHECKER:
     case SWR_PS_HASH:
+    case SWR_PS_VISIBILITY:
         need[0] = need[1] = true;
         break;
     case SWR_PS_RGB:
@@ -287,14 +300,18 @@ void collect_stats(swr_context *c)
     }
 }
 
-// The triangle draw: drawElements (window_input = false) or the IRasterizer seam.
+// The triangle draw: drawElements (window_input = false), the IRasterizer seam, or a batch
+// of scenes (bt != nullptr: inputs, state and targets come from the scene table).
 int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool window_input,
-                   const swr_rasterizer_vertex *wverts, size_t nverts, bool sync)
+                   const swr_rasterizer_vertex *wverts, size_t nverts, bool sync,
+                   const swr_batch *bt = nullptr)
 {
-    const size_t T = count / 3;
+    const size_t T = bt ? bt->vtris : count / 3;
     std::memset(&c->stats, 0, sizeof c->stats);
-    c->stats.input_primitives = T;
-    c->stats.tiles = (uint64_t)((c->ps.width + TILE - 1) / TILE) * (uint64_t)((c->ps.height + TILE - 1) / TILE);
+    c->stats.input_primitives = bt ? bt->tris : T;
+    c->stats.tiles = bt ? bt->tiles
+                        : (uint64_t)((c->ps.width + TILE - 1) / TILE) *
+                              (uint64_t)((c->ps.height + TILE - 1) / TILE);
     if (T == 0 || c->ps.program == SWR_PS_NULL)
         return SWR_OK;  // nothing observable happens
     if (T > (size_t)(1u << 31) / 8)
@@ -306,31 +323,47 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
                     c->ps.program, c->ps.interpolate_z, c->ps.interpolate_w, c->ps.avar_count,
                     c->ps.pvar_count);
     const swr_pixel_shader_desc &ps = c->ps;
-    const int W = ps.width, H = ps.height;
+    const int W = bt ? 0 : ps.width, H = bt ? 0 : ps.height;
     const int tiles_x = (W + TILE - 1) / TILE, tiles_y = (H + TILE - 1) / TILE;
-    const uint32_t tiles = (uint32_t)tiles_x * (uint32_t)tiles_y;
+    const uint32_t tiles = bt ? (uint32_t)bt->tiles : (uint32_t)tiles_x * (uint32_t)tiles_y;
     if (tiles == 0)
         return SWR_OK;
-    if (!sync) {
+    if (!sync && !bt) {
         if (!is_device_ptr(indices))
             return fail(c, SWR_ERR_INVALID_ARG, "async draws take device buffers only");
     }
 
     TargetMap tm;
-    int rc = map_targets(c, tm);
-    if (rc)
-        return rc;
+    int rc = SWR_OK;
+    if (bt) {
+        bool need_t[3];
+        targets_needed(ps.program, need_t);
+        for (int i = 0; i < 3; ++i)
+            if (need_t[i] && !((bt->tmask >> i) & 1u))
+                return fail(c, SWR_ERR_INVALID_ARG,
+                            "pixel program %d needs target%d in every scene", ps.program, i);
+        if (ps.program == SWR_PS_LOG)
+            return fail(c, SWR_ERR_UNSUPPORTED, "the log program is not batched");
+    } else {
+        rc = map_targets(c, tm);
+        if (rc)
+            return rc;
+    }
 
     // inputs
     const void *didx = nullptr;
-    rc = resolve_in(c, indices, T * 3 * sizeof(int32_t), c->idx, &didx);
-    if (rc)
-        return rc;
     const void *dattr = nullptr;
     const void *dwv = nullptr;
     uint64_t attr_bytes = 0;
     const uint64_t need = vs_need_bytes(c->vs);
-    if (window_input) {
+    if (!bt) {
+        rc = resolve_in(c, indices, T * 3 * sizeof(int32_t), c->idx, &didx);
+        if (rc)
+            return rc;
+    }
+    if (bt) {
+        // per-scene inputs, validated by swr_batch_create
+    } else if (window_input) {
         rc = resolve_in(c, wverts, nverts * sizeof(swr_rasterizer_vertex), c->wverts, &dwv);
         if (rc)
             return rc;
@@ -444,6 +477,11 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
         g.prim_pb = (uint32_t *)c->bigq.p;
         g.clipq = (uint32_t *)c->clipq.p;
         g.clipq_cnt = (uint32_t *)c->clipq_cnt.p;
+        if (bt) {
+            g.scenes = (const SceneDev *)bt->scenes.p;
+            g.batch_scene = (const uint32_t *)bt->bmap.p;
+            g.nscenes = bt->nscenes;
+        }
 
         record(c, 0);
         HIP_TRY(c, launch_begin(g, c->stream));
@@ -498,6 +536,10 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
         r.log_cap = ps.log_capacity;
         r.log_count = tm.logcount;
         r.checker_cells = ps.checker_cells;
+        if (bt) {
+            r.scenes = g.scenes;
+            r.nscenes = g.nscenes;
+        }
         HIP_TRY(c, launch_raster(ps, L, r, c->stream));
         record(c, 6);
         HIP_TRY(c, hipMemcpyAsync(c->status_host, c->status.p, ST_WORDS * 8, hipMemcpyDeviceToHost,
@@ -519,7 +561,8 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
     rc = check_status(c);
     if (rc)
         return rc;
-    rc = unmap_targets(c, tm);
+    if (!bt)
+        rc = unmap_targets(c, tm);
     if (rc)
         return rc;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -528,6 +571,20 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
 }
 
 }  // namespace
+
+namespace swr {
+
+hipStream_t context_stream(swr_context *c)
+{
+    (void)hipSetDevice(c->device);
+    return c->stream;
+}
+
+int context_fail(swr_context *c, int code, const char *msg) { return fail(c, code, "%s", msg); }
+
+bool context_is_device_ptr(const void *p) { return is_device_ptr(p); }
+
+}  // namespace swr
 
 // ---- C ABI ------------------------------------------------------------------------------
 extern "C" {
@@ -640,7 +697,7 @@ int swr_set_pixel_shader(swr_context *c, const swr_pixel_shader_desc *d)
 {
     if (!c || !d)
         return fail(c, SWR_ERR_INVALID_ARG, "null pixel shader");
-    if (d->program < SWR_PS_NULL || d->program > SWR_PS_LOG)
+    if (d->program < SWR_PS_NULL || d->program > SWR_PS_VISIBILITY)
         return fail(c, SWR_ERR_UNSUPPORTED, "pixel program %d is not built", d->program);
     ParamLayout L;
     if (d->program != SWR_PS_NULL && !raster_layout(*d, L))
@@ -753,6 +810,129 @@ int swr_draw_point_list(swr_context *c, const swr_rasterizer_vertex *, size_t, c
                         size_t)
 {
     return fail(c, SWR_ERR_UNSUPPORTED, "point lists are not built yet");
+}
+
+int swr_batch_create(swr_context *c, const swr_scene *sc, size_t n, swr_batch **out)
+{
+    if (!c || !out || (n && !sc))
+        return fail(c, SWR_ERR_INVALID_ARG, "null batch arguments");
+    *out = nullptr;
+    if (n > (1u << 20))
+        return fail(c, SWR_ERR_CAPACITY, "too many scenes (%zu)", n);
+    (void)hipSetDevice(c->device);
+    std::vector<SceneDev> tab(n);
+    std::vector<uint32_t> bmap;
+    uint64_t tiles = 0, tris = 0;
+    uint32_t batches = 0;
+    unsigned tmask = 7;
+    for (size_t i = 0; i < n; ++i) {
+        const swr_scene &s = sc[i];
+        const uint64_t T = s.index_count / 3;
+        if (s.cull_mode < SWR_CULL_NONE || s.cull_mode > SWR_CULL_CW ||
+            s.raster_mode < SWR_RASTER_SPAN || s.raster_mode > SWR_RASTER_ADAPTIVE)
+            return fail(c, SWR_ERR_INVALID_ARG, "scene %zu: bad cull or raster mode", i);
+        if (s.width < 0 || s.height < 0 || s.pitch < s.width)
+            return fail(c, SWR_ERR_INVALID_ARG, "scene %zu: bad target size", i);
+        if (T && (!is_device_ptr(s.indices) || !is_device_ptr(s.attrib0) ||
+                  s.attrib0_stride < 0 || s.attrib0_bytes == 0))
+            return fail(c, SWR_ERR_INVALID_ARG,
+                        "scene %zu: indices and attribute 0 must be device buffers with a size",
+                        i);
+        void *const t[3] = {s.target0, s.target1, s.target2};
+        for (int k = 0; k < 3; ++k) {
+            if (!t[k])
+                tmask &= ~(1u << k);
+            else if (!is_device_ptr(t[k]))
+                return fail(c, SWR_ERR_INVALID_ARG, "scene %zu: target%d is not device memory",
+                            i, k);
+        }
+        SceneDev &d = tab[i];
+        std::memset(&d, 0, sizeof d);
+        d.indices = s.indices;
+        d.attr0 = (const uint8_t *)s.attrib0;
+        d.attr0_bytes = s.attrib0_bytes;
+        d.t0 = s.target0;
+        d.t1 = s.target1;
+        d.t2 = s.target2;
+        std::memcpy(d.mvp, s.mvp, sizeof d.mvp);
+        // VertexProcessor.cpp:50-53 and 373, as for a single draw
+        volatile float px = (float)s.viewport[2] / 2.0f, py = (float)s.viewport[3] / 2.0f;
+        d.vp_px = px;
+        d.vp_py = py;
+        d.vp_ox = (float)s.viewport[0] + px;
+        d.vp_oy = (float)s.viewport[1] + py;
+        volatile float dzs = 0.5f * (s.depth_far - s.depth_near),
+                       dzb = 0.5f * (s.depth_near + s.depth_far);
+        d.dz_scale = dzs;
+        d.dz_bias = dzb;
+        d.stride0 = s.attrib0_stride;
+        d.cull_mode = s.cull_mode;
+        d.raster_mode = s.raster_mode;
+        d.sc_minx = s.scissor[0];
+        d.sc_miny = s.scissor[1];
+        d.sc_maxx = s.scissor[0] + s.scissor[2];
+        d.sc_maxy = s.scissor[1] + s.scissor[3];
+        d.width = s.width;
+        d.height = s.height;
+        d.pitch = s.pitch;
+        d.tiles_x = (s.width + TILE - 1) / TILE;
+        d.tiles_y = (s.height + TILE - 1) / TILE;
+        d.tris = (uint32_t)T;
+        d.batch0 = batches;
+        d.tile0 = (uint32_t)tiles;
+        const uint32_t nb = (uint32_t)((T + BATCH - 1) / BATCH);
+        for (uint32_t b = 0; b < nb; ++b)
+            bmap.push_back((uint32_t)i);
+        batches += nb;
+        tiles += (uint64_t)d.tiles_x * (uint64_t)d.tiles_y;
+        tris += T;
+        if (T > (1u << 31) / 8 || (uint64_t)batches * BATCH > (1u << 31) / 8 ||
+            tiles >= 0xFFFFFFF0ull)
+            return fail(c, SWR_ERR_CAPACITY, "batch too large at scene %zu", i);
+    }
+    swr_batch *b = new swr_batch();
+    b->ctx = c;
+    b->nscenes = (uint32_t)n;
+    b->batches = batches;
+    b->vtris = (uint64_t)batches * BATCH;
+    b->tris = tris;
+    b->tiles = tiles;
+    b->tmask = n ? tmask : 0;
+    hipError_t e = ensure(c, b->scenes, std::max<size_t>(1, n) * sizeof(SceneDev));
+    if (e == hipSuccess)
+        e = ensure(c, b->bmap, std::max<size_t>(1, bmap.size()) * 4);
+    if (e == hipSuccess && n)
+        e = hipMemcpy(b->scenes.p, tab.data(), n * sizeof(SceneDev), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !bmap.empty())
+        e = hipMemcpy(b->bmap.p, bmap.data(), bmap.size() * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        swr_batch_destroy(b);
+        return fail(c, SWR_ERR_HIP, "batch upload failed: %s", hipGetErrorString(e));
+    }
+    *out = b;
+    return SWR_OK;
+}
+
+int swr_batch_draw(swr_context *c, const swr_batch *b, int async)
+{
+    if (!c || !b || b->ctx != c)
+        return fail(c, SWR_ERR_INVALID_ARG, "batch does not belong to this context");
+    (void)hipSetDevice(c->device);
+    return draw_triangles(c, nullptr, 0, false, nullptr, 0, async == 0, b);
+}
+
+int swr_batch_destroy(swr_batch *b)
+{
+    if (!b)
+        return SWR_ERR_INVALID_ARG;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    if (b->scenes.p)
+        (void)hipFree(b->scenes.p);
+    if (b->bmap.p)
+        (void)hipFree(b->bmap.p);
+    delete b;
+    return SWR_OK;
 }
 
 int swr_get_draw_stats(swr_context *c, swr_draw_stats *out)

@@ -53,6 +53,10 @@ constexpr int ST_WORDS = 8;
 
 // record word 6: index of the primitive's first (tile, primitive) pair
 constexpr int REC_PAIRBASE = 6;
+// record word 7: the input triangle the primitive comes from (scene-local index)
+constexpr int REC_SRC = 7;
+// record word 29: the scene of a batched draw (0 otherwise)
+constexpr int REC_SCENE = 29;
 
 // Which plane equations the rasterizer interpolates (chosen by the pixel program).
 struct ParamLayout {
@@ -159,6 +163,24 @@ __device__ inline bool span_row_extent(const float *rec, uint32_t flags, int y, 
 // pair value: primitive index | classification << 30 (Block primitives)
 constexpr uint32_t PAIR_PRIM_MASK = 0x3FFFFFFFu;
 
+// ---- batched draws ------------------------------------------------------------------
+// One scene of a batched draw: its inputs, its VertexProcessor / Rasterizer state and its
+// render targets.  Scene s owns the virtual triangles [batch0 * 1024, batch0 * 1024 + tris)
+// (each scene starts on a fresh 1024-triangle batch, so its draw order is exactly that of
+// a draw of its own) and the global tiles [tile0, tile0 + tiles_x * tiles_y).
+struct SceneDev {
+    const int32_t *indices;
+    const uint8_t *attr0;
+    uint64_t attr0_bytes;
+    void *t0, *t1, *t2;
+    float mvp[16];
+    float vp_px, vp_py, vp_ox, vp_oy, dz_scale, dz_bias;
+    int stride0, cull_mode, raster_mode;
+    int sc_minx, sc_maxx, sc_miny, sc_maxy;
+    int width, height, pitch, tiles_x, tiles_y;
+    uint32_t tris, batch0, tile0, pad0;
+};
+
 // ---- launch arguments ---------------------------------------------------------------
 struct GeomArgs {
     // input
@@ -199,6 +221,10 @@ struct GeomArgs {
     uint32_t *prim_pb;          // [prim_cap] index of each primitive's first pair
     uint32_t *clipq;            // [tris + 64*256] triangles that straddle a clip plane
     uint32_t *clipq_cnt;        // [64 * 16] per-region queue counters
+    // batched draw: per-scene state replaces the fields above (nscenes == 0: single draw)
+    const SceneDev *scenes;
+    const uint32_t *batch_scene;  // [batches] scene of each virtual batch
+    uint32_t nscenes;
 };
 
 struct RasterArgs {
@@ -217,6 +243,50 @@ struct RasterArgs {
     uint64_t log_cap;
     unsigned long long *log_count;
     float checker_cells;
+    const SceneDev *scenes;     // batched draw (nscenes == 0: single draw)
+    uint32_t nscenes;
 };
+
+// Scene-local state of a batched draw over the single-draw arguments.
+__device__ inline void apply_scene(GeomArgs &a, const SceneDev &s)
+{
+    a.indices = s.indices;
+    a.attr0 = s.attr0;
+    a.attr0_bytes = s.attr0_bytes;
+    a.stride0 = s.stride0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+        a.mvp[i] = s.mvp[i];
+    a.vp_px = s.vp_px;
+    a.vp_py = s.vp_py;
+    a.vp_ox = s.vp_ox;
+    a.vp_oy = s.vp_oy;
+    a.dz_scale = s.dz_scale;
+    a.dz_bias = s.dz_bias;
+    a.cull_mode = s.cull_mode;
+    a.raster_mode = s.raster_mode;
+    a.sc_minx = s.sc_minx;
+    a.sc_maxx = s.sc_maxx;
+    a.sc_miny = s.sc_miny;
+    a.sc_maxy = s.sc_maxy;
+    a.width = s.width;
+    a.height = s.height;
+    a.tiles_x = s.tiles_x;
+    a.tiles_y = s.tiles_y;
+}
+
+// The scene owning global tile `tile` (scenes sorted by tile0).
+__device__ inline uint32_t scene_of_tile(const SceneDev *sc, uint32_t n, uint32_t tile)
+{
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (sc[mid].tile0 <= tile)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
 
 }  // namespace swr

@@ -1,0 +1,655 @@
+// swr_diff.hip — differentiable shading over a visibility buffer, forward and backward.
+//
+// The visibility draw (SWR_PS_VISIBILITY, one batched draw over every view) leaves per
+// pixel the nearest input triangle and its depth.  Everything else is recomputed from it:
+//
+//   barycentrics   r = (X, Y, 1), the pixel centre in NDC; h_i = (x_i, y_i, w_i) the clip
+//                  coordinates of the triangle's vertices.  D_i = r . (h_j x h_k) over the
+//                  cyclic (i, j, k); b_i = D_i / (D_0 + D_1 + D_2): the perspective-correct
+//                  weights (sum_i b_i h_i is parallel to r).
+//   colour         C = sum_i b_i A_i (per channel), 0 on the background.
+//   edge antialiasing
+//                  for every pair of 4-neighbour pixels (p, q) showing different triangles:
+//                  the front pixel f is the one with the smaller depth (background is
+//                  farthest; ties go to the lower pixel index) and T its triangle.  Among
+//                  T's edges that separate the two pixel centres (edge function >= 0 at f,
+//                  < 0 at the other pixel o) take the first one crossed from f, at
+//                  t = E(f) / (E(f) - E(o)) in [0, 1], and walk on across interior edges
+//                  to the first silhouette edge the segment crosses (no neighbouring
+//                  triangle across it, or the neighbour faces the other way).  The pixel
+//                  whose half of the segment holds the crossing is blended with the other:
+//                    t < 1/2: f += a (1/2 - t) (C_o - C_f);  else o += a (t - 1/2) (E - C_o)
+//                  where E is the silhouette triangle's colour at the crossing point,
+//                  with the axis weight a = ey^2 / (ex^2 + ey^2) for horizontal pairs and
+//                  ex^2 / (ex^2 + ey^2) for vertical pairs, (ex, ey) the edge's direction on
+//                  screen.  Each axis alone keeps the image continuous as an edge sweeps
+//                  over a pixel centre, and the weights sum to one, so the sum does too.
+//                  Blends use the colours before antialiasing and add up.
+//
+// The backward pass recomputes the same quantities per pixel (gather form: every pixel
+// re-evaluates the pairs it belongs to), so the only per-pixel state kept from the forward
+// pass is the visibility buffer, the barycentrics and the colour.  Gradients reach the
+// vertex attributes through b_i, the clip coordinates through D_i (barycentrics) and
+// through t (the silhouette crossing), and are accumulated with float atomics.
+//
+// Precision.  D_i is a cancellation-prone quantity: for a small or grazing triangle the
+// three clip vectors are nearly parallel, and fp32 loses up to 1e-4 of D.  The values
+// (D, b, t and every derivative) are therefore evaluated in fp64 from the fp32 inputs; the
+// kernels are bound by memory and atomics, not by the fp64 rate.  The discrete choices of
+// the antialiasing pass use fp32, so they are reproducible by a CPU restatement.
+#include "swr_common.h"
+#include "swr_internal.h"
+
+namespace swr {
+
+struct DiffArgs {
+    int V, W, H, N, T;
+    float sx, sy;  // W/2, H/2: NDC -> window scale (viewport (0, 0, W, H))
+    const float4 *clip;
+    const int32_t *tris;
+    const int32_t *adj;
+    const float *attr;
+    const uint32_t *ids;
+    const float *depth;
+    float *bary;
+    float *color;
+    float *image;
+    const float *gimage;
+    float *gclip;
+    float *gattr;
+};
+
+// clip (x, y, w) of a vertex, in fp64
+struct H3 {
+    double x, y, w;
+};
+
+__device__ inline H3 cross3(const H3 &a, const H3 &b)
+{
+    return {a.y * b.w - a.w * b.y, a.w * b.x - a.x * b.w, a.x * b.y - a.y * b.x};
+}
+
+__device__ inline double dot3(const H3 &a, const H3 &b) { return a.x * b.x + a.y * b.y + a.w * b.w; }
+
+__device__ inline H3 load_h(const DiffArgs &d, int view, int v)
+{
+    const float4 c = d.clip[(size_t)view * d.N + v];
+    return {(double)c.x, (double)c.y, (double)c.w};
+}
+
+__device__ inline H3 pixel_ray(const DiffArgs &d, int px, int py)
+{
+    const double X = 2.0 * ((double)px + 0.5) / (double)d.W - 1.0;
+    const double Y = 1.0 - 2.0 * ((double)py + 0.5) / (double)d.H;
+    return {X, Y, 1.0};
+}
+
+// Window position of a clip-space vertex (w > 0): fp32 for decisions, fp64 for values.
+__device__ inline float2 to_win_f(const DiffArgs &d, int view, int v)
+{
+    const float4 h = d.clip[(size_t)view * d.N + v];
+    return make_float2(d.sx * (h.x / h.w) + d.sx, -d.sy * (h.y / h.w) + d.sy);
+}
+
+__device__ inline double2 to_win_d(const DiffArgs &d, const H3 &h)
+{
+    const double sx = 0.5 * d.W, sy = 0.5 * d.H;
+    return make_double2(sx * (h.x / h.w) + sx, -sy * (h.y / h.w) + sy);
+}
+
+__device__ inline float edge_fn(float2 j, float2 k, float2 p)
+{
+    return (k.x - j.x) * (p.y - j.y) - (k.y - j.y) * (p.x - j.x);
+}
+
+__device__ inline double edge_d(double2 j, double2 k, double2 p)
+{
+    return (k.x - j.x) * (p.y - j.y) - (k.y - j.y) * (p.x - j.x);
+}
+
+__device__ inline bool w_positive(const DiffArgs &d, int view, int v)
+{
+    return d.clip[(size_t)view * d.N + v].w > 0.0f;
+}
+
+// One neighbour pair as the antialiasing pass sees it.
+struct Pair {
+    bool active;      // a silhouette edge crosses the segment between the centres
+    bool p_blended;   // the blended pixel is p (else q)
+    bool horiz;       // horizontal pair
+    float tf, s;      // fp32 crossing (decisions), orientation of the triangle
+    int tri, vj, vk;  // triangle and the edge's vertices
+    int front_is_p;
+};
+
+// Values of an active pair in fp64: the crossing t and the axis weight a, with their
+// derivatives with respect to the window positions of the edge's vertices j and k.
+struct PairVal {
+    double t, a;
+    double2 tj, tk, aj, ak;
+};
+
+__device__ inline PairVal pair_val(const DiffArgs &d, int view, const Pair &pr, int px, int py,
+                                   int qx, int qy)
+{
+    PairVal r;
+    const double2 j = to_win_d(d, load_h(d, view, pr.vj)), k = to_win_d(d, load_h(d, view, pr.vk));
+    const double2 P = make_double2(px + 0.5, py + 0.5), Qp = make_double2(qx + 0.5, qy + 0.5);
+    const double2 F = pr.front_is_p ? P : Qp, Q = pr.front_is_p ? Qp : P;
+    const double s = pr.s;
+    const double ef = s * edge_d(j, k, F), eq = s * edge_d(j, k, Q);
+    const double den = ef - eq;
+    r.t = ef / den;
+    const double dtf = -eq / (den * den), dtq = ef / (den * den);
+    // edge(j, k, P) partials: d/dj = (k.y - P.y, P.x - k.x), d/dk = (P.y - j.y, j.x - P.x)
+    r.tj = make_double2(s * (dtf * (k.y - F.y) + dtq * (k.y - Q.y)),
+                        s * (dtf * (F.x - k.x) + dtq * (Q.x - k.x)));
+    r.tk = make_double2(s * (dtf * (F.y - j.y) + dtq * (Q.y - j.y)),
+                        s * (dtf * (j.x - F.x) + dtq * (j.x - Q.x)));
+    // a = ey^2 / L2 (horizontal pair) or ex^2 / L2, L2 = ex^2 + ey^2, (ex, ey) = k - j
+    const double ex = k.x - j.x, ey = k.y - j.y, L2 = ex * ex + ey * ey;
+    const double wx = ey * ey / L2;
+    const double dex = -2.0 * ex * ey * ey / (L2 * L2), dey = 2.0 * ey * ex * ex / (L2 * L2);
+    const double sg = pr.horiz ? 1.0 : -1.0;  // d(1 - wx) = -d(wx)
+    r.a = pr.horiz ? wx : 1.0 - wx;
+    r.ak = make_double2(sg * dex, sg * dey);
+    r.aj = make_double2(-sg * dex, -sg * dey);
+    return r;
+}
+
+// Window positions of triangle t's vertices (fp32); false when a vertex has w <= 0.
+__device__ inline bool tri_win(const DiffArgs &d, int view, int t, float2 *v, int *vid)
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        vid[i] = d.tris[3 * t + i];
+        if (!w_positive(d, view, vid[i]))
+            return false;
+        v[i] = to_win_f(d, view, vid[i]);
+    }
+    return true;
+}
+
+constexpr int MARCH_STEPS = 8;
+
+// The pair (p, q): start in the front pixel's triangle and walk along the segment from the
+// front centre F to the other centre Q across interior edges (adjacent triangles facing the
+// same way) until the segment leaves the mesh through a silhouette edge, ends inside a
+// triangle (no crossing: inactive), or MARCH_STEPS triangles were visited (inactive).  The
+// exit edge of a triangle is the one the segment crosses first while leaving its half-plane
+// (edge value < 0 at Q and falling from F to Q).
+__device__ inline Pair eval_pair(const DiffArgs &d, int view, int px, int py, int qx, int qy)
+{
+    Pair r;
+    r.active = false;
+    const size_t base = (size_t)view * d.W * d.H;
+    const size_t ip = base + (size_t)py * d.W + px, iq = base + (size_t)qy * d.W + qx;
+    const uint32_t a = d.ids[ip], b = d.ids[iq];
+    if (a == b)
+        return r;
+    bool pf;
+    if (a == 0)
+        pf = false;
+    else if (b == 0)
+        pf = true;
+    else {
+        const float dp = d.depth[ip], dq = d.depth[iq];
+        pf = dp < dq || (dp == dq && ip < iq);
+    }
+    int cur = (int)(pf ? a : b) - 1;
+    if (cur < 0 || cur >= d.T)
+        return r;
+    const bool horiz = py == qy;
+    const float2 F = pf ? make_float2(px + 0.5f, py + 0.5f) : make_float2(qx + 0.5f, qy + 0.5f);
+    const float2 Q = pf ? make_float2(qx + 0.5f, qy + 0.5f) : make_float2(px + 0.5f, py + 0.5f);
+    float2 v[3];
+    int vid[3];
+    if (!tri_win(d, view, cur, v, vid))
+        return r;
+    float area = edge_fn(v[0], v[1], v[2]);
+    if (!(area != 0.0f))
+        return r;
+    int from = -1;
+    for (int step = 0; step < MARCH_STEPS; ++step) {
+        const float s = area > 0.0f ? 1.0f : -1.0f;
+        float best = 2.0f;
+        int be = -1;
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            if (e == from)
+                continue;
+            const int j = (e + 1) % 3, k = (e + 2) % 3;
+            const float ef = s * edge_fn(v[j], v[k], F), eq = s * edge_fn(v[j], v[k], Q);
+            if (eq < 0.0f && ef > eq) {
+                const float tt = ef / (ef - eq);
+                if (tt < best) {
+                    best = tt;
+                    be = e;
+                }
+            }
+        }
+        if (be < 0)
+            return r;  // the segment ends inside this triangle
+        const int n = d.adj[3 * cur + be];
+        float2 nv[3];
+        int nvid[3];
+        float an = 0.0f;
+        bool interior = false;
+        if (n >= 0 && n < d.T && tri_win(d, view, n, nv, nvid)) {
+            an = edge_fn(nv[0], nv[1], nv[2]);
+            interior = an != 0.0f && ((an > 0.0f) == (area > 0.0f));
+        }
+        if (!interior) {
+            const int j = (be + 1) % 3, k = (be + 2) % 3;
+            if (v[k].x == v[j].x && v[k].y == v[j].y)
+                return r;  // a degenerate edge has no direction
+            r.active = true;
+            r.horiz = horiz;
+            r.tf = best;
+            r.s = s;
+            r.tri = cur;
+            r.vj = vid[j];
+            r.vk = vid[k];
+            r.front_is_p = pf;
+            r.p_blended = (best < 0.5f) == pf;
+            return r;
+        }
+        int fe = -1;
+#pragma unroll
+        for (int e = 0; e < 3; ++e)
+            if (d.adj[3 * n + e] == cur)
+                fe = e;
+        if (fe < 0)
+            return r;
+        cur = n;
+        from = fe;
+        area = an;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            v[i] = nv[i];
+            vid[i] = nvid[i];
+        }
+    }
+    return r;
+}
+
+// Perspective-correct barycentrics of triangle t at the centre of pixel (px, py), in fp64
+// (the pixel may lie outside the triangle: the antialiasing extrapolates).  False when the
+// three clip vectors are linearly dependent there (S == 0).
+struct Bary {
+    int v[3];
+    H3 h[3];
+    H3 r;
+    double b[3], S;
+};
+
+__device__ inline bool bary_at_ray(const DiffArgs &d, int view, int t, const H3 &r, Bary &B)
+{
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        B.v[m] = d.tris[3 * t + m];
+        B.h[m] = load_h(d, view, B.v[m]);
+    }
+    B.r = r;
+    const double D0 = dot3(B.r, cross3(B.h[1], B.h[2])), D1 = dot3(B.r, cross3(B.h[2], B.h[0])),
+                 D2 = dot3(B.r, cross3(B.h[0], B.h[1]));
+    B.S = D0 + D1 + D2;
+    if (B.S == 0.0)
+        return false;
+    B.b[0] = D0 / B.S;
+    B.b[1] = D1 / B.S;
+    B.b[2] = D2 / B.S;
+    return true;
+}
+
+__device__ inline bool bary_at(const DiffArgs &d, int view, int t, int px, int py, Bary &B)
+{
+    return bary_at_ray(d, view, t, pixel_ray(d, px, py), B);
+}
+
+// The point where the pair's segment crosses the silhouette edge, as a ray (NDC, 1), and
+// its derivative with respect to t.
+__device__ inline H3 crossing_ray(const DiffArgs &d, const Pair &pr, double t, int px, int py,
+                                  int qx, int qy, H3 &drdt)
+{
+    const double fx = (pr.front_is_p ? px : qx) + 0.5, fy = (pr.front_is_p ? py : qy) + 0.5;
+    const double ox = (pr.front_is_p ? qx : px) + 0.5, oy = (pr.front_is_p ? qy : py) + 0.5;
+    const double wx = fx + t * (ox - fx), wy = fy + t * (oy - fy);
+    drdt = {2.0 * (ox - fx) / d.W, -2.0 * (oy - fy) / d.H, 0.0};
+    return {2.0 * wx / d.W - 1.0, 1.0 - 2.0 * wy / d.H, 1.0};
+}
+
+// d(colour at ray r)/d(r) . dr, per channel, for barycentrics B: b_m = D_m / S with
+// dD_m/dr = h_j x h_k.
+template <int C>
+__device__ inline void bary_color_dr(const DiffArgs &d, const Bary &B, const H3 &dr, double *out)
+{
+    const H3 c0 = cross3(B.h[1], B.h[2]), c1 = cross3(B.h[2], B.h[0]), c2 = cross3(B.h[0], B.h[1]);
+    const double e0 = dot3(c0, dr), e1 = dot3(c1, dr), e2 = dot3(c2, dr);
+    const double es = e0 + e1 + e2;
+    const double db[3] = {(e0 - B.b[0] * es) / B.S, (e1 - B.b[1] * es) / B.S,
+                          (e2 - B.b[2] * es) / B.S};
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+        out[c] = db[0] * d.attr[(size_t)B.v[0] * C + c] + db[1] * d.attr[(size_t)B.v[1] * C + c] +
+                 db[2] * d.attr[(size_t)B.v[2] * C + c];
+}
+
+template <int C>
+__device__ inline void bary_color(const DiffArgs &d, const Bary &B, double *col)
+{
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+        col[c] = B.b[0] * d.attr[(size_t)B.v[0] * C + c] + B.b[1] * d.attr[(size_t)B.v[1] * C + c] +
+                 B.b[2] * d.attr[(size_t)B.v[2] * C + c];
+}
+
+// Backward of col = sum_m b_m A_m given dL/dcol: adds to the attributes and, through
+// b_m = D_m / S with D_m = r . (h_j x h_k), to the clip coordinates of the three vertices.
+template <int C>
+__device__ inline void bary_backward(const DiffArgs &d, int view, const Bary &B, const double *gcol)
+{
+    double u[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        u[m] = 0.0;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            u[m] += gcol[c] * d.attr[(size_t)B.v[m] * C + c];
+            atomicAdd(d.gattr + (size_t)B.v[m] * C + c, (float)(B.b[m] * gcol[c]));
+        }
+    }
+    const double ub = u[0] * B.b[0] + u[1] * B.b[1] + u[2] * B.b[2];
+    H3 g[3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        const double gD = (u[m] - ub) / B.S;
+        const int j = (m + 1) % 3, k = (m + 2) % 3;
+        // dD_m/dh_j = h_k x r, dD_m/dh_k = r x h_j
+        const H3 dj = cross3(B.h[k], B.r), dk = cross3(B.r, B.h[j]);
+        g[j].x += gD * dj.x;
+        g[j].y += gD * dj.y;
+        g[j].w += gD * dj.w;
+        g[k].x += gD * dk.x;
+        g[k].y += gD * dk.y;
+        g[k].w += gD * dk.w;
+    }
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        float *gv = d.gclip + ((size_t)view * d.N + B.v[m]) * 4;
+        atomicAdd(gv + 0, (float)g[m].x);
+        atomicAdd(gv + 1, (float)g[m].y);
+        atomicAdd(gv + 3, (float)g[m].w);
+    }
+}
+
+template <int C>
+__global__ void __launch_bounds__(256) k_diff_interp(DiffArgs d)
+{
+    const size_t HW = (size_t)d.W * d.H;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= HW * d.V)
+        return;
+    const int view = (int)(i / HW);
+    const int rem = (int)(i - (size_t)view * HW);
+    const int py = rem / d.W, px = rem - py * d.W;
+    const uint32_t id = d.ids[i];
+    double col[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+        col[c] = 0.0;
+    float2 bb = make_float2(0.0f, 0.0f);
+    Bary B;
+    if (id > 0 && (int)id <= d.T && bary_at(d, view, (int)id - 1, px, py, B)) {
+        bary_color<C>(d, B, col);
+        bb = make_float2((float)B.b[0], (float)B.b[1]);
+    }
+    reinterpret_cast<float2 *>(d.bary)[i] = bb;
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+        d.color[i * C + c] = (float)col[c];
+}
+
+__device__ inline bool neighbour(const DiffArgs &d, int px, int py, int k, int &qx, int &qy)
+{
+    qx = px + (k == 0 ? 1 : (k == 1 ? -1 : 0));
+    qy = py + (k == 2 ? 1 : (k == 3 ? -1 : 0));
+    return qx >= 0 && qx < d.W && qy >= 0 && qy < d.H;
+}
+
+// The blend a pixel receives from one active pair where it is the blended pixel:
+//   front pixel (t < 1/2):  + a (1/2 - t) (C_q - C_p)
+//   other pixel (t >= 1/2): + a (t - 1/2) (E - C_p), E = the silhouette triangle's colour
+//                           where the segment crosses the edge.  As the edge sweeps onto
+//                           this pixel's centre E becomes the colour the pixel then shows,
+//                           so the image stays continuous; on the edge, E is a convex mix of
+//                           the edge's endpoint colours, so it stays bounded.
+template <int C>
+__global__ void __launch_bounds__(256) k_diff_aa(DiffArgs d)
+{
+    const size_t HW = (size_t)d.W * d.H;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= HW * d.V)
+        return;
+    const int view = (int)(i / HW);
+    const int rem = (int)(i - (size_t)view * HW);
+    const int py = rem / d.W, px = rem - py * d.W;
+    double cp[C], out[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+        out[c] = cp[c] = d.color[i * C + c];
+    for (int k = 0; k < 4; ++k) {
+        int qx, qy;
+        if (!neighbour(d, px, py, k, qx, qy))
+            continue;
+        const Pair pr = eval_pair(d, view, px, py, qx, qy);
+        if (!pr.active || !pr.p_blended)
+            continue;
+        const PairVal pv = pair_val(d, view, pr, px, py, qx, qy);
+        double src[C];
+        if (pr.front_is_p) {
+            const size_t iq = (size_t)view * HW + (size_t)qy * d.W + qx;
+#pragma unroll
+            for (int c = 0; c < C; ++c)
+                src[c] = d.color[iq * C + c];
+        } else {
+            Bary B;
+            H3 drdt;
+            if (!bary_at_ray(d, view, pr.tri, crossing_ray(d, pr, pv.t, px, py, qx, qy, drdt), B))
+                continue;
+            bary_color<C>(d, B, src);
+        }
+        const double w = pv.a * (pr.front_is_p ? 0.5 - pv.t : pv.t - 0.5);
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+            out[c] += w * (src[c] - cp[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+        d.image[i * C + c] = (float)out[c];
+}
+
+// d(window x, y) of a clip-space vertex -> d(clip x, y, w), accumulated into gclip.
+__device__ inline void scatter_window_grad(const DiffArgs &d, int view, int v, double gx,
+                                           double gy)
+{
+    const H3 h = load_h(d, view, v);
+    const double iw = 1.0 / h.w, sx = 0.5 * d.W, sy = 0.5 * d.H;
+    float *g = d.gclip + ((size_t)view * d.N + v) * 4;
+    atomicAdd(g + 0, (float)(gx * sx * iw));
+    atomicAdd(g + 1, (float)(-gy * sy * iw));
+    atomicAdd(g + 3, (float)((-gx * sx * h.x + gy * sy * h.y) * iw * iw));
+}
+
+template <int C>
+__global__ void __launch_bounds__(256) k_diff_backward(DiffArgs d)
+{
+    const size_t HW = (size_t)d.W * d.H;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= HW * d.V)
+        return;
+    const int view = (int)(i / HW);
+    const int rem = (int)(i - (size_t)view * HW);
+    const int py = rem / d.W, px = rem - py * d.W;
+    double gp[C], gc[C], cp[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        gc[c] = gp[c] = d.gimage[i * C + c];
+        cp[c] = d.color[i * C + c];
+    }
+
+    // antialiasing: colour gradients (gather) and silhouette crossings (scatter)
+    for (int k = 0; k < 4; ++k) {
+        int qx, qy;
+        if (!neighbour(d, px, py, k, qx, qy))
+            continue;
+        const Pair pr = eval_pair(d, view, px, py, qx, qy);
+        if (!pr.active)
+            continue;
+        const size_t iq = (size_t)view * HW + (size_t)qy * d.W + qx;
+        const bool t_low = pr.tf < 0.5f;  // the front pixel is the blended one
+        if (!pr.p_blended) {
+            // q is blended; its blend reads C_p only when q is the front pixel
+            if (t_low) {
+                const PairVal pv = pair_val(d, view, pr, px, py, qx, qy);
+                const double w = pv.a * (0.5 - pv.t);
+#pragma unroll
+                for (int c = 0; c < C; ++c)
+                    gc[c] += w * d.gimage[iq * C + c];
+            }
+            continue;
+        }
+        const PairVal pv = pair_val(d, view, pr, px, py, qx, qy);
+        const double wt = t_low ? 0.5 - pv.t : pv.t - 0.5;
+        const double w = pv.a * wt;
+        double src[C];
+        double dedt = 0.0;  // d(w gp . E)/dt through the crossing point
+        Bary B;
+        if (t_low) {
+#pragma unroll
+            for (int c = 0; c < C; ++c)
+                src[c] = d.color[iq * C + c];
+        } else {
+            H3 drdt;
+            if (!bary_at_ray(d, view, pr.tri, crossing_ray(d, pr, pv.t, px, py, qx, qy, drdt), B))
+                continue;
+            bary_color<C>(d, B, src);
+            double gsrc[C], de[C];
+#pragma unroll
+            for (int c = 0; c < C; ++c)
+                gsrc[c] = w * gp[c];
+            bary_backward<C>(d, view, B, gsrc);
+            bary_color_dr<C>(d, B, drdt, de);
+#pragma unroll
+            for (int c = 0; c < C; ++c)
+                dedt += gsrc[c] * de[c];
+        }
+        double dot = 0.0;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            gc[c] -= w * gp[c];
+            dot += gp[c] * (src[c] - cp[c]);
+        }
+        const double dldt = pv.a * (t_low ? -dot : dot) + dedt, dlda = wt * dot;
+        scatter_window_grad(d, view, pr.vj, dldt * pv.tj.x + dlda * pv.aj.x,
+                            dldt * pv.tj.y + dlda * pv.aj.y);
+        scatter_window_grad(d, view, pr.vk, dldt * pv.tk.x + dlda * pv.ak.x,
+                            dldt * pv.tk.y + dlda * pv.ak.y);
+    }
+
+    // barycentric interpolation of the pixel's own triangle
+    const uint32_t id = d.ids[i];
+    Bary B;
+    if (id == 0 || (int)id > d.T || !bary_at(d, view, (int)id - 1, px, py, B))
+        return;
+    bary_backward<C>(d, view, B, gc);
+}
+
+template <int C>
+static hipError_t launch_diff(const DiffArgs &d, int which, hipStream_t s)
+{
+    const size_t n = (size_t)d.W * d.H * d.V;
+    const uint32_t blocks = (uint32_t)((n + 255) / 256);
+    if (!blocks)
+        return hipSuccess;
+    if (which == 0) {
+        hipLaunchKernelGGL(k_diff_interp<C>, dim3(blocks), dim3(256), 0, s, d);
+        hipLaunchKernelGGL(k_diff_aa<C>, dim3(blocks), dim3(256), 0, s, d);
+    } else {
+        hipLaunchKernelGGL(k_diff_backward<C>, dim3(blocks), dim3(256), 0, s, d);
+    }
+    return hipGetLastError();
+}
+
+static int diff_run(swr_context *c, const swr_diff_desc *dd, const float *gimage, float *gclip,
+                    float *gattr, int which)
+{
+    if (!c || !dd)
+        return SWR_ERR_INVALID_ARG;
+    const swr_diff_desc &q = *dd;
+    if (q.views < 0 || q.width < 0 || q.height < 0 || q.vertex_count < 0 || q.triangle_count < 0)
+        return context_fail(c, SWR_ERR_INVALID_ARG, "negative size in swr_diff_desc");
+    if (q.channels < 1 || q.channels > 4)
+        return context_fail(c, SWR_ERR_UNSUPPORTED, "channels must be 1..4");
+    const size_t px = (size_t)q.views * q.width * q.height;
+    if (px == 0)
+        return SWR_OK;
+    const void *need[] = {q.clip, q.triangles, q.adjacency, q.attributes, q.ids, q.depth, q.bary,
+                          q.color};
+    for (const void *p : need)
+        if (!p || !context_is_device_ptr(p))
+            return context_fail(c, SWR_ERR_INVALID_ARG, "swr_diff buffers must be device memory");
+    if (which == 0 && (!q.image || !context_is_device_ptr(q.image)))
+        return context_fail(c, SWR_ERR_INVALID_ARG, "swr_diff_forward needs image");
+    if (which == 1 && (!gimage || !gclip || !gattr || !context_is_device_ptr(gimage) ||
+                       !context_is_device_ptr(gclip) || !context_is_device_ptr(gattr)))
+        return context_fail(c, SWR_ERR_INVALID_ARG, "swr_diff_backward needs device gradients");
+    DiffArgs d;
+    d.V = q.views;
+    d.W = q.width;
+    d.H = q.height;
+    d.N = q.vertex_count;
+    d.T = q.triangle_count;
+    d.sx = (float)q.width / 2.0f;
+    d.sy = (float)q.height / 2.0f;
+    d.clip = reinterpret_cast<const float4 *>(q.clip);
+    d.tris = q.triangles;
+    d.adj = q.adjacency;
+    d.attr = q.attributes;
+    d.ids = q.ids;
+    d.depth = q.depth;
+    d.bary = q.bary;
+    d.color = q.color;
+    d.image = q.image;
+    d.gimage = gimage;
+    d.gclip = gclip;
+    d.gattr = gattr;
+    hipStream_t s = context_stream(c);
+    hipError_t e;
+    switch (q.channels) {
+    case 1: e = launch_diff<1>(d, which, s); break;
+    case 2: e = launch_diff<2>(d, which, s); break;
+    case 3: e = launch_diff<3>(d, which, s); break;
+    default: e = launch_diff<4>(d, which, s); break;
+    }
+    if (e != hipSuccess)
+        return context_fail(c, SWR_ERR_HIP, hipGetErrorString(e));
+    return SWR_OK;
+}
+
+}  // namespace swr
+
+extern "C" {
+
+int swr_diff_forward(swr_context *c, const swr_diff_desc *d)
+{
+    return swr::diff_run(c, d, nullptr, nullptr, nullptr, 0);
+}
+
+int swr_diff_backward(swr_context *c, const swr_diff_desc *d, const float *grad_image,
+                      float *grad_clip, float *grad_attributes)
+{
+    return swr::diff_run(c, d, grad_image, grad_clip, grad_attributes, 1);
+}
+
+}  // extern "C"

@@ -360,7 +360,16 @@ __device__ inline bool prim_geometry(const GeomArgs &a, const WinV &v0, const Wi
 __device__ inline void pair_for_tile(const GeomArgs &a, const float *rec, uint32_t flags,
                                      uint32_t pos, int tx, int ty, uint32_t &key, uint32_t &val)
 {
-    key = (uint32_t)(ty * a.tiles_x + tx);
+    int tiles_x = a.tiles_x, sc_minx = a.sc_minx, sc_maxx = a.sc_maxx;
+    uint32_t tile0 = 0;
+    if (a.nscenes) {
+        const SceneDev &S = a.scenes[__float_as_uint(rec[REC_SCENE])];
+        tiles_x = S.tiles_x;
+        sc_minx = S.sc_minx;
+        sc_maxx = S.sc_maxx;
+        tile0 = S.tile0;
+    }
+    key = tile0 + (uint32_t)(ty * tiles_x + tx);
     val = pos;
     const int bx = tx * TILE, by = ty * TILE;
     if (flags & FLAG_BLOCK) {
@@ -373,7 +382,7 @@ __device__ inline void pair_for_tile(const GeomArgs &a, const float *rec, uint32
         bool any = false;
         for (int r = 0; r < TILE && !any; ++r) {
             int xl, xr;
-            bool rowv = span_row_extent(rec, flags, by + r, a.sc_minx, a.sc_maxx, xl, xr);
+            bool rowv = span_row_extent(rec, flags, by + r, sc_minx, sc_maxx, xl, xr);
             any = rowv && imax(xl, bx) < imin(xr, bx + TILE);
         }
         key = any ? key : a.sentinel;
@@ -390,13 +399,16 @@ __device__ inline void param_write(float *rec, int slot, float p0, float p1, flo
 }
 
 template <int NV>
-__device__ __forceinline__ void write_prim(const GeomArgs &a, uint64_t pos, uint32_t key, const PrimGeo &g,
+__device__ __forceinline__ void write_prim(const GeomArgs &a, uint64_t pos, uint32_t key,
+                                           uint32_t src, uint32_t sid, const PrimGeo &g,
                            const WinV &v0, const WinV &v1, const WinV &v2, const float *var0,
                            const float *var1, const float *var2, uint64_t pair_base)
 {
     float *rec = a.prims + pos * (uint64_t)a.lay.stride;
     uint32_t *ru = reinterpret_cast<uint32_t *>(rec);
     ru[REC_KEY] = key;
+    ru[REC_SRC] = src;
+    ru[REC_SCENE] = sid;
     ru[REC_FLAGS] = g.flags;
     ru[REC_TILES + 0] = (uint32_t)g.tx0;
     ru[REC_TILES + 1] = (uint32_t)g.ty0;
@@ -522,7 +534,8 @@ __device__ inline uint64_t rect_tiles(const PrimGeo &g)
 
 // Per-input-triangle bookkeeping shared by the fan triangles it produces.
 struct FanCtx {
-    uint32_t t, b, j;
+    uint32_t t, b, j;               // virtual triangle, its batch, its slot in the batch
+    uint32_t src, kb, sid;          // scene-local triangle and batch, scene
     uint32_t slot1;                 // batch-local slot of fan triangle 1
     uint64_t pos0, pairs0, pos1, pairs1;
     uint32_t cnt0, cntp0, cnt1, cntp1;
@@ -566,8 +579,8 @@ __device__ __forceinline__ void fan_triangle(const GeomArgs &a, FanCtx &c, int k
         }
         return;
     }
-    uint32_t key = a.window_input ? c.t
-                                  : ((c.b << 15) + (k == 0 ? c.j : c.slot1 + (uint32_t)(k - 1)));
+    uint32_t key = a.window_input ? c.src
+                                  : ((c.kb << 15) + (k == 0 ? c.j : c.slot1 + (uint32_t)(k - 1)));
     uint64_t pos, pb;
     if (k == 0) {
         pos = c.pos0;
@@ -590,7 +603,7 @@ __device__ __forceinline__ void fan_triangle(const GeomArgs &a, FanCtx &c, int k
         x0[q] = swap ? v2[q] : v0[q];
         x2[q] = swap ? v0[q] : v2[q];
     }
-    write_prim<NV>(a, pos, key, g, w0, w1, w2, x0, v1, x2, pb);
+    write_prim<NV>(a, pos, key, c.src, c.sid, g, w0, w1, w2, x0, v1, x2, pb);
 }
 
 // Clipped triangles: polygon nodes live in per-thread arrays (rare path, kept out of line).
@@ -826,16 +839,43 @@ __device__ __forceinline__ int clip_one_plane(const GeomArgs &a, const Pos4 *P,
 // go to a queue (count pass) or are left to the queue kernel (emit pass).
 // CLIPQ = true: one queued triangle, clipped.
 template <int NV, bool EMIT, bool CLIPQ = false>
-__device__ void process_triangle(const GeomArgs &a, uint32_t t)
+__device__ void process_triangle(const GeomArgs &a0, uint32_t t)
 {
+    GeomArgs a = a0;
     FanCtx c;
     c.t = t;
     c.b = t >> 10;
     c.j = t & 1023u;
+    c.src = t;
+    c.kb = c.b;
+    c.sid = 0;
     c.slot1 = 0;
     c.pos0 = c.pairs0 = c.pos1 = c.pairs1 = 0;
     c.cnt0 = c.cntp0 = c.cnt1 = c.cntp1 = 0;
-    const uint32_t nbt = (c.b + 1 < a.batches) ? (uint32_t)BATCH : a.tris - c.b * (uint32_t)BATCH;
+    uint32_t ntris = a.tris, nbatches = a.batches;  // extent of this triangle's draw
+    bool live = true;
+    if (a0.nscenes) {
+        uint32_t sid = a0.batch_scene[c.b];
+        if (!CLIPQ)  // a 256-thread block lies inside one 1024-triangle batch
+            sid = __builtin_amdgcn_readfirstlane(sid);
+        const SceneDev &S = a0.scenes[sid];
+        apply_scene(a, S);
+        c.sid = sid;
+        c.kb = c.b - S.batch0;
+        c.src = t - S.batch0 * (uint32_t)BATCH;
+        ntris = S.tris;
+        nbatches = (S.tris + BATCH - 1) / BATCH;
+        live = c.src < S.tris;  // else padding up to the scene's last batch boundary
+    }
+    const uint32_t nbt = (c.kb + 1 < nbatches) ? (uint32_t)BATCH : ntris - c.kb * (uint32_t)BATCH;
+    if (!live) {
+        if (!EMIT) {
+            a.app[t] = 0;
+            a.grp[(size_t)c.b * 2048 + c.j] = make_uint2(0, 0);
+            a.grp[(size_t)c.b * 2048 + 1024 + c.j] = make_uint2(0, 0);
+        }
+        return;
+    }
     if (EMIT) {
         if (!a.window_input)
             c.slot1 = nbt + (a.app_ex[t] - a.app_ex[c.b * BATCH]);
@@ -856,7 +896,8 @@ __device__ void process_triangle(const GeomArgs &a, uint32_t t)
 
     if (a.window_input) {
         // IRasterizer seam: window-space vertices, no clip, transform or cull
-        int32_t i0 = a.indices[3 * t], i1 = a.indices[3 * t + 1], i2 = a.indices[3 * t + 2];
+        const uint64_t ib = 3ull * c.src;
+        int32_t i0 = a.indices[ib], i1 = a.indices[ib + 1], i2 = a.indices[ib + 2];
         if (i0 != -1) {
             if (i0 < 0 || i1 < 0 || i2 < 0 || (uint64_t)i0 >= a.wvert_count ||
                 (uint64_t)i1 >= a.wvert_count || (uint64_t)i2 >= a.wvert_count) {
@@ -884,7 +925,8 @@ __device__ void process_triangle(const GeomArgs &a, uint32_t t)
             }
         }
     } else {
-        int32_t ids[3] = {a.indices[3 * t], a.indices[3 * t + 1], a.indices[3 * t + 2]};
+        const uint64_t ib = 3ull * c.src;
+        int32_t ids[3] = {a.indices[ib], a.indices[ib + 1], a.indices[ib + 2]};
         bool bad = false;
         for (int q = 0; q < 3; ++q)
             if (ids[q] < 0 ||

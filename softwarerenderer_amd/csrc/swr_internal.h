@@ -32,6 +32,11 @@ hipError_t launch_ranges(const SortBuffers &sb, const unsigned long long *status
                          uint32_t tiles, hipStream_t s);
 uint32_t pair_sentinel(uint32_t tiles);
 
+// context access for the kernel files' own entry points (swr_api.cpp)
+hipStream_t context_stream(swr_context *c);
+int context_fail(swr_context *c, int code, const char *msg);
+bool context_is_device_ptr(const void *p);
+
 // raster (swr_raster.hip)
 bool raster_layout(const swr_pixel_shader_desc &ps, ParamLayout &L);
 hipError_t launch_raster(const swr_pixel_shader_desc &ps, const ParamLayout &L,

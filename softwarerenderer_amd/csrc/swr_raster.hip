@@ -39,6 +39,7 @@ __device__ inline float clamp01(float x) { return x > 0.0f ? (x < 1.0f ? x : 1.0
 // v[] holds one pixel's chained values in the Layout order: z, invw, avar..., pvarTemp...
 template <class L>
 struct ProgCoverage {
+    static constexpr int KEY_WORD = REC_KEY;
     struct State { bool hit; };
     __device__ static void load(State &s, const RasterArgs &, size_t) { s.hit = false; }
     __device__ static void shade(State &s, const RasterArgs &, const float *, uint32_t, int, int)
@@ -55,6 +56,7 @@ struct ProgCoverage {
 // RasterizerTest.cpp:38-48
 template <class L>
 struct ProgRGB {
+    static constexpr int KEY_WORD = REC_KEY;
     struct State { uint32_t c; bool hit; };
     __device__ static void load(State &s, const RasterArgs &, size_t) { s.hit = false; s.c = 0; }
     __device__ static void shade(State &s, const RasterArgs &, const float *v, uint32_t, int, int)
@@ -74,6 +76,7 @@ struct ProgRGB {
 
 template <class L>
 struct ProgDepthChecker {
+    static constexpr int KEY_WORD = REC_KEY;
     struct State { float depth; uint32_t color; bool hit; };
     __device__ static void load(State &s, const RasterArgs &a, size_t o)
     {
@@ -109,6 +112,7 @@ struct ProgDepthChecker {
 
 template <class L>
 struct ProgHash {
+    static constexpr int KEY_WORD = REC_KEY;
     struct State { uint32_t h, n, n0; };
     __device__ static void load(State &s, const RasterArgs &a, size_t o)
     {
@@ -148,6 +152,7 @@ struct ProgHash {
 
 template <class L>
 struct ProgLog {
+    static constexpr int KEY_WORD = REC_KEY;
     struct State { uint32_t n, n0; };
     __device__ static void load(State &s, const RasterArgs &a, size_t o)
     {
@@ -184,6 +189,37 @@ struct ProgLog {
     }
 };
 
+// Visibility buffer: the nearest primitive's source triangle (+1, 0 = none) and its depth,
+// under the depth-checker's `z < depth` test.
+template <class L>
+struct ProgVisibility {
+    static constexpr int KEY_WORD = REC_SRC;
+    struct State { float depth; uint32_t id; bool hit; };
+    __device__ static void load(State &s, const RasterArgs &a, size_t o)
+    {
+        s.depth = static_cast<const float *>(a.t1)[o];
+        s.id = 0;
+        s.hit = false;
+    }
+    __device__ static void shade(State &s, const RasterArgs &, const float *v, uint32_t src, int,
+                                 int)
+    {
+        const float z = v[L::Z];
+        if (z < s.depth) {
+            s.depth = z;
+            s.id = src + 1u;
+            s.hit = true;
+        }
+    }
+    __device__ static void store(const State &s, const RasterArgs &a, size_t o)
+    {
+        if (s.hit) {
+            static_cast<float *>(a.t1)[o] = s.depth;
+            static_cast<uint32_t *>(a.t0)[o] = s.id;
+        }
+    }
+};
+
 // ---- the tile kernel ----------------------------------------------------------------------
 __device__ inline void lds_sync()
 {
@@ -196,7 +232,7 @@ __device__ inline void lds_sync()
 }
 
 template <template <class> class ProgT, int NZ, int NW, int NA, int NP>
-__global__ void __launch_bounds__(256) k_raster(RasterArgs a)
+__global__ void __launch_bounds__(256) k_raster(RasterArgs a0)
 {
     using L = Layout<NZ, NW, NA, NP>;
     using Prog = ProgT<L>;
@@ -207,15 +243,30 @@ __global__ void __launch_bounds__(256) k_raster(RasterArgs a)
     __shared__ unsigned long long s_cov[4][GS];
 
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int tile = blockIdx.x * 4 + wv;
-    if (tile >= a.tiles)
+    const int tile = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv);
+    if (tile >= a0.tiles)
         return;
-    if (a.status[ST_OVERFLOW] | a.status[ST_ERROR])
+    if (a0.status[ST_OVERFLOW] | a0.status[ST_ERROR])
         return;
-    const uint32_t beg = a.ranges[2 * tile], end = a.ranges[2 * tile + 1];
+    const uint32_t beg = a0.ranges[2 * tile], end = a0.ranges[2 * tile + 1];
     if (beg >= end)
         return;
-    const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+    RasterArgs a = a0;
+    int lt = tile;  // tile index within its target
+    if (a0.nscenes) {
+        const SceneDev &S = a0.scenes[scene_of_tile(a0.scenes, a0.nscenes, (uint32_t)tile)];
+        a.width = S.width;
+        a.height = S.height;
+        a.pitch = S.pitch;
+        a.tiles_x = S.tiles_x;
+        a.sc_minx = S.sc_minx;
+        a.sc_maxx = S.sc_maxx;
+        a.t0 = S.t0;
+        a.t1 = S.t1;
+        a.t2 = S.t2;
+        lt = tile - (int)S.tile0;
+    }
+    const int tx = lt % a.tiles_x, ty = lt / a.tiles_x;
     const int bx = tx * TILE, by = ty * TILE;
     // phase B: lane = pixel (pi, pj)
     const int pi = lane & 7, pj = lane >> 3;
@@ -241,7 +292,7 @@ __global__ void __launch_bounds__(256) k_raster(RasterArgs a)
         const float *__restrict__ rec = a.prims + (size_t)(pv & PAIR_PRIM_MASK) * (size_t)a.stride;
         const int cls = (int)(pv >> 30);
         const uint32_t flags = act ? __float_as_uint(rec[REC_FLAGS]) : 0u;
-        const uint32_t key = act ? __float_as_uint(rec[REC_KEY]) : 0u;
+        const uint32_t key = act ? __float_as_uint(rec[Prog::KEY_WORD]) : 0u;
         const bool blk = (flags & FLAG_BLOCK) != 0;
 
         // Span: this row's part and [xl, xr) (Rasterizer.h:370-380, 397-407)
@@ -380,6 +431,11 @@ bool raster_layout(const swr_pixel_shader_desc &ps, ParamLayout &L)
         L.na = L.set_na = 3;
         L.np = L.set_np = 2;
         break;
+    case SWR_PS_VISIBILITY:
+        if (!ps.interpolate_z)
+            return false;
+        L.nz = 1;
+        break;
     case SWR_PS_HASH:
     case SWR_PS_LOG: {
         if (ps.avar_count < 0 || ps.avar_count > 16 || ps.pvar_count < 0 || ps.pvar_count > 16)
@@ -410,6 +466,8 @@ hipError_t launch_raster(const swr_pixel_shader_desc &ps, const ParamLayout &L,
         return launch<ProgRGB, 0, 0, 3, 0>(a, s);
     case SWR_PS_DEPTH_CHECKER:
         return launch<ProgDepthChecker, 1, 1, 3, 2>(a, s);
+    case SWR_PS_VISIBILITY:
+        return launch<ProgVisibility, 1, 0, 0, 0>(a, s);
     case SWR_PS_HASH:
         return L.na == 4 ? launch<ProgHash, 1, 1, 4, 4>(a, s) : launch<ProgHash, 1, 1, 16, 16>(a, s);
     case SWR_PS_LOG:

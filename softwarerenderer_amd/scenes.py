@@ -255,3 +255,52 @@ def clip_scene(ntri: int = 2000, seed: int = 3, width: int = 256, height: int = 
     idx = np.arange(n, dtype=np.int32)
     return Scene(verts, idx, np.eye(4, dtype=np.float32), width, height,
                  {"name": "clip", "ntri": ntri, "seed": seed})
+
+
+def icosphere(subdivisions: int = 6):
+    """Unit icosphere: (vertices [N, 3] float64, triangles [T, 3] int32), outward faces
+    counter-clockwise.  Subdivision s gives 10 * 4^s + 2 vertices and 20 * 4^s triangles
+    (s = 6: 40,962 and 81,920)."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t),
+         (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+         (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+         (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    verts = [np.asarray(p, float) / np.linalg.norm(p) for p in v]
+    faces = np.asarray(f, np.int64)
+    for _ in range(subdivisions):
+        cache = {}
+
+        def mid(a, b):
+            k = (min(a, b), max(a, b))
+            if k not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[k] = len(verts) - 1
+            return cache[k]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = np.asarray(nf, np.int64)
+    V = np.asarray(verts)
+    nrm = np.cross(V[faces[:, 1]] - V[faces[:, 0]], V[faces[:, 2]] - V[faces[:, 0]])
+    assert np.all(np.einsum("ij,ij->i", nrm, V[faces].sum(1)) > 0), "icosphere winding"
+    return V, faces.astype(np.int32)
+
+
+def orbit_cameras(views: int, width: int, height: int, radius: float = 3.0, seed: int = 0):
+    """MVPs [views, 4, 4] (float64) of cameras uniform on a sphere of `radius` looking at
+    the origin: 60-degree fovy, near 0.1, far 10."""
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(views, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    proj = perspective(60.0, width / height, 0.1, 10.0)
+    out = np.zeros((views, 4, 4))
+    for i in range(views):
+        eye = radius * d[i]
+        up = (0.0, 1.0, 0.0) if abs(d[i][1]) < 0.99 else (1.0, 0.0, 0.0)
+        out[i] = proj @ lookat(eye, (0, 0, 0), up)
+    return out
