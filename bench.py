@@ -4,7 +4,12 @@ One step = one drawElements of the whole frame (vertex program, clip, cull, setu
 binning, raster, depth-tested checker shading into RGBA8 + fp32 depth) preceded by the
 target clear, with vertices and indices resident in HBM.  Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--copy-probe]
+The line also carries the differentiation workload (64 views at 512x512 of a 40,962-vertex
+icosphere, forward + backward of an L2 image loss): views per second of the batched step
+(one visibility draw for all views, captured in a HIP graph) against the same work issued
+one view at a time.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-diff] [--copy-probe]
 """
 from __future__ import annotations
 
@@ -34,7 +39,101 @@ def parse():
     p.add_argument("--cpu-runs", type=int, default=3)
     p.add_argument("--copy-probe", action="store_true",
                    help="run one plain 256 MiB device copy first (counter calibration)")
+    p.add_argument("--no-diff", action="store_true", help="skip the differentiation workload")
+    p.add_argument("--diff-views", type=int, default=64)
+    p.add_argument("--diff-size", type=int, default=512)
+    p.add_argument("--diff-subdiv", type=int, default=6)
     return p.parse_args()
+
+
+def measured_traffic(workload: str):
+    """HBM bytes per raster launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded
+    in profiles/traffic.json (scaled by the copy-probe factors recorded with them)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("workload") != workload:
+        return None
+    return t.get("raster_bytes")
+
+
+def diff_bench(args, dev):
+    """Batched forward + backward over all views (graph replay) against one view at a
+    time, on the same renderer code."""
+    import torch
+
+    from softwarerenderer_amd import scenes
+    from softwarerenderer_amd.diff import DiffRenderer
+
+    V, S_, sub = args.diff_views, args.diff_size, args.diff_subdiv
+    Vt, F = scenes.icosphere(sub)
+    N = Vt.shape[0]
+    rng = np.random.default_rng(0)
+    mvps = torch.from_numpy(scenes.orbit_cameras(V, S_, S_, seed=0)).float().to(dev)
+    pos = torch.from_numpy(Vt).float().to(dev)
+    col = torch.from_numpy(rng.random((N, 3))).float().to(dev)
+    hom = torch.cat([pos, torch.ones_like(pos[:, :1])], 1)
+    tp = pos + torch.from_numpy(rng.normal(0, 0.02, Vt.shape)).float().to(dev)
+    tc = col + torch.from_numpy(rng.normal(0, 0.1, (N, 3))).float().to(dev)
+    thom = torch.cat([tp, torch.ones_like(tp[:, :1])], 1)
+
+    r = DiffRenderer(F, N, V, S_, S_)
+    target = r.forward(torch.einsum("vij,nj->vni", mvps, thom).contiguous(), tc, sync=True).clone()
+
+    def step(rr, m, tgt):
+        clip = torch.einsum("vij,nj->vni", m, hom).contiguous()
+        img = rr.forward(clip, col)
+        g = 2.0 * (img - tgt)
+        gclip, gattr = rr.backward(g)
+        gpos = torch.einsum("vni,vij->nj", gclip, m)[:, :3]
+        return gpos, gattr
+
+    s = torch.cuda.Stream(dev)
+    s.wait_stream(torch.cuda.current_stream(dev))
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        step(r, mvps, target)
+        with torch.cuda.graph(graph, stream=s):
+            step(r, mvps, target)
+    torch.cuda.current_stream(dev).wait_stream(s)
+    torch.cuda.synchronize()
+    reps = 10
+    graph.replay()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        graph.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    r.check()
+    t_batch = e0.elapsed_time(e1) / reps
+
+    # the same work one view at a time: a one-view renderer, eager launches per view
+    r1 = DiffRenderer(F, N, 1, S_, S_)
+    for v in range(2):
+        step(r1, mvps[v:v + 1], target[v:v + 1])
+    torch.cuda.synchronize()
+    e0.record()
+    for v in range(V):
+        step(r1, mvps[v:v + 1], target[v:v + 1])
+    e1.record()
+    torch.cuda.synchronize()
+    r1.check()
+    t_plain = e0.elapsed_time(e1)
+    r.close()
+    r1.close()
+    return {"workload": f"icosphere-s{sub}-{V}views-{S_}px-fwd+bwd", "vertices": N,
+            "triangles": int(F.shape[0]), "views": V, "size": S_,
+            "views_per_s": round(V / (t_batch * 1e-3), 1),
+            "plain_views_per_s": round(V / (t_plain * 1e-3), 1),
+            "speedup": round(t_plain / t_batch, 2),
+            "batched_ms_per_step": round(t_batch, 4), "plain_ms_per_step": round(t_plain, 4),
+            "batched": "one visibility draw for all views + forward + backward, HIP graph replay",
+            "plain": "per view: visibility draw + forward + backward, eager launches"}
 
 
 def main():
@@ -151,7 +250,8 @@ def main():
                      "ranges": round(stage[4], 4), "raster": round(stage[5], 4)},
         "roofline": {"bound": "hbm", "kernel": "k_raster<DepthChecker>",
                      "achieved": round(raster_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(raster_gbs / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(raster_gbs / HBM_PEAK_GBS, 5),
+                     "traffic": measured_traffic("soup-1M-1080p-block"),
                      "algorithmic_bytes": raster_bytes},
         "frame": {"algorithmic_bytes": frame_bytes, "kernel_ms": round(kern_ms, 4),
                   "achieved_GBs": round(frame_bytes / (kern_ms * 1e-3) / 1e9, 2)
@@ -159,6 +259,8 @@ def main():
     }
     r.close()
 
+    if not args.no_diff:
+        out["diff"] = diff_bench(args, dev)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_runs)
     print(json.dumps(out))

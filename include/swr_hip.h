@@ -252,10 +252,15 @@ int  swr_batch_destroy(swr_batch *batch);
  * All buffers are device memory; both calls only enqueue kernels on the context stream
  * (they can be captured into a HIP graph).  The backward pass recomputes everything else
  * from ids, depth, bary and color. */
+#define SWR_DIFF_CULLED 1          /* every visible triangle faces the same way (the visibility
+                                      draw culled one winding): pixels of two edge-adjacent
+                                      triangles are never separated by a silhouette */
 typedef struct swr_diff_desc {
     int32_t views, width, height;
     int32_t vertex_count, triangle_count;
     int32_t channels;              /* 1..4 */
+    int32_t flags;                 /* SWR_DIFF_CULLED                                      */
+    int32_t reserved;
     const float    *clip;          /* [views][vertex_count][4] clip-space x, y, z, w       */
     const int32_t  *triangles;     /* [triangle_count][3]                                  */
     const int32_t  *adjacency;     /* [triangle_count][3]: the triangle across the edge

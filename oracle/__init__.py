@@ -216,7 +216,7 @@ class CpuRenderer:
         return out[: min(n, max_out)], n
 
 
-def diff_reference(clip, tris, adj, attr, ids, depth, grad_image=None):
+def diff_reference(clip, tris, adj, attr, ids, depth, grad_image=None, culled=True):
     """FP64 reference of swr_diff_forward / swr_diff_backward (oracle/swr_diff_ref.c).
 
     clip [V,N,4] f32, tris [T,3] i32, adj [T,3] i32, attr [N,C] f32, ids [V,H,W] u32,
@@ -225,7 +225,7 @@ def diff_reference(clip, tris, adj, attr, ids, depth, grad_image=None):
     lib = C.CDLL(ORACLE_SO)
     f = lib.swr_oracle_diff
     f.restype = C.c_int
-    f.argtypes = [C.c_int] * 6 + [C.c_void_p] * 10
+    f.argtypes = [C.c_int] * 7 + [C.c_void_p] * 10
     clip = np.ascontiguousarray(clip, np.float32)
     tris = np.ascontiguousarray(tris, np.int32)
     adj = np.ascontiguousarray(adj, np.int32)
@@ -240,7 +240,7 @@ def diff_reference(clip, tris, adj, attr, ids, depth, grad_image=None):
         gi = np.ascontiguousarray(grad_image, np.float32)
         gclip = np.zeros((V, N, 4), np.float64)
         gattr = np.zeros((N, Cc), np.float64)
-    rc = f(V, W, H, N, tris.shape[0], Cc, clip.ctypes.data, tris.ctypes.data, adj.ctypes.data,
+    rc = f(V, W, H, N, tris.shape[0], Cc, int(bool(culled)), clip.ctypes.data, tris.ctypes.data, adj.ctypes.data,
            attr.ctypes.data, ids.ctypes.data, depth.ctypes.data, image.ctypes.data,
            None if gi is None else gi.ctypes.data, None if gclip is None else gclip.ctypes.data,
            None if gattr is None else gattr.ctypes.data)

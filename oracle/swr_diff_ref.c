@@ -25,6 +25,7 @@ typedef struct { double x, y; } d2;
 
 typedef struct {
     int V, W, H, N, T, C;
+    int culled;
     const float *clip;
     const int32_t *tris, *adj;
     const float *attr;
@@ -119,6 +120,12 @@ static pair_t eval_pair(const dctx *d, int view, int px, int py, int qx, int qy)
     int cur = (int)(pf ? a : b) - 1;
     if (cur < 0 || cur >= d->T)
         return r;
+    if (d->culled && a != 0 && b != 0) {
+        /* two visible, edge-adjacent triangles of a culled draw: an interior edge */
+        int other = (int)(pf ? b : a) - 1;
+        if (d->adj[3 * cur] == other || d->adj[3 * cur + 1] == other || d->adj[3 * cur + 2] == other)
+            return r;
+    }
     int horiz = py == qy;
     f2 F = {(float)(pf ? px : qx) + 0.5f, (float)(pf ? py : qy) + 0.5f};
     f2 Q = {(float)(pf ? qx : px) + 0.5f, (float)(pf ? qy : py) + 0.5f};
@@ -347,13 +354,14 @@ static void add_window_grad(const dctx *d, int view, int v, double gx, double gy
  * a (1/2 - t) (C_q - C_p); as the other pixel, a (t - 1/2) (E - C_p) with E the silhouette
  * triangle's colour where the segment crosses the edge.  Returns 0, or -1 for channels outside 1..4. */
 int swr_oracle_diff(int views, int width, int height, int nverts, int ntris, int channels,
-                    const float *clip, const int32_t *tris, const int32_t *adj, const float *attr,
+                    int culled, const float *clip, const int32_t *tris, const int32_t *adj, const float *attr,
                     const uint32_t *ids, const float *depth, double *image,
                     const float *grad_image, double *gclip, double *gattr)
 {
     if (channels < 1 || channels > 4)
         return -1;
-    dctx d = {views, width, height, nverts, ntris, channels, clip, tris, adj, attr, ids, depth};
+    dctx d = {views, width, height, nverts, ntris, channels, culled, clip, tris, adj, attr, ids,
+              depth};
     const int C = channels;
     for (int view = 0; view < views; ++view) {
         for (int py = 0; py < height; ++py) {

@@ -61,7 +61,8 @@ class DiffDesc(C.Structure):
     """swr_diff_desc: differentiable shading over a visibility buffer."""
     _fields_ = [("views", C.c_int32), ("width", C.c_int32), ("height", C.c_int32),
                 ("vertex_count", C.c_int32), ("triangle_count", C.c_int32),
-                ("channels", C.c_int32), ("clip", C.c_void_p), ("triangles", C.c_void_p),
+                ("channels", C.c_int32), ("flags", C.c_int32), ("reserved", C.c_int32),
+                ("clip", C.c_void_p), ("triangles", C.c_void_p),
                 ("adjacency", C.c_void_p), ("attributes", C.c_void_p), ("ids", C.c_void_p),
                 ("depth", C.c_void_p), ("bary", C.c_void_p), ("color", C.c_void_p),
                 ("image", C.c_void_p)]

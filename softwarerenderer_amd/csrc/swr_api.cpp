@@ -64,7 +64,8 @@ struct swr_context {
         ranges, hist, hist_ex, hist_tmp, status, tgt[3], log, logcount,
         bigq,       // per-primitive first pair index
         clipq,      // triangles queued for the clipper
-        clipq_cnt;  // its per-region counters
+        clipq_cnt,  // its per-region counters
+        zscratch;   // zeroed scratch of the differentiation kernels
     uint64_t prim_cap = 0, pair_cap = 0;
     unsigned long long *status_host = nullptr;
 
@@ -584,6 +585,20 @@ int context_fail(swr_context *c, int code, const char *msg) { return fail(c, cod
 
 bool context_is_device_ptr(const void *p) { return is_device_ptr(p); }
 
+void *context_zeroed_scratch(swr_context *c, size_t bytes)
+{
+    if (c->zscratch.bytes < bytes) {
+        hipError_t e = ensure(c, c->zscratch, bytes);
+        if (e == hipSuccess)  // ordered before the kernels that use it on this stream
+            e = hipMemsetAsync(c->zscratch.p, 0, c->zscratch.bytes, c->stream);
+        if (e != hipSuccess) {
+            fail(c, SWR_ERR_HIP, "scratch allocation failed: %s", hipGetErrorString(e));
+            return nullptr;
+        }
+    }
+    return c->zscratch.p;
+}
+
 }  // namespace swr
 
 // ---- C ABI ------------------------------------------------------------------------------
@@ -632,7 +647,7 @@ int swr_destroy(swr_context *c)
                       &c->scan_tmp, &c->prims, &c->pk_a, &c->pv_a, &c->pk_b, &c->pv_b,
                       &c->ranges, &c->hist, &c->hist_ex, &c->hist_tmp, &c->status, &c->tgt[0],
                       &c->tgt[1], &c->tgt[2], &c->log, &c->logcount, &c->bigq, &c->clipq,
-                      &c->clipq_cnt};
+                      &c->clipq_cnt, &c->zscratch};
     for (DevBuf *b : bufs)
         if (b->p)
             (void)hipFree(b->p);

@@ -44,6 +44,8 @@ namespace swr {
 
 struct DiffArgs {
     int V, W, H, N, T;
+    int culled;    // every visible triangle faces the same way (the visibility draw culled)
+    int tiles_x, tiles_per_view;  // 16x16-pixel tiles, one per workgroup
     float sx, sy;  // W/2, H/2: NDC -> window scale (viewport (0, 0, W, H))
     const float4 *clip;
     const int32_t *tris;
@@ -56,8 +58,40 @@ struct DiffArgs {
     float *image;
     const float *gimage;
     float *gclip;
-    float *gattr;
+    float *gattr;  // per-view partial attribute gradients [V][N][C], summed afterwards
+    float *gattr_out;
 };
+
+// Pixel of this thread: a workgroup of 256 threads covers one 16x16 tile of one view, so a
+// wave (16x4 pixels) reads its neighbours, triangles and vertices from a small area.
+struct PixelIdx {
+    int view, px, py;
+    size_t i;
+};
+
+// Workgroup b runs on XCD b % 8 (round-robin dispatch).  Renumber so that each XCD works
+// through one contiguous run of (view, tile) blocks: the views an XCD is working on at any
+// moment have their vertex data and gradient slices in that XCD's L2, where the float
+// atomics of the backward pass resolve.
+__device__ inline int xcd_order(unsigned b, unsigned n)
+{
+    const unsigned q = n / 8u, r = n % 8u, x = b % 8u, i = b / 8u;
+    return (int)(x * q + (x < r ? x : r) + i);
+}
+
+__device__ inline bool pixel_of_thread(const DiffArgs &d, PixelIdx &q)
+{
+    const int b = xcd_order(blockIdx.x, gridDim.x);
+    q.view = b / d.tiles_per_view;
+    const int t = b - q.view * d.tiles_per_view;
+    const int ty = t / d.tiles_x, tx = t - ty * d.tiles_x;
+    q.px = tx * 16 + (int)(threadIdx.x & 15);
+    q.py = ty * 16 + (int)(threadIdx.x >> 4);
+    if (q.px >= d.W || q.py >= d.H)
+        return false;
+    q.i = ((size_t)q.view * d.H + q.py) * d.W + q.px;
+    return true;
+}
 
 // clip (x, y, w) of a vertex, in fp64
 struct H3 {
@@ -199,6 +233,13 @@ __device__ inline Pair eval_pair(const DiffArgs &d, int view, int px, int py, in
     int cur = (int)(pf ? a : b) - 1;
     if (cur < 0 || cur >= d.T)
         return r;
+    if (d.culled && a != 0 && b != 0) {
+        // two visible triangles of a culled draw that share an edge face the same way: the
+        // edge between them is interior
+        const int other = (int)(pf ? b : a) - 1;
+        if (d.adj[3 * cur] == other || d.adj[3 * cur + 1] == other || d.adj[3 * cur + 2] == other)
+            return r;
+    }
     const bool horiz = py == qy;
     const float2 F = pf ? make_float2(px + 0.5f, py + 0.5f) : make_float2(qx + 0.5f, qy + 0.5f);
     const float2 Q = pf ? make_float2(qx + 0.5f, qy + 0.5f) : make_float2(px + 0.5f, py + 0.5f);
@@ -356,7 +397,7 @@ __device__ inline void bary_backward(const DiffArgs &d, int view, const Bary &B,
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             u[m] += gcol[c] * d.attr[(size_t)B.v[m] * C + c];
-            atomicAdd(d.gattr + (size_t)B.v[m] * C + c, (float)(B.b[m] * gcol[c]));
+            atomicAdd(d.gattr + ((size_t)view * d.N + B.v[m]) * C + c, (float)(B.b[m] * gcol[c]));
         }
     }
     const double ub = u[0] * B.b[0] + u[1] * B.b[1] + u[2] * B.b[2];
@@ -386,13 +427,11 @@ __device__ inline void bary_backward(const DiffArgs &d, int view, const Bary &B,
 template <int C>
 __global__ void __launch_bounds__(256) k_diff_interp(DiffArgs d)
 {
-    const size_t HW = (size_t)d.W * d.H;
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= HW * d.V)
+    PixelIdx pi;
+    if (!pixel_of_thread(d, pi))
         return;
-    const int view = (int)(i / HW);
-    const int rem = (int)(i - (size_t)view * HW);
-    const int py = rem / d.W, px = rem - py * d.W;
+    const size_t i = pi.i;
+    const int view = pi.view, px = pi.px, py = pi.py;
     const uint32_t id = d.ids[i];
     double col[C];
 #pragma unroll
@@ -428,12 +467,11 @@ template <int C>
 __global__ void __launch_bounds__(256) k_diff_aa(DiffArgs d)
 {
     const size_t HW = (size_t)d.W * d.H;
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= HW * d.V)
+    PixelIdx pi;
+    if (!pixel_of_thread(d, pi))
         return;
-    const int view = (int)(i / HW);
-    const int rem = (int)(i - (size_t)view * HW);
-    const int py = rem / d.W, px = rem - py * d.W;
+    const size_t i = pi.i;
+    const int view = pi.view, px = pi.px, py = pi.py;
     double cp[C], out[C];
 #pragma unroll
     for (int c = 0; c < C; ++c)
@@ -485,12 +523,11 @@ template <int C>
 __global__ void __launch_bounds__(256) k_diff_backward(DiffArgs d)
 {
     const size_t HW = (size_t)d.W * d.H;
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= HW * d.V)
+    PixelIdx pi;
+    if (!pixel_of_thread(d, pi))
         return;
-    const int view = (int)(i / HW);
-    const int rem = (int)(i - (size_t)view * HW);
-    const int py = rem / d.W, px = rem - py * d.W;
+    const size_t i = pi.i;
+    const int view = pi.view, px = pi.px, py = pi.py;
     double gp[C], gc[C], cp[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
@@ -565,11 +602,27 @@ __global__ void __launch_bounds__(256) k_diff_backward(DiffArgs d)
     bary_backward<C>(d, view, B, gc);
 }
 
+// grad_attributes += sum over views of the partials; the partials are cleared for the next
+// call (they start at zero when the buffer is created).
+__global__ void __launch_bounds__(256) k_diff_reduce_attr(DiffArgs d, int C)
+{
+    const size_t n = (size_t)d.N * C;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (size_t)gridDim.x * blockDim.x) {
+        float s = 0.0f;
+        for (int v = 0; v < d.V; ++v) {
+            float *p = d.gattr + (size_t)v * n + k;
+            s += *p;
+            *p = 0.0f;
+        }
+        d.gattr_out[k] += s;
+    }
+}
+
 template <int C>
 static hipError_t launch_diff(const DiffArgs &d, int which, hipStream_t s)
 {
-    const size_t n = (size_t)d.W * d.H * d.V;
-    const uint32_t blocks = (uint32_t)((n + 255) / 256);
+    const uint32_t blocks = (uint32_t)((size_t)d.tiles_per_view * d.V);
     if (!blocks)
         return hipSuccess;
     if (which == 0) {
@@ -577,6 +630,10 @@ static hipError_t launch_diff(const DiffArgs &d, int which, hipStream_t s)
         hipLaunchKernelGGL(k_diff_aa<C>, dim3(blocks), dim3(256), 0, s, d);
     } else {
         hipLaunchKernelGGL(k_diff_backward<C>, dim3(blocks), dim3(256), 0, s, d);
+        const size_t n = (size_t)d.N * C;
+        const uint32_t rb = (uint32_t)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096);
+        if (rb)
+            hipLaunchKernelGGL(k_diff_reduce_attr, dim3(rb), dim3(256), 0, s, d, C);
     }
     return hipGetLastError();
 }
@@ -610,6 +667,11 @@ static int diff_run(swr_context *c, const swr_diff_desc *dd, const float *gimage
     d.H = q.height;
     d.N = q.vertex_count;
     d.T = q.triangle_count;
+    d.culled = (q.flags & SWR_DIFF_CULLED) ? 1 : 0;
+    d.tiles_x = (q.width + 15) / 16;
+    d.tiles_per_view = d.tiles_x * ((q.height + 15) / 16);
+    if ((size_t)d.tiles_per_view * d.V > 0x7FFFFFFFull)
+        return context_fail(c, SWR_ERR_CAPACITY, "too many pixels for one swr_diff call");
     d.sx = (float)q.width / 2.0f;
     d.sy = (float)q.height / 2.0f;
     d.clip = reinterpret_cast<const float4 *>(q.clip);
@@ -623,7 +685,14 @@ static int diff_run(swr_context *c, const swr_diff_desc *dd, const float *gimage
     d.image = q.image;
     d.gimage = gimage;
     d.gclip = gclip;
-    d.gattr = gattr;
+    d.gattr = nullptr;
+    d.gattr_out = gattr;
+    if (which == 1) {
+        // per-view partials: views do not contend on one vertex's attribute gradient
+        d.gattr = (float *)context_zeroed_scratch(c, (size_t)d.V * d.N * q.channels * 4);
+        if (!d.gattr)
+            return SWR_ERR_HIP;
+    }
     hipStream_t s = context_stream(c);
     hipError_t e;
     switch (q.channels) {

@@ -835,6 +835,133 @@ __device__ __forceinline__ int clip_one_plane(const GeomArgs &a, const Pos4 *P,
     return quad ? 2 : 1;
 }
 
+// One Sutherland-Hodgman pass (PolyClipper.cpp:52-78) of a convex polygon held in
+// registers (n <= NIN vertices) against plane p, in the clipper's output order: for
+// i = 0..n-1, vertex i if it is inside, then the intersection with edge (i, i+1 mod n) if the
+// sign changes there.  Output slot j takes the candidate of rank j through select chains, so
+// no register array is indexed by a runtime value and nothing spills to scratch.  Returns
+// the output count, or -1 when a vertex lies on the plane (the queue clipper handles the
+// duplicate vertices that creates).
+template <int NIN, int NV, bool EMIT>
+__device__ __forceinline__ int clip_poly_reg(const GeomArgs &a, int p, int n, const Pos4 *P,
+                                             const float (*Vv)[NV], Pos4 *Po, float (*Vo)[NV])
+{
+    constexpr int NOUT = NIN + 1;
+    float dp[NIN];
+    bool in[NIN];
+    bool onplane = false;
+#pragma unroll
+    for (int i = 0; i < NIN; ++i) {
+        dp[i] = plane_dot(p, P[i]);
+        onplane = onplane || (i < n && dp[i] == 0.0f);
+        in[i] = dp[i] > 0.0f;
+    }
+    if (onplane)
+        return -1;
+    CVert<NV> cand[2 * NIN];
+    bool emit[2 * NIN];
+#pragma unroll
+    for (int i = 0; i < NIN; ++i) {
+        const bool wrap = i + 1 >= n;
+        const int nx = i + 1 < NIN ? i + 1 : 0;
+        const Pos4 Pn = wrap ? P[0] : P[nx];
+        const float dn = wrap ? dp[0] : dp[nx];
+        const bool innx = wrap ? in[0] : in[nx];
+        cand[2 * i].p = P[i];
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            cand[2 * i].v[k] = Vv[i][k];
+        emit[2 * i] = i < n && in[i];
+        // I(v_i, v_next): t = dcur < 0 ? dprev / (dprev - dcur) : -dprev / (dcur - dprev)
+        const float dpr = dp[i];
+        const float t = dn < 0 ? dpr / (dpr - dn) : -dpr / (dn - dpr);
+        CVert<NV> &x = cand[2 * i + 1];
+        x.p.x = lerp1(P[i].x, Pn.x, t);
+        x.p.y = lerp1(P[i].y, Pn.y, t);
+        x.p.z = lerp1(P[i].z, Pn.z, t);
+        x.p.w = lerp1(P[i].w, Pn.w, t);
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            const bool lerped = k < a.lay.na ? (k < a.vs_na) : ((k - a.lay.na) < a.vs_np);
+            const float vn = wrap ? Vv[0][k] : Vv[nx][k];
+            x.v[k] = (EMIT && lerped) ? lerp1(Vv[i][k], vn, t) : 0.0f;
+        }
+        emit[2 * i + 1] = i < n && in[i] != innx;
+    }
+    int rank[2 * NIN];
+    int m = 0;
+#pragma unroll
+    for (int c = 0; c < 2 * NIN; ++c) {
+        rank[c] = m;
+        m += emit[c] ? 1 : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < NOUT; ++j) {
+        Pos4 q = P[0];
+        float qv[NV > 0 ? NV : 1];
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            qv[k] = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 2 * NIN; ++c) {
+            const bool sel = emit[c] && rank[c] == j;
+            q.x = sel ? cand[c].p.x : q.x;
+            q.y = sel ? cand[c].p.y : q.y;
+            q.z = sel ? cand[c].p.z : q.z;
+            q.w = sel ? cand[c].p.w : q.w;
+#pragma unroll
+            for (int k = 0; k < NV; ++k)
+                qv[k] = sel ? cand[c].v[k] : qv[k];
+        }
+        Po[j] = q;
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            Vo[j][k] = qv[k];
+    }
+    return m;
+}
+
+// Exactly two planes selected and finite coordinates: clip in registers, plane by plane in
+// the order of VertexProcessor.cpp:237-242 (lowest bit first).  The polygon has at most 5
+// vertices.  Writes the window-space polygon to W / V and returns the fan count (0 when
+// fully clipped), or -1 to leave the triangle to the queue clipper.
+template <int NV, bool EMIT>
+__device__ __forceinline__ int clip_two_planes(const GeomArgs &a, int cm, const Pos4 *P,
+                                               const float (*var)[NV], WinV *W, float (*V)[NV])
+{
+    const int p1 = __builtin_ctz(cm);
+    const int rest = cm & (cm - 1);
+    if (!rest || (rest & (rest - 1)))
+        return -1;
+    const int p2 = __builtin_ctz(rest);
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+        if (!(finitef(P[q].x) && finitef(P[q].y) && finitef(P[q].z) && finitef(P[q].w)))
+            return -1;
+    Pos4 P1[4];
+    float V1[4][NV > 0 ? NV : 1];
+    const int n1 = clip_poly_reg<3, NV, EMIT>(a, p1, 3, P, var, P1, V1);
+    if (n1 < 0)
+        return -1;
+    if (n1 < 3)
+        return 0;  // clipToPlane returns fewer than 3 vertices: fully clipped
+    Pos4 P2[5];
+    float V2[5][NV > 0 ? NV : 1];
+    const int n2 = clip_poly_reg<4, NV, EMIT>(a, p2, n1, P1, V1, P2, V2);
+    if (n2 < 0)
+        return -1;
+    if (n2 < 3)
+        return 0;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        W[i] = to_window(a, P2[i]);
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            V[i][k] = V2[i][k];
+    }
+    return n2 - 2;
+}
+
 // CLIPQ = false: the main pass over all input triangles; triangles that need the clipper
 // go to a queue (count pass) or are left to the queue kernel (emit pass).
 // CLIPQ = true: one queued triangle, clipped.
@@ -888,9 +1015,9 @@ __device__ void process_triangle(const GeomArgs &a0, uint32_t t)
     }
     constexpr int NVS = NV > 0 ? NV : 1;
     float var3[3][NVS];
-    // the polygon handed to the fan loop below: 3 or 4 window-space vertices
-    WinV W[4];
-    float V[4][NVS];
+    // the polygon handed to the fan loop below: 3 to 5 window-space vertices
+    WinV W[5];
+    float V[5][NVS];
     int nfan = 0;
     bool cull = true;
 
@@ -944,7 +1071,18 @@ __device__ void process_triangle(const GeomArgs &a0, uint32_t t)
             int cm = clip_mask(P[0]) | clip_mask(P[1]) | clip_mask(P[2]);
             float dp1[3];
             if (CLIPQ) {
-                clip_path<NVS, EMIT>(a, c, cm, P, var3);
+                // two planes in registers (the common queued case); anything else through
+                // the general clipper
+                int r2 = -1;
+                if constexpr (NVS <= 8)
+                    r2 = clip_two_planes<NVS, EMIT>(a, cm, P, var3, W, V);
+                if (r2 >= 0) {
+                    nfan = r2;
+                    if (!EMIT)
+                        a.app[t] = nfan > 1 ? (uint32_t)(nfan - 1) : 0u;
+                } else {
+                    clip_path<NVS, EMIT>(a, c, cm, P, var3);
+                }
             } else if (cm == 0) {
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
@@ -970,17 +1108,17 @@ __device__ void process_triangle(const GeomArgs &a0, uint32_t t)
                 a.app[t] = nfan == 2 ? 1u : 0u;
         }
     }
-    // fan triangles (p0, p1, p2) and (p0, p2, p3), VertexProcessor.cpp:254-261; one call
-    // site so the vertex arrays stay in registers
+    // fan triangles (p0, p[k+1], p[k+2]), VertexProcessor.cpp:254-261; one call site so the
+    // vertex arrays stay in registers
 #pragma unroll 1
     for (int k = 0; k < nfan; ++k) {
-        const WinV wb = k ? W[2] : W[1];
-        const WinV wc = k ? W[3] : W[2];
+        const WinV wb = k == 0 ? W[1] : (k == 1 ? W[2] : W[3]);
+        const WinV wc = k == 0 ? W[2] : (k == 1 ? W[3] : W[4]);
         float vb[NVS], vc[NVS];
 #pragma unroll
         for (int q = 0; q < NVS; ++q) {
-            vb[q] = k ? V[2][q] : V[1][q];
-            vc[q] = k ? V[3][q] : V[2][q];
+            vb[q] = k == 0 ? V[1][q] : (k == 1 ? V[2][q] : V[3][q]);
+            vc[q] = k == 0 ? V[2][q] : (k == 1 ? V[3][q] : V[4][q]);
         }
         fan_triangle<NV, EMIT>(a, c, k, W[0], wb, wc, V[0], vb, vc, cull);
     }
@@ -1033,13 +1171,20 @@ template <int NV, bool EMIT>
 __device__ void run_clip_queue(const GeomArgs &a)
 {
     __shared__ uint32_t s_off[CLIPQ_REGIONS + 1];
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (uint32_t r = 0; r < CLIPQ_REGIONS; ++r) {
-            s_off[r] = acc;
-            acc += a.clipq_cnt[r * CLIPQ_STRIDE];
+    static_assert(CLIPQ_REGIONS == 64, "one region per lane of the first wave");
+    if (threadIdx.x < 64) {
+        // the 64 counters load in parallel, one per lane; an inclusive scan across the wave
+        const uint32_t lane = threadIdx.x;
+        const uint32_t cnt = a.clipq_cnt[lane * CLIPQ_STRIDE];
+        uint32_t incl = cnt;
+#pragma unroll
+        for (int sh = 1; sh < 64; sh <<= 1) {
+            const uint32_t o = __shfl_up(incl, sh, 64);
+            incl += lane >= (uint32_t)sh ? o : 0u;
         }
-        s_off[CLIPQ_REGIONS] = acc;
+        s_off[lane] = incl - cnt;
+        if (lane == 63)
+            s_off[CLIPQ_REGIONS] = incl;
     }
     __syncthreads();
     const uint32_t n = s_off[CLIPQ_REGIONS], cap = clipq_region_cap(a.tris);

@@ -36,6 +36,9 @@ uint32_t pair_sentinel(uint32_t tiles);
 hipStream_t context_stream(swr_context *c);
 int context_fail(swr_context *c, int code, const char *msg);
 bool context_is_device_ptr(const void *p);
+// A device buffer of at least `bytes` owned by the context, zero when first handed out at
+// this size; its users leave it zeroed.  nullptr (and the context's error set) on failure.
+void *context_zeroed_scratch(swr_context *c, size_t bytes);
 
 // raster (swr_raster.hip)
 bool raster_layout(const swr_pixel_shader_desc &ps, ParamLayout &L);

@@ -221,6 +221,19 @@ struct ProgVisibility {
 };
 
 // ---- the tile kernel ----------------------------------------------------------------------
+// Workgroups are handed to the 8 XCDs round-robin (workgroup b runs on XCD b % 8), and
+// each XCD has its own L2.  Renumber so that XCD x works through one contiguous run of
+// tile blocks (a band of tile rows): primitive records shared by neighbouring tiles are
+// then fetched into one L2 instead of up to eight.  A bijection on [0, n).
+constexpr unsigned XCDS = 8;
+
+__device__ inline unsigned xcd_block(unsigned b, unsigned n)
+{
+    const unsigned q = n / XCDS, r = n % XCDS;
+    const unsigned x = b % XCDS, i = b / XCDS;
+    return x * q + (x < r ? x : r) + i;
+}
+
 __device__ inline void lds_sync()
 {
     // LDS operations of one wave complete in issue order; keep the compiler from moving
@@ -243,7 +256,7 @@ __global__ void __launch_bounds__(256) k_raster(RasterArgs a0)
     __shared__ unsigned long long s_cov[4][GS];
 
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int tile = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv);
+    const int tile = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * 4 + wv);
     if (tile >= a0.tiles)
         return;
     if (a0.status[ST_OVERFLOW] | a0.status[ST_ERROR])

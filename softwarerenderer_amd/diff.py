@@ -65,6 +65,7 @@ class DiffRenderer:
         self.bary = torch.zeros((views, height, width, 2), dtype=torch.float32, device=dev)
         self.color = torch.zeros((views, height, width, channels), dtype=torch.float32, device=dev)
         self.image = torch.zeros_like(self.color)
+        self.flags = 1 if cull_mode != api.CullMode.None_ else 0  # SWR_DIFF_CULLED
         self.r = api.Rasterizer(device)
         self.r.setPixelShader(api.VisibilityShader())
         vp = api.VertexProcessor(self.r)
@@ -92,6 +93,7 @@ class DiffRenderer:
         d = _abi.DiffDesc()
         d.views, d.width, d.height = self.V, self.W, self.H
         d.vertex_count, d.triangle_count, d.channels = self.N, self.T, self.C
+        d.flags = self.flags
         d.clip, d.triangles, d.adjacency = (self.clip.data_ptr(), self.tris.data_ptr(),
                                             self.adj.data_ptr())
         d.attributes = attr.data_ptr()
