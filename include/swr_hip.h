@@ -253,8 +253,8 @@ int  swr_batch_destroy(swr_batch *batch);
  * (they can be captured into a HIP graph).  The backward pass recomputes everything else
  * from ids, depth, bary and color. */
 #define SWR_DIFF_CULLED 1          /* every visible triangle faces the same way (the visibility
-                                      draw culled one winding): pixels of two edge-adjacent
-                                      triangles are never separated by a silhouette */
+                                      draw culled one winding): pixels of two triangles that
+                                      share a vertex are never separated by a silhouette */
 typedef struct swr_diff_desc {
     int32_t views, width, height;
     int32_t vertex_count, triangle_count;
@@ -275,6 +275,21 @@ typedef struct swr_diff_desc {
 int  swr_diff_forward(swr_context *ctx, const swr_diff_desc *d);
 int  swr_diff_backward(swr_context *ctx, const swr_diff_desc *d, const float *grad_image,
                        float *grad_clip, float *grad_attributes);
+
+/* ---- tuning ---------------------------------------------------------------------------
+ * No counterpart in the reference.  Launch shapes of the draw path that change speed, never
+ * results (every setting draws the same bits); softwarerenderer_amd/autotune.py measures
+ * them on a workload and picks the fastest.  0 selects the built-in default. */
+typedef struct swr_tuning {
+    int32_t raster_waves;    /* 8x8 tiles (one wave64 each) per raster workgroup: 1, 2, 4, 8 (4) */
+    int32_t raster_group;    /* primitives per row-walk group in the raster kernel of the
+                                depth-checker and visibility programs: 1, 2, 4, 8 (program's) */
+    int32_t xcd_order;       /* 1: raster tile blocks grouped per XCD (default), -1: linear */
+    int32_t expand_chunk;    /* tile pairs per expansion workgroup: 256 .. 4096 (1024) */
+    int32_t reserved[4];
+} swr_tuning;
+int  swr_set_tuning(swr_context *ctx, const swr_tuning *t);
+int  swr_get_tuning(swr_context *ctx, swr_tuning *t);
 
 /* ---- statistics of the last completed draw ------------------------------------------- */
 typedef struct swr_draw_stats {

@@ -121,10 +121,18 @@ static pair_t eval_pair(const dctx *d, int view, int px, int py, int qx, int qy)
     if (cur < 0 || cur >= d->T)
         return r;
     if (d->culled && a != 0 && b != 0) {
-        /* two visible, edge-adjacent triangles of a culled draw: an interior edge */
+        /* two visible triangles of a culled draw that share a vertex: no silhouette taken
+         * to separate them */
         int other = (int)(pf ? b : a) - 1;
-        if (d->adj[3 * cur] == other || d->adj[3 * cur + 1] == other || d->adj[3 * cur + 2] == other)
-            return r;
+        if (other >= 0 && other < d->T) {
+            const int32_t *c = d->tris + 3 * cur, *o = d->tris + 3 * other;
+            int share = 0;
+            for (int x = 0; x < 3; ++x)
+                for (int y = 0; y < 3; ++y)
+                    share |= c[x] == o[y];
+            if (share)
+                return r;
+        }
     }
     int horiz = py == qy;
     f2 F = {(float)(pf ? px : qx) + 0.5f, (float)(pf ? py : qy) + 0.5f};

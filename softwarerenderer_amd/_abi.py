@@ -68,6 +68,13 @@ class DiffDesc(C.Structure):
                 ("image", C.c_void_p)]
 
 
+class Tuning(C.Structure):
+    """swr_tuning: launch shapes of the draw path (0 = default)."""
+    _fields_ = [("raster_waves", C.c_int32), ("raster_group", C.c_int32),
+                ("xcd_order", C.c_int32), ("expand_chunk", C.c_int32),
+                ("reserved", C.c_int32 * 4)]
+
+
 # name -> (restype, argtypes): every entry point include/swr_hip.h declares
 _P, _I, _S, _F = C.c_void_p, C.c_int, C.c_size_t, C.c_float
 SIGNATURES = {
@@ -97,6 +104,8 @@ SIGNATURES = {
     "swr_batch_destroy": (_I, [_P]),
     "swr_diff_forward": (_I, [_P, C.POINTER(DiffDesc)]),
     "swr_diff_backward": (_I, [_P, C.POINTER(DiffDesc), _P, _P, _P]),
+    "swr_set_tuning": (_I, [_P, C.POINTER(Tuning)]),
+    "swr_get_tuning": (_I, [_P, C.POINTER(Tuning)]),
 }
 
 _lib = None

@@ -300,6 +300,18 @@ class Rasterizer:
     def reserve(self, primitives: int, tile_pairs: int):
         self._check(self._lib.swr_reserve(self._ctx, primitives, tile_pairs), "reserve")
 
+    def setTuning(self, raster_waves: int = 0, raster_group: int = 0, xcd_order: int = 0,
+                  expand_chunk: int = 0):
+        """Launch shapes of the draw path (0 = built-in default); results never change."""
+        t = _abi.Tuning(raster_waves, raster_group, xcd_order, expand_chunk)
+        self._check(self._lib.swr_set_tuning(self._ctx, C.byref(t)), "setTuning")
+
+    def tuning(self) -> dict:
+        t = _abi.Tuning()
+        self._check(self._lib.swr_get_tuning(self._ctx, C.byref(t)), "tuning")
+        return {"raster_waves": t.raster_waves, "raster_group": t.raster_group,
+                "xcd_order": t.xcd_order, "expand_chunk": t.expand_chunk}
+
     # drawing through the IRasterizer seam (Rasterizer.h:99-141)
     def _list(self, fn, what, vertices, indices):
         keep: list = []

@@ -65,7 +65,7 @@ struct swr_context {
         bigq,       // per-primitive first pair index
         clipq,      // triangles queued for the clipper
         clipq_cnt,  // its per-region counters
-        zscratch;   // zeroed scratch of the differentiation kernels
+        zscratch[2];  // scratch of the differentiation kernels
     uint64_t prim_cap = 0, pair_cap = 0;
     unsigned long long *status_host = nullptr;
 
@@ -73,6 +73,7 @@ struct swr_context {
     hipEvent_t ev[S_COUNT + 1] = {};
     swr_draw_stats stats{};
     bool async_pending = false;
+    swr_tuning tune{};
 };
 
 // A batched draw's device tables (include/swr_hip.h, swr_batch_create).
@@ -495,7 +496,7 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
                               c->prim_cap, c->pair_cap, c->stream));
         record(c, 2);
         HIP_TRY(c, launch_geom_emit(g, c->stream));
-        HIP_TRY(c, launch_expand(g, c->stream));
+        HIP_TRY(c, launch_expand(g, c->stream, c->tune.expand_chunk));
         record(c, 3);
         SortBuffers sb;
         sb.keys_a = (uint32_t *)c->pk_a.p;
@@ -541,7 +542,9 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
             r.scenes = g.scenes;
             r.nscenes = g.nscenes;
         }
-        HIP_TRY(c, launch_raster(ps, L, r, c->stream));
+        r.xcd = c->tune.xcd_order >= 0 ? 1 : 0;
+        HIP_TRY(c, launch_raster(ps, L, r, c->stream, c->tune.raster_waves ? c->tune.raster_waves : 4,
+                                 c->tune.raster_group));
         record(c, 6);
         HIP_TRY(c, hipMemcpyAsync(c->status_host, c->status.p, ST_WORDS * 8, hipMemcpyDeviceToHost,
                                   c->stream));
@@ -585,18 +588,19 @@ int context_fail(swr_context *c, int code, const char *msg) { return fail(c, cod
 
 bool context_is_device_ptr(const void *p) { return is_device_ptr(p); }
 
-void *context_zeroed_scratch(swr_context *c, size_t bytes)
+void *context_zeroed_scratch(swr_context *c, int slot, size_t bytes)
 {
-    if (c->zscratch.bytes < bytes) {
-        hipError_t e = ensure(c, c->zscratch, bytes);
+    DevBuf &b = c->zscratch[slot & 1];
+    if (b.bytes < bytes) {
+        hipError_t e = ensure(c, b, bytes);
         if (e == hipSuccess)  // ordered before the kernels that use it on this stream
-            e = hipMemsetAsync(c->zscratch.p, 0, c->zscratch.bytes, c->stream);
+            e = hipMemsetAsync(b.p, 0, b.bytes, c->stream);
         if (e != hipSuccess) {
             fail(c, SWR_ERR_HIP, "scratch allocation failed: %s", hipGetErrorString(e));
             return nullptr;
         }
     }
-    return c->zscratch.p;
+    return b.p;
 }
 
 }  // namespace swr
@@ -647,7 +651,7 @@ int swr_destroy(swr_context *c)
                       &c->scan_tmp, &c->prims, &c->pk_a, &c->pv_a, &c->pk_b, &c->pv_b,
                       &c->ranges, &c->hist, &c->hist_ex, &c->hist_tmp, &c->status, &c->tgt[0],
                       &c->tgt[1], &c->tgt[2], &c->log, &c->logcount, &c->bigq, &c->clipq,
-                      &c->clipq_cnt, &c->zscratch};
+                      &c->clipq_cnt, &c->zscratch[0], &c->zscratch[1]};
     for (DevBuf *b : bufs)
         if (b->p)
             (void)hipFree(b->p);
@@ -947,6 +951,26 @@ int swr_batch_destroy(swr_batch *b)
     if (b->bmap.p)
         (void)hipFree(b->bmap.p);
     delete b;
+    return SWR_OK;
+}
+
+int swr_set_tuning(swr_context *c, const swr_tuning *t)
+{
+    if (!c || !t)
+        return fail(c, SWR_ERR_INVALID_ARG, "null tuning");
+    auto pow2_in = [](int v, int lo, int hi) { return v == 0 || (v >= lo && v <= hi && !(v & (v - 1))); };
+    if (!pow2_in(t->raster_waves, 1, 8) || !pow2_in(t->raster_group, 1, 8) ||
+        !pow2_in(t->expand_chunk, 256, 4096) || t->xcd_order < -1 || t->xcd_order > 1)
+        return fail(c, SWR_ERR_INVALID_ARG, "tuning value out of range");
+    c->tune = *t;
+    return SWR_OK;
+}
+
+int swr_get_tuning(swr_context *c, swr_tuning *t)
+{
+    if (!c || !t)
+        return SWR_ERR_INVALID_ARG;
+    *t = c->tune;
     return SWR_OK;
 }
 

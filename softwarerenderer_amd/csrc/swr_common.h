@@ -245,6 +245,7 @@ struct RasterArgs {
     float checker_cells;
     const SceneDev *scenes;     // batched draw (nscenes == 0: single draw)
     uint32_t nscenes;
+    int xcd;                    // 1: tile blocks grouped per XCD
 };
 
 // Scene-local state of a batched draw over the single-draw arguments.

@@ -234,11 +234,18 @@ __device__ inline Pair eval_pair(const DiffArgs &d, int view, int px, int py, in
     if (cur < 0 || cur >= d.T)
         return r;
     if (d.culled && a != 0 && b != 0) {
-        // two visible triangles of a culled draw that share an edge face the same way: the
-        // edge between them is interior
+        // two visible triangles of a culled draw that share a vertex both face the viewer:
+        // no silhouette is taken to separate their pixels
         const int other = (int)(pf ? b : a) - 1;
-        if (d.adj[3 * cur] == other || d.adj[3 * cur + 1] == other || d.adj[3 * cur + 2] == other)
-            return r;
+        if (other >= 0 && other < d.T) {
+            const int c0 = d.tris[3 * cur], c1 = d.tris[3 * cur + 1], c2 = d.tris[3 * cur + 2];
+            const int o0 = d.tris[3 * other], o1 = d.tris[3 * other + 1],
+                      o2 = d.tris[3 * other + 2];
+            const bool share = c0 == o0 || c0 == o1 || c0 == o2 || c1 == o0 || c1 == o1 ||
+                               c1 == o2 || c2 == o0 || c2 == o1 || c2 == o2;
+            if (share)
+                return r;
+        }
     }
     const bool horiz = py == qy;
     const float2 F = pf ? make_float2(px + 0.5f, py + 0.5f) : make_float2(qx + 0.5f, qy + 0.5f);
@@ -385,19 +392,113 @@ __device__ inline void bary_color(const DiffArgs &d, const Bary &B, double *col)
                  B.b[2] * d.attr[(size_t)B.v[2] * C + c];
 }
 
+// Gradient accumulation of one workgroup (one 16x16 tile of one view): an open-addressing
+// table in LDS keyed by vertex, flushed to global memory once per distinct vertex.  The
+// pixels of a tile touch a few dozen vertices, so this replaces most global float atomics
+// (which serialise on shared vertices) by LDS atomics.  A vertex that finds no slot after
+// ACC_PROBES probes goes straight to global memory.
+constexpr int ACC_SLOTS = 512;   // power of two
+constexpr int ACC_PROBES = 32;
+constexpr int ACC_VALS = 8;      // clip x, y, w, attributes 0..4
+
+struct VertexAcc {
+    int *key;
+    float *val;
+};
+
+__device__ inline int acc_slot(const VertexAcc &A, int v)
+{
+    const unsigned h = ((unsigned)v * 2654435761u) >> 23;  // 9 bits: ACC_SLOTS
+#pragma unroll 1
+    for (int probe = 0; probe < ACC_PROBES; ++probe) {
+        const int sl = (int)((h + (unsigned)probe) & (ACC_SLOTS - 1));
+        const int k = __atomic_load_n(&A.key[sl], __ATOMIC_RELAXED);
+        if (k == v)
+            return sl;
+        if (k == -1) {
+            const int old = atomicCAS(&A.key[sl], -1, v);
+            if (old == -1 || old == v)
+                return sl;
+        }
+    }
+    return -1;
+}
+
+__device__ inline void acc_clip(const DiffArgs &d, const VertexAcc &A, int view, int v, float gx,
+                                float gy, float gw)
+{
+    const int sl = acc_slot(A, v);
+    if (sl >= 0) {
+        atomicAdd(&A.val[sl * ACC_VALS + 0], gx);
+        atomicAdd(&A.val[sl * ACC_VALS + 1], gy);
+        atomicAdd(&A.val[sl * ACC_VALS + 2], gw);
+    } else {
+        float *g = d.gclip + ((size_t)view * d.N + v) * 4;
+        atomicAdd(g + 0, gx);
+        atomicAdd(g + 1, gy);
+        atomicAdd(g + 3, gw);
+    }
+}
+
+// clip and attribute gradients of one vertex with a single table lookup
+template <int C>
+__device__ inline void acc_vertex(const DiffArgs &d, const VertexAcc &A, int view, int v,
+                                  float gx, float gy, float gw, const float *ga)
+{
+    const int sl = acc_slot(A, v);
+    if (sl >= 0) {
+        float *x = &A.val[sl * ACC_VALS];
+        atomicAdd(x + 0, gx);
+        atomicAdd(x + 1, gy);
+        atomicAdd(x + 2, gw);
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+            atomicAdd(x + 3 + c, ga[c]);
+    } else {
+        float *g = d.gclip + ((size_t)view * d.N + v) * 4;
+        atomicAdd(g + 0, gx);
+        atomicAdd(g + 1, gy);
+        atomicAdd(g + 3, gw);
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+            atomicAdd(d.gattr + ((size_t)view * d.N + v) * C + c, ga[c]);
+    }
+}
+
+template <int C>
+__device__ inline void acc_flush(const DiffArgs &d, const VertexAcc &A, int view)
+{
+    for (int sl = threadIdx.x; sl < ACC_SLOTS; sl += blockDim.x) {
+        const int v = A.key[sl];
+        if (v < 0)
+            continue;
+        const float *x = &A.val[sl * ACC_VALS];
+        float *g = d.gclip + ((size_t)view * d.N + v) * 4;
+        if (x[0] != 0.0f) atomicAdd(g + 0, x[0]);
+        if (x[1] != 0.0f) atomicAdd(g + 1, x[1]);
+        if (x[2] != 0.0f) atomicAdd(g + 3, x[2]);
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+            if (x[3 + c] != 0.0f)
+                atomicAdd(d.gattr + ((size_t)view * d.N + v) * C + c, x[3 + c]);
+    }
+}
+
 // Backward of col = sum_m b_m A_m given dL/dcol: adds to the attributes and, through
 // b_m = D_m / S with D_m = r . (h_j x h_k), to the clip coordinates of the three vertices.
 template <int C>
-__device__ inline void bary_backward(const DiffArgs &d, int view, const Bary &B, const double *gcol)
+__device__ inline void bary_backward(const DiffArgs &d, const VertexAcc &A, int view,
+                                     const Bary &B, const double *gcol)
 {
     double u[3];
+    float ga[3][C];
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
         u[m] = 0.0;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             u[m] += gcol[c] * d.attr[(size_t)B.v[m] * C + c];
-            atomicAdd(d.gattr + ((size_t)view * d.N + B.v[m]) * C + c, (float)(B.b[m] * gcol[c]));
+            ga[m][c] = (float)(B.b[m] * gcol[c]);
         }
     }
     const double ub = u[0] * B.b[0] + u[1] * B.b[1] + u[2] * B.b[2];
@@ -416,12 +517,8 @@ __device__ inline void bary_backward(const DiffArgs &d, int view, const Bary &B,
         g[k].w += gD * dk.w;
     }
 #pragma unroll
-    for (int m = 0; m < 3; ++m) {
-        float *gv = d.gclip + ((size_t)view * d.N + B.v[m]) * 4;
-        atomicAdd(gv + 0, (float)g[m].x);
-        atomicAdd(gv + 1, (float)g[m].y);
-        atomicAdd(gv + 3, (float)g[m].w);
-    }
+    for (int m = 0; m < 3; ++m)
+        acc_vertex<C>(d, A, view, B.v[m], (float)g[m].x, (float)g[m].y, (float)g[m].w, ga[m]);
 }
 
 template <int C>
@@ -456,6 +553,82 @@ __device__ inline bool neighbour(const DiffArgs &d, int px, int py, int k, int &
     return qx >= 0 && qx < d.W && qy >= 0 && qy < d.H;
 }
 
+// ---- pair decisions shared by a tile -------------------------------------------------------
+// Each pair of neighbouring pixels is evaluated once per 16x16 tile: thread (lx, ly) takes the
+// pair to its right and the pair below it, the tile's first column and row also the pairs
+// across the tile's left and top border.  h[ly][lx + 1] holds the pair (lx, ly)-(lx + 1, ly),
+// v[ly + 1][lx] the pair (lx, ly)-(lx, ly + 1), both seen from the left / upper pixel; the
+// other pixel reads it with the roles swapped.
+struct PairLds {
+    int flags;  // 1 active, 2 first pixel blended, 4 first pixel in front, 8 s < 0
+    float tf;
+    int tri, vj, vk;
+};
+
+__device__ inline PairLds pack_pair(const Pair &p)
+{
+    PairLds x;
+    x.flags = (p.active ? 1 : 0) | (p.p_blended ? 2 : 0) | (p.front_is_p ? 4 : 0) |
+              (p.s < 0.0f ? 8 : 0);
+    x.tf = p.tf;
+    x.tri = p.tri;
+    x.vj = p.vj;
+    x.vk = p.vk;
+    return x;
+}
+
+__device__ inline Pair unpack_pair(const PairLds &x, bool swap, bool horiz)
+{
+    Pair p;
+    p.active = x.flags & 1;
+    p.p_blended = ((x.flags & 2) != 0) != swap;
+    p.front_is_p = ((x.flags & 4) != 0) != swap;
+    p.s = (x.flags & 8) ? -1.0f : 1.0f;
+    p.tf = x.tf;
+    p.tri = x.tri;
+    p.vj = x.vj;
+    p.vk = x.vk;
+    p.horiz = horiz;
+    return p;
+}
+
+constexpr int TP_H = 16 * 17, TP_V = 17 * 16;
+
+__device__ inline PairLds eval_packed(const DiffArgs &d, int view, int ax, int ay, int bx, int by)
+{
+    if (ax < 0 || ay < 0 || bx >= d.W || by >= d.H || ax >= d.W || ay >= d.H) {
+        PairLds x;
+        x.flags = 0;
+        return x;
+    }
+    return pack_pair(eval_pair(d, view, ax, ay, bx, by));
+}
+
+// Fills the tile's pair table; every thread of the workgroup calls it, then a barrier.
+__device__ inline void tile_pairs(const DiffArgs &d, const PixelIdx &pi, PairLds *h, PairLds *v)
+{
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int px = pi.px, py = pi.py, view = pi.view;
+    h[ly * 17 + lx + 1] = eval_packed(d, view, px, py, px + 1, py);
+    v[(ly + 1) * 16 + lx] = eval_packed(d, view, px, py, px, py + 1);
+    if (lx == 0)
+        h[ly * 17] = eval_packed(d, view, px - 1, py, px, py);
+    if (ly == 0)
+        v[lx] = eval_packed(d, view, px, py - 1, px, py);
+}
+
+// The pair between this thread's pixel and neighbour k (0 right, 1 left, 2 below, 3 above).
+__device__ inline Pair tile_pair(const PairLds *h, const PairLds *v, int k)
+{
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    switch (k) {
+    case 0: return unpack_pair(h[ly * 17 + lx + 1], false, true);
+    case 1: return unpack_pair(h[ly * 17 + lx], true, true);
+    case 2: return unpack_pair(v[(ly + 1) * 16 + lx], false, false);
+    default: return unpack_pair(v[ly * 16 + lx], true, false);
+    }
+}
+
 // The blend a pixel receives from one active pair where it is the blended pixel:
 //   front pixel (t < 1/2):  + a (1/2 - t) (C_q - C_p)
 //   other pixel (t >= 1/2): + a (t - 1/2) (E - C_p), E = the silhouette triangle's colour
@@ -466,9 +639,13 @@ __device__ inline bool neighbour(const DiffArgs &d, int px, int py, int k, int &
 template <int C>
 __global__ void __launch_bounds__(256) k_diff_aa(DiffArgs d)
 {
+    __shared__ PairLds s_h[TP_H], s_v[TP_V];
     const size_t HW = (size_t)d.W * d.H;
     PixelIdx pi;
-    if (!pixel_of_thread(d, pi))
+    const bool valid = pixel_of_thread(d, pi);
+    tile_pairs(d, pi, s_h, s_v);
+    __syncthreads();
+    if (!valid)
         return;
     const size_t i = pi.i;
     const int view = pi.view, px = pi.px, py = pi.py;
@@ -480,7 +657,7 @@ __global__ void __launch_bounds__(256) k_diff_aa(DiffArgs d)
         int qx, qy;
         if (!neighbour(d, px, py, k, qx, qy))
             continue;
-        const Pair pr = eval_pair(d, view, px, py, qx, qy);
+        const Pair pr = tile_pair(s_h, s_v, k);
         if (!pr.active || !pr.p_blended)
             continue;
         const PairVal pv = pair_val(d, view, pr, px, py, qx, qy);
@@ -507,25 +684,21 @@ __global__ void __launch_bounds__(256) k_diff_aa(DiffArgs d)
         d.image[i * C + c] = (float)out[c];
 }
 
-// d(window x, y) of a clip-space vertex -> d(clip x, y, w), accumulated into gclip.
-__device__ inline void scatter_window_grad(const DiffArgs &d, int view, int v, double gx,
-                                           double gy)
+// d(window x, y) of a clip-space vertex -> d(clip x, y, w), accumulated.
+__device__ inline void scatter_window_grad(const DiffArgs &d, const VertexAcc &A, int view, int v,
+                                           double gx, double gy)
 {
     const H3 h = load_h(d, view, v);
     const double iw = 1.0 / h.w, sx = 0.5 * d.W, sy = 0.5 * d.H;
-    float *g = d.gclip + ((size_t)view * d.N + v) * 4;
-    atomicAdd(g + 0, (float)(gx * sx * iw));
-    atomicAdd(g + 1, (float)(-gy * sy * iw));
-    atomicAdd(g + 3, (float)((-gx * sx * h.x + gy * sy * h.y) * iw * iw));
+    acc_clip(d, A, view, v, (float)(gx * sx * iw), (float)(-gy * sy * iw),
+             (float)((-gx * sx * h.x + gy * sy * h.y) * iw * iw));
 }
 
 template <int C>
-__global__ void __launch_bounds__(256) k_diff_backward(DiffArgs d)
+__device__ inline void backward_pixel(const DiffArgs &d, const VertexAcc &A, const PixelIdx &pi,
+                                      const PairLds *s_h, const PairLds *s_v)
 {
     const size_t HW = (size_t)d.W * d.H;
-    PixelIdx pi;
-    if (!pixel_of_thread(d, pi))
-        return;
     const size_t i = pi.i;
     const int view = pi.view, px = pi.px, py = pi.py;
     double gp[C], gc[C], cp[C];
@@ -540,7 +713,7 @@ __global__ void __launch_bounds__(256) k_diff_backward(DiffArgs d)
         int qx, qy;
         if (!neighbour(d, px, py, k, qx, qy))
             continue;
-        const Pair pr = eval_pair(d, view, px, py, qx, qy);
+        const Pair pr = tile_pair(s_h, s_v, k);
         if (!pr.active)
             continue;
         const size_t iq = (size_t)view * HW + (size_t)qy * d.W + qx;
@@ -575,7 +748,7 @@ __global__ void __launch_bounds__(256) k_diff_backward(DiffArgs d)
 #pragma unroll
             for (int c = 0; c < C; ++c)
                 gsrc[c] = w * gp[c];
-            bary_backward<C>(d, view, B, gsrc);
+            bary_backward<C>(d, A, view, B, gsrc);
             bary_color_dr<C>(d, B, drdt, de);
 #pragma unroll
             for (int c = 0; c < C; ++c)
@@ -588,9 +761,9 @@ __global__ void __launch_bounds__(256) k_diff_backward(DiffArgs d)
             dot += gp[c] * (src[c] - cp[c]);
         }
         const double dldt = pv.a * (t_low ? -dot : dot) + dedt, dlda = wt * dot;
-        scatter_window_grad(d, view, pr.vj, dldt * pv.tj.x + dlda * pv.aj.x,
+        scatter_window_grad(d, A, view, pr.vj, dldt * pv.tj.x + dlda * pv.aj.x,
                             dldt * pv.tj.y + dlda * pv.aj.y);
-        scatter_window_grad(d, view, pr.vk, dldt * pv.tk.x + dlda * pv.ak.x,
+        scatter_window_grad(d, A, view, pr.vk, dldt * pv.tk.x + dlda * pv.ak.x,
                             dldt * pv.tk.y + dlda * pv.ak.y);
     }
 
@@ -599,7 +772,32 @@ __global__ void __launch_bounds__(256) k_diff_backward(DiffArgs d)
     Bary B;
     if (id == 0 || (int)id > d.T || !bary_at(d, view, (int)id - 1, px, py, B))
         return;
-    bary_backward<C>(d, view, B, gc);
+    bary_backward<C>(d, A, view, B, gc);
+}
+
+// Gradients of one 16x16 tile: pair decisions in LDS, vertex gradients accumulated in LDS
+// and flushed once per vertex.
+template <int C>
+__global__ void __launch_bounds__(256) k_diff_backward(DiffArgs d)
+{
+    __shared__ int s_key[ACC_SLOTS];
+    __shared__ float s_val[ACC_SLOTS * ACC_VALS];
+    __shared__ PairLds s_h[TP_H], s_v[TP_V];
+    for (int sl = threadIdx.x; sl < ACC_SLOTS; sl += blockDim.x) {
+        s_key[sl] = -1;
+#pragma unroll
+        for (int k = 0; k < ACC_VALS; ++k)
+            s_val[sl * ACC_VALS + k] = 0.0f;
+    }
+    PixelIdx pi;
+    const bool valid = pixel_of_thread(d, pi);
+    tile_pairs(d, pi, s_h, s_v);
+    __syncthreads();
+    const VertexAcc A{s_key, s_val};
+    if (valid)  // every thread reaches the barrier below
+        backward_pixel<C>(d, A, pi, s_h, s_v);
+    __syncthreads();
+    acc_flush<C>(d, A, pi.view);
 }
 
 // grad_attributes += sum over views of the partials; the partials are cleared for the next
@@ -689,7 +887,7 @@ static int diff_run(swr_context *c, const swr_diff_desc *dd, const float *gimage
     d.gattr_out = gattr;
     if (which == 1) {
         // per-view partials: views do not contend on one vertex's attribute gradient
-        d.gattr = (float *)context_zeroed_scratch(c, (size_t)d.V * d.N * q.channels * 4);
+        d.gattr = (float *)context_zeroed_scratch(c, 0, (size_t)d.V * d.N * q.channels * 4);
         if (!d.gattr)
             return SWR_ERR_HIP;
     }

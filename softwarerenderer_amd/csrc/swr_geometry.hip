@@ -466,11 +466,11 @@ __device__ __forceinline__ void write_prim(const GeomArgs &a, uint64_t pos, uint
 }
 
 // (tile, primitive) pairs, load-balanced: each workgroup owns EXPAND_CHUNK consecutive pair
-// slots.  Pair bases are increasing in draw order, so the primitives owning a chunk are a
-// contiguous run found by one binary search; each thread then searches that run (in LDS)
-// for the primitive of its slot and writes the slot's tile in row-major rect order.
-constexpr int EXPAND_CHUNK = 1024;
-
+// slots (256 to 4096; tunable).  Pair bases are increasing in draw order, so the primitives
+// owning a chunk are a contiguous run found by one binary search; each thread then searches
+// that run (in LDS) for the primitive of its slot and writes the slot's tile in row-major
+// rect order.
+template <int EXPAND_CHUNK>
 __global__ void __launch_bounds__(256) k_expand(GeomArgs a)
 {
     __shared__ uint32_t s_pb[EXPAND_CHUNK + 1];
@@ -1237,13 +1237,25 @@ hipError_t launch_geom_count(const GeomArgs &a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_expand(const GeomArgs &a, hipStream_t s)
+template <int CH>
+static hipError_t launch_expand_ch(const GeomArgs &a, hipStream_t s)
 {
-    uint64_t blocks = (a.pair_cap + EXPAND_CHUNK - 1) / EXPAND_CHUNK;
+    uint64_t blocks = (a.pair_cap + CH - 1) / CH;
     if (blocks < 1)
         blocks = 1;
-    hipLaunchKernelGGL(k_expand, dim3((uint32_t)blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_expand<CH>, dim3((uint32_t)blocks), dim3(256), 0, s, a);
     return hipGetLastError();
+}
+
+hipError_t launch_expand(const GeomArgs &a, hipStream_t s, int chunk)
+{
+    switch (chunk) {
+    case 256: return launch_expand_ch<256>(a, s);
+    case 512: return launch_expand_ch<512>(a, s);
+    case 2048: return launch_expand_ch<2048>(a, s);
+    case 4096: return launch_expand_ch<4096>(a, s);
+    default: return launch_expand_ch<1024>(a, s);
+    }
 }
 
 int emit_class(int nvary)

@@ -17,7 +17,7 @@ struct SortBuffers {
 hipError_t launch_begin(const GeomArgs &a, hipStream_t s);
 hipError_t launch_geom_count(const GeomArgs &a, hipStream_t s);
 hipError_t launch_geom_emit(const GeomArgs &a, hipStream_t s);
-hipError_t launch_expand(const GeomArgs &a, hipStream_t s);
+hipError_t launch_expand(const GeomArgs &a, hipStream_t s, int chunk);
 
 // scans and sort (swr_sort.hip)
 uint64_t scan_tmp_elems(uint64_t n);
@@ -36,13 +36,13 @@ uint32_t pair_sentinel(uint32_t tiles);
 hipStream_t context_stream(swr_context *c);
 int context_fail(swr_context *c, int code, const char *msg);
 bool context_is_device_ptr(const void *p);
-// A device buffer of at least `bytes` owned by the context, zero when first handed out at
-// this size; its users leave it zeroed.  nullptr (and the context's error set) on failure.
-void *context_zeroed_scratch(swr_context *c, size_t bytes);
+// Device buffer `slot` (0 or 1) of at least `bytes` owned by the context, zero when first
+// handed out at this size.  nullptr (and the context's error set) on failure.
+void *context_zeroed_scratch(swr_context *c, int slot, size_t bytes);
 
 // raster (swr_raster.hip)
 bool raster_layout(const swr_pixel_shader_desc &ps, ParamLayout &L);
 hipError_t launch_raster(const swr_pixel_shader_desc &ps, const ParamLayout &L,
-                         const RasterArgs &a, hipStream_t s);
+                         const RasterArgs &a, hipStream_t s, int waves, int group);
 
 }  // namespace swr

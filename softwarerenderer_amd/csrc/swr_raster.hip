@@ -244,19 +244,24 @@ __device__ inline void lds_sync()
     __asm__ volatile("" ::: "memory");
 }
 
-template <template <class> class ProgT, int NZ, int NW, int NA, int NP>
-__global__ void __launch_bounds__(256) k_raster(RasterArgs a0)
+template <template <class> class ProgT, int NZ, int NW, int NA, int NP, int GSX>
+__global__ void __launch_bounds__(512) k_raster(RasterArgs a0)
 {
     using L = Layout<NZ, NW, NA, NP>;
     using Prog = ProgT<L>;
     constexpr int N = L::N;
     constexpr int NS = N > 0 ? N : 1;
-    constexpr int GS = N <= 8 ? 8 : (N <= 16 ? 4 : (N <= 32 ? 2 : 1));  // primitives per group
-    __shared__ float s_rows[4][GS * 64];
-    __shared__ unsigned long long s_cov[4][GS];
+    // primitives per group: the program's default, or the tuned value
+    constexpr int GS = GSX > 0 ? GSX : (N <= 8 ? 8 : (N <= 16 ? 4 : (N <= 32 ? 2 : 1)));
+    static_assert(GS >= 1 && GS <= 8, "a group is at most 8 primitives (one per row slot)");
+    constexpr int MAXW = 8;  // tiles (waves) per workgroup: blockDim.x / 64, at most 8
+    __shared__ float s_rows[MAXW][GS * 64];
+    __shared__ unsigned long long s_cov[MAXW][GS];
 
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int tile = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * 4 + wv);
+    const int wpb = (int)(blockDim.x >> 6);
+    const unsigned blk = a0.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int tile = __builtin_amdgcn_readfirstlane((int)blk * wpb + wv);
     if (tile >= a0.tiles)
         return;
     if (a0.status[ST_OVERFLOW] | a0.status[ST_ERROR])
@@ -412,14 +417,28 @@ __global__ void __launch_bounds__(256) k_raster(RasterArgs a0)
         Prog::store(st, a, o);
 }
 
-template <template <class> class ProgT, int NZ, int NW, int NA, int NP>
-static hipError_t launch(const RasterArgs &a, hipStream_t s)
+template <template <class> class ProgT, int NZ, int NW, int NA, int NP, int GSX = 0>
+static hipError_t launch(const RasterArgs &a, hipStream_t s, int waves)
 {
-    uint32_t blocks = (uint32_t)((a.tiles + 3) / 4);
+    const uint32_t w = (uint32_t)waves;
+    uint32_t blocks = (uint32_t)((a.tiles + w - 1) / w);
     if (!blocks)
         return hipSuccess;
-    hipLaunchKernelGGL((k_raster<ProgT, NZ, NW, NA, NP>), dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_raster<ProgT, NZ, NW, NA, NP, GSX>), dim3(blocks), dim3(64 * w), 0, s,
+                       a);
     return hipGetLastError();
+}
+
+// The programs the tuner may give another group size (the benchmark and visibility ones).
+template <template <class> class ProgT, int NZ, int NW, int NA, int NP>
+static hipError_t launch_tuned(const RasterArgs &a, hipStream_t s, int waves, int group)
+{
+    switch (group) {
+    case 1: return launch<ProgT, NZ, NW, NA, NP, 1>(a, s, waves);
+    case 2: return launch<ProgT, NZ, NW, NA, NP, 2>(a, s, waves);
+    case 4: return launch<ProgT, NZ, NW, NA, NP, 4>(a, s, waves);
+    default: return launch<ProgT, NZ, NW, NA, NP>(a, s, waves);
+    }
 }
 
 // The plane layout each program's kernel expects (see ParamLayout).
@@ -470,21 +489,25 @@ bool raster_layout(const swr_pixel_shader_desc &ps, ParamLayout &L)
 }
 
 hipError_t launch_raster(const swr_pixel_shader_desc &ps, const ParamLayout &L,
-                         const RasterArgs &a, hipStream_t s)
+                         const RasterArgs &a, hipStream_t s, int waves, int group)
 {
+    if (waves != 1 && waves != 2 && waves != 4 && waves != 8)
+        waves = 4;
     switch (ps.program) {
     case SWR_PS_COVERAGE:
-        return launch<ProgCoverage, 0, 0, 0, 0>(a, s);
+        return launch<ProgCoverage, 0, 0, 0, 0>(a, s, waves);
     case SWR_PS_RGB:
-        return launch<ProgRGB, 0, 0, 3, 0>(a, s);
+        return launch<ProgRGB, 0, 0, 3, 0>(a, s, waves);
     case SWR_PS_DEPTH_CHECKER:
-        return launch<ProgDepthChecker, 1, 1, 3, 2>(a, s);
+        return launch_tuned<ProgDepthChecker, 1, 1, 3, 2>(a, s, waves, group);
     case SWR_PS_VISIBILITY:
-        return launch<ProgVisibility, 1, 0, 0, 0>(a, s);
+        return launch_tuned<ProgVisibility, 1, 0, 0, 0>(a, s, waves, group);
     case SWR_PS_HASH:
-        return L.na == 4 ? launch<ProgHash, 1, 1, 4, 4>(a, s) : launch<ProgHash, 1, 1, 16, 16>(a, s);
+        return L.na == 4 ? launch<ProgHash, 1, 1, 4, 4>(a, s, waves)
+                         : launch<ProgHash, 1, 1, 16, 16>(a, s, waves);
     case SWR_PS_LOG:
-        return L.na == 4 ? launch<ProgLog, 1, 1, 4, 4>(a, s) : launch<ProgLog, 1, 1, 16, 16>(a, s);
+        return L.na == 4 ? launch<ProgLog, 1, 1, 4, 4>(a, s, waves)
+                         : launch<ProgLog, 1, 1, 16, 16>(a, s, waves);
     default:
         return hipSuccess;
     }
