@@ -43,6 +43,8 @@ def parse():
     p.add_argument("--diff-views", type=int, default=64)
     p.add_argument("--diff-size", type=int, default=512)
     p.add_argument("--diff-subdiv", type=int, default=6)
+    p.add_argument("--no-tune", action="store_true",
+                   help="skip the launch-shape autotuner (built-in defaults)")
     return p.parse_args()
 
 
@@ -147,6 +149,9 @@ def main():
     from softwarerenderer_amd import api, scenes
 
     dev = torch.device("cuda:0")
+    # one stream for the clears, the draws and the timing events
+    stream = torch.cuda.Stream(dev)
+    torch.cuda.set_stream(stream)
     if args.copy_probe:
         n = 256 * 1024 * 1024 // 4
         src = torch.ones(n, dtype=torch.float32, device=dev)
@@ -164,6 +169,7 @@ def main():
     depth = torch.ones((H, W), dtype=torch.float32, device=dev)
 
     r = api.Rasterizer(0)
+    r.setStream(stream.cuda_stream)
     v = api.VertexProcessor(r)
     r.setRasterMode(api.RasterMode.Block)
     r.setScissorRect(0, 0, W, H)
@@ -181,7 +187,14 @@ def main():
         else:
             v.drawElementsAsync(api.DrawMode.Triangle, idx)
 
-    # warmup: the first synchronous draw sizes the arenas
+    # the first synchronous draw sizes the arenas
+    step(True)
+    tuning = None
+    if not args.no_tune:
+        from softwarerenderer_amd.autotune import autotune
+        tu = autotune(r, lambda: step(False))
+        tuning = {"chosen": tu["best"], "tuned_ms": round(tu["ms"], 4),
+                  "default_ms": round(tu["default_ms"], 4), "configs": len(tu["table"])}
     for _ in range(max(1, args.warmup)):
         step(True)
     st = r.stats()
@@ -193,10 +206,10 @@ def main():
     t1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     wall0 = time.perf_counter()
-    t0.record()
+    t0.record(stream)
     for _ in range(args.steps):
         step(False)
-    t1.record()
+    t1.record(stream)
     r.synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
@@ -253,6 +266,7 @@ def main():
                      "frac": round(raster_gbs / HBM_PEAK_GBS, 5),
                      "traffic": measured_traffic("soup-1M-1080p-block"),
                      "algorithmic_bytes": raster_bytes},
+        "tuning": tuning,
         "frame": {"algorithmic_bytes": frame_bytes, "kernel_ms": round(kern_ms, 4),
                   "achieved_GBs": round(frame_bytes / (kern_ms * 1e-3) / 1e9, 2)
                   if kern_ms > 0 else None},

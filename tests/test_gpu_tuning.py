@@ -1,0 +1,132 @@
+"""Launch-shape tuning: every swr_tuning setting draws the same bits as the oracle, invalid
+settings are refused, and the autotuner leaves a valid setting behind."""
+import numpy as np
+import pytest
+
+import oracle as O
+from softwarerenderer_amd import scenes
+from helpers import assert_same, run_cpu, run_gpu
+
+pytestmark = pytest.mark.gpu
+
+TUNINGS = [
+    dict(raster_waves=1, raster_group=1, xcd_order=-1, expand_chunk=256),
+    dict(raster_waves=2, raster_group=2, xcd_order=1, expand_chunk=512),
+    dict(raster_waves=8, raster_group=4, xcd_order=1, expand_chunk=2048),
+    dict(raster_waves=4, raster_group=8, xcd_order=-1, expand_chunk=4096),
+]
+
+
+@pytest.fixture(scope="module")
+def sphere():
+    return scenes.sphere_scene()
+
+
+@pytest.fixture(scope="module")
+def clip():
+    return scenes.clip_scene(ntri=3000, seed=3)
+
+
+def _rasterizer(tune):
+    from softwarerenderer_amd import api
+    r = api.Rasterizer(0)
+    r.setTuning(**tune)
+    assert r.tuning() == tune
+    return r
+
+
+@pytest.mark.parametrize("tune", TUNINGS, ids=lambda t: "w{raster_waves}g{raster_group}x{xcd_order}c{expand_chunk}".format(**t))
+@pytest.mark.parametrize("prog", [O.PS_LOG, O.PS_DEPTH_CHECKER, O.PS_VISIBILITY])
+def test_sphere_tuned(gpu, cpu_oracle, sphere, tune, prog):
+    pipe = O.Pipeline(640, 480, raster_mode=O.RASTER_BLOCK, mvp=sphere.mvp, ps_program=prog)
+    cap = 100_000 if prog == O.PS_LOG else 0
+    a = run_cpu(cpu_oracle, pipe, sphere, log_capacity=cap)
+    r = _rasterizer(tune)
+    try:
+        b = run_gpu(pipe, sphere, log_capacity=cap, rasterizer=r)
+    finally:
+        r.close()
+    assert_same(a, b, f"sphere prog={prog} tuning={tune}")
+
+
+@pytest.mark.parametrize("tune", TUNINGS[:2] + TUNINGS[3:])
+def test_clip_tuned(gpu, cpu_oracle, clip, tune):
+    pipe = O.Pipeline(clip.width, clip.height, raster_mode=O.RASTER_SPAN, cull_mode=O.CULL_NONE,
+                      vs_program=O.VS_RAW, vs_counts=(1, 3, 2), ps_program=O.PS_HASH)
+    a = run_cpu(cpu_oracle, pipe, clip)
+    r = _rasterizer(tune)
+    try:
+        b = run_gpu(pipe, clip, rasterizer=r)
+    finally:
+        r.close()
+    assert_same(a, b, f"clip tuning={tune}")
+
+
+def test_soup_tuned(gpu, cpu_oracle):
+    sc = scenes.soup_scene(40_000, seed=9)
+    pipe = O.Pipeline(1920, 1080, raster_mode=O.RASTER_BLOCK, mvp=sc.mvp,
+                      ps_program=O.PS_DEPTH_CHECKER)
+    a = run_cpu(cpu_oracle, pipe, sc)
+    for tune in (TUNINGS[0], TUNINGS[2]):
+        r = _rasterizer(tune)
+        try:
+            b = run_gpu(pipe, sc, rasterizer=r)
+        finally:
+            r.close()
+        assert_same(a, b, f"soup tuning={tune}")
+
+
+@pytest.mark.parametrize("bad", [dict(raster_waves=3), dict(raster_waves=16),
+                                 dict(raster_group=5), dict(expand_chunk=128),
+                                 dict(expand_chunk=3000), dict(xcd_order=2)])
+def test_invalid_tuning(gpu, bad):
+    from softwarerenderer_amd import _abi, api
+    r = api.Rasterizer(0)
+    try:
+        before = r.tuning()
+        with pytest.raises(_abi.SwrError):
+            r.setTuning(**bad)
+        assert r.tuning() == before
+    finally:
+        r.close()
+
+
+def test_autotune_small(gpu, cpu_oracle, sphere):
+    import torch
+    from softwarerenderer_amd import api
+    from softwarerenderer_amd.autotune import autotune
+    W, H = 640, 480
+    color = torch.zeros((H, W), dtype=torch.int32, device=gpu)
+    depth = torch.ones((H, W), dtype=torch.float32, device=gpu)
+    r = api.Rasterizer(0)
+    try:
+        v = api.VertexProcessor(r)
+        r.setRasterMode(api.RasterMode.Block)
+        r.setScissorRect(0, 0, W, H)
+        r.setPixelShader(api.DepthCheckerShader(color, depth))
+        v.setViewport(0, 0, W, H)
+        v.setCullMode(api.CullMode.CW)
+        v.setVertexShader(api.MvpVertexShader(sphere.mvp))
+        v.setVertexAttribPointer(0, sphere.stride, torch.from_numpy(sphere.vertices).to(gpu))
+        idx = torch.from_numpy(sphere.indices).to(gpu)
+
+        def draw():
+            color.zero_()
+            depth.fill_(1.0)
+            v.drawElementsAsync(api.DrawMode.Triangle, idx)
+
+        v.drawElements(api.DrawMode.Triangle, idx)
+        res = autotune(r, draw, space={"raster_waves": (2, 4), "raster_group": (0, 2),
+                                       "expand_chunk": (1024,)}, reps=2, warmup=1, rounds=1)
+        assert len(res["table"]) == 2 * 2 * 2 * 1
+        assert r.tuning() == res["best"]
+        assert res["ms"] > 0
+        draw()
+        r.synchronize()
+        pipe = O.Pipeline(W, H, raster_mode=O.RASTER_BLOCK, mvp=sphere.mvp,
+                          ps_program=O.PS_DEPTH_CHECKER)
+        a = run_cpu(cpu_oracle, pipe, sphere)
+        assert np.array_equal(a.t0.view(np.uint32), color.cpu().numpy().view(np.uint32))
+        assert np.array_equal(a.t1.view(np.uint32), depth.cpu().numpy().view(np.uint32))
+    finally:
+        r.close()
