@@ -43,6 +43,10 @@ extern "C" {
 #define SWR_MAX_VERTEX_ATTRIBS 8
 #define SWR_BATCH_PRIMITIVES 1024  /* VertexProcessor.cpp:110 */
 #define SWR_MAX_CLIP_VERTS    32   /* polygon vertices after 6 planes; 3 + 6*4 is the convex bound */
+/* DDA steps of one line (Rasterizer.h:177-179: max(|dx|, |dy|) of the truncated endpoints).
+ * A longer line fails the draw with SWR_ERR_CAPACITY: the reference would walk it step by
+ * step (NaN or huge window coordinates give ~2^31 steps). */
+#define SWR_MAX_LINE_STEPS   (1 << 20)
 
 /* ---- enums (same values as the reference's enum classes) ---------------------------- */
 /* RasterMode, Rasterizer.h:45-49 */
@@ -175,10 +179,12 @@ int  swr_set_pixel_shader(swr_context *ctx, const swr_pixel_shader_desc *d);/* R
  * IRasterizer.h:70 / Rasterizer.h:134-141 (drawTriangle, :111-114, is a list of one). */
 int  swr_draw_triangle_list(swr_context *ctx, const swr_rasterizer_vertex *vertices,
                             size_t vertex_count, const int32_t *indices, size_t index_count);
-/* IRasterizer.h:66 / Rasterizer.h:125-132 */
+/* IRasterizer.h:66 / Rasterizer.h:125-132: index pairs; a pair whose first index is -1 is
+ * skipped.  Lines are DDA-walked on the device (Rasterizer.h:175-222). */
 int  swr_draw_line_list(swr_context *ctx, const swr_rasterizer_vertex *vertices,
                         size_t vertex_count, const int32_t *indices, size_t index_count);
-/* IRasterizer.h:62 / Rasterizer.h:116-123 */
+/* IRasterizer.h:62 / Rasterizer.h:116-123: one pixel per point that passes the scissor
+ * test (Rasterizer.h:149-158); index -1 is skipped. */
 int  swr_draw_point_list(swr_context *ctx, const swr_rasterizer_vertex *vertices,
                          size_t vertex_count, const int32_t *indices, size_t index_count);
 
@@ -191,7 +197,9 @@ int  swr_set_vertex_shader(swr_context *ctx, const swr_vertex_shader_desc *d);/*
  * the reference); otherwise indices addressing past the buffer fail with SWR_ERR_INDEX_RANGE. */
 int  swr_set_vertex_attrib_pointer(swr_context *ctx, int index, int stride,
                                    const void *buffer, size_t buffer_bytes);
-/* VertexProcessor.cpp:78-120.  count indices; triangles use count/3 complete triples. */
+/* VertexProcessor.cpp:78-120.  count indices; triangles use count/3 complete triples, lines
+ * count/2 pairs, points every index.  Lines and points: clipLines / clipPoints
+ * (VertexProcessor.cpp:152-215), then the line and point lists above. */
 int  swr_draw_elements(swr_context *ctx, int mode, size_t count, const int32_t *indices);
 /* Same, enqueued on the context stream without waiting.  Device buffers only.  The context's
  * arenas must already be large enough (swr_reserve, or a previous synchronous draw of the

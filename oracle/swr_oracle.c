@@ -622,6 +622,7 @@ static void draw_line_list(rctx *r, uint32_t batch, const vtx *v, const int32_t 
         if (idx[i] == -1)
             continue;
         r->key = make_key(batch, i / 2);
+        r->src = batch * (uint32_t)SWR_BATCH_PRIMITIVES + (uint32_t)(i / 2);
         draw_line(r, &v[idx[i]], &v[idx[i + 1]]);
     }
 }
@@ -633,6 +634,7 @@ static void draw_point_list(rctx *r, uint32_t batch, const vtx *v, const int32_t
         if (idx[i] == -1)
             continue;
         r->key = make_key(batch, i);
+        r->src = batch * (uint32_t)SWR_BATCH_PRIMITIVES + (uint32_t)i;
         draw_point(r, &v[idx[i]]);
     }
 }

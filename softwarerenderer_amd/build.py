@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(HERE, "libswr_hip.so")
-SOURCES = ["swr_geometry.hip", "swr_sort.hip", "swr_raster.hip", "swr_diff.hip",
+SOURCES = ["swr_geometry.hip", "swr_sort.hip", "swr_raster.hip", "swr_lines.hip", "swr_diff.hip",
            "swr_api.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -172,6 +172,43 @@ int resolve_in(swr_context *c, const void *src, size_t bytes, DevBuf &stage, con
     return SWR_OK;
 }
 
+// Vertex attribute 0 on the device for a drawElements over indices[0 .. n): device data is
+// used in place; host data is copied, sized from buffer_bytes or from the largest index.
+int resolve_attr0(swr_context *c, const int32_t *indices, size_t n, const void **dattr,
+                  uint64_t *attr_bytes)
+{
+    *dattr = nullptr;
+    *attr_bytes = 0;
+    if (c->vs.program == SWR_VS_DUMMY)
+        return SWR_OK;
+    const auto &A = c->attr[0];
+    if (!A.ptr)
+        return fail(c, SWR_ERR_INVALID_ARG, "vertex attribute 0 is not set");
+    if (is_device_ptr(A.ptr)) {
+        *dattr = A.ptr;
+        *attr_bytes = A.bytes;
+        return SWR_OK;
+    }
+    size_t bytes = A.bytes;
+    if (!bytes) {
+        if (is_device_ptr(indices))
+            return fail(c, SWR_ERR_INVALID_ARG,
+                        "host vertex data with device indices needs buffer_bytes");
+        int32_t mx = 0;
+        for (size_t i = 0; i < n; ++i) {
+            if (indices[i] < 0)
+                return fail(c, SWR_ERR_INDEX_RANGE, "negative index %d at %zu", indices[i], i);
+            mx = std::max(mx, indices[i]);
+        }
+        bytes = (size_t)mx * (size_t)A.stride + vs_need_bytes(c->vs);
+    }
+    int rc = resolve_in(c, A.ptr, bytes, c->attr0, dattr);
+    if (rc)
+        return rc;
+    *attr_bytes = bytes;
+    return SWR_OK;
+}
+
 struct TargetMap {
     void *dev[3] = {nullptr, nullptr, nullptr};
     void *host[3] = {nullptr, nullptr, nullptr};
@@ -369,33 +406,10 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
         rc = resolve_in(c, wverts, nverts * sizeof(swr_rasterizer_vertex), c->wverts, &dwv);
         if (rc)
             return rc;
-    } else if (c->vs.program != SWR_VS_DUMMY) {
-        const auto &A = c->attr[0];
-        if (!A.ptr)
-            return fail(c, SWR_ERR_INVALID_ARG, "vertex attribute 0 is not set");
-        if (is_device_ptr(A.ptr)) {
-            dattr = A.ptr;
-            attr_bytes = A.bytes;
-        } else {
-            size_t bytes = A.bytes;
-            if (!bytes) {
-                if (is_device_ptr(indices))
-                    return fail(c, SWR_ERR_INVALID_ARG,
-                                "host vertex data with device indices needs buffer_bytes");
-                int32_t mx = 0;
-                for (size_t i = 0; i < T * 3; ++i) {
-                    if (indices[i] < 0)
-                        return fail(c, SWR_ERR_INDEX_RANGE, "negative index %d at %zu",
-                                    indices[i], i);
-                    mx = std::max(mx, indices[i]);
-                }
-                bytes = (size_t)mx * (size_t)A.stride + need;
-            }
-            rc = resolve_in(c, A.ptr, bytes, c->attr0, &dattr);
-            if (rc)
-                return rc;
-            attr_bytes = bytes;
-        }
+    } else {
+        rc = resolve_attr0(c, indices, T * 3, &dattr, &attr_bytes);
+        if (rc)
+            return rc;
     }
 
     const uint32_t batches = (uint32_t)((T + BATCH - 1) / BATCH);
@@ -571,6 +585,194 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
         return rc;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     collect_stats(c);
+    return SWR_OK;
+}
+
+// Points and lines: drawElements (window_input = false) or the IRasterizer list entries.
+// The count pass publishes the fragment total, which sizes the fragment arena before the
+// emit pass (one host round trip per draw).
+int draw_lines_points(swr_context *c, int mode, const int32_t *indices, size_t count,
+                      bool window_input, const swr_rasterizer_vertex *wverts, size_t nverts)
+{
+    const size_t per = mode == SWR_DRAW_LINE ? 2 : 1;
+    const size_t P = count / per;  // a partial last line is dropped
+    std::memset(&c->stats, 0, sizeof c->stats);
+    c->stats.input_primitives = P;
+    c->stats.tiles = (uint64_t)((c->ps.width + TILE - 1) / TILE) *
+                     (uint64_t)((c->ps.height + TILE - 1) / TILE);
+    if (P == 0 || c->ps.program == SWR_PS_NULL)
+        return SWR_OK;
+    if (P > (size_t)(1u << 30))
+        return fail(c, SWR_ERR_CAPACITY, "too many primitives (%zu)", P);
+    ParamLayout L;
+    if (!raster_layout(c->ps, L))
+        return fail(c, SWR_ERR_UNSUPPORTED,
+                    "pixel program %d does not support z=%d w=%d avars=%d pvars=%d",
+                    c->ps.program, c->ps.interpolate_z, c->ps.interpolate_w, c->ps.avar_count,
+                    c->ps.pvar_count);
+    const swr_pixel_shader_desc &ps = c->ps;
+    const int W = ps.width, H = ps.height;
+    const int tiles_x = (W + TILE - 1) / TILE, tiles_y = (H + TILE - 1) / TILE;
+    const uint32_t tiles = (uint32_t)tiles_x * (uint32_t)tiles_y;
+    if (tiles == 0)
+        return SWR_OK;
+
+    TargetMap tm;
+    int rc = map_targets(c, tm);
+    if (rc)
+        return rc;
+    const void *didx = nullptr, *dattr = nullptr, *dwv = nullptr;
+    uint64_t attr_bytes = 0;
+    rc = resolve_in(c, indices, P * per * sizeof(int32_t), c->idx, &didx);
+    if (rc)
+        return rc;
+    if (window_input)
+        rc = resolve_in(c, wverts, nverts * sizeof(swr_rasterizer_vertex), c->wverts, &dwv);
+    else
+        rc = resolve_attr0(c, indices, P * per, &dattr, &attr_bytes);
+    if (rc)
+        return rc;
+
+    HIP_TRY(c, ensure(c, c->app, P * 4));
+    HIP_TRY(c, ensure(c, c->app_ex, P * 4));
+    HIP_TRY(c, ensure(c, c->scan_tmp, scan_tmp_elems(P) * 8));
+    HIP_TRY(c, ensure(c, c->ranges, (size_t)tiles * 8));
+    HIP_TRY(c, ensure(c, c->status, ST_WORDS * 8));
+
+    LineArgs A;
+    std::memset(&A, 0, sizeof A);
+    GeomArgs &g = A.g;
+    g.indices = (const int32_t *)didx;
+    g.attr0 = (const uint8_t *)dattr;
+    g.stride0 = c->attr[0].stride;
+    g.attr0_bytes = attr_bytes;
+    g.attr0_need = vs_need_bytes(c->vs);
+    std::memcpy(g.mvp, c->vs.mvp, sizeof g.mvp);
+    g.vs_program = c->vs.program;
+    g.vs_na = c->vs.avar_count;
+    g.vs_np = c->vs.pvar_count;
+    g.sc_minx = c->sc[0];
+    g.sc_miny = c->sc[1];
+    g.sc_maxx = c->sc[0] + c->sc[2];
+    g.sc_maxy = c->sc[1] + c->sc[3];
+    // VertexProcessor.cpp:50-53, 373 (host float arithmetic, one rounding per op)
+    volatile float px = (float)c->vp[2] / 2.0f, py = (float)c->vp[3] / 2.0f;
+    g.vp_px = px;
+    g.vp_py = py;
+    g.vp_ox = (float)c->vp[0] + px;
+    g.vp_oy = (float)c->vp[1] + py;
+    volatile float dzs = 0.5f * (c->df - c->dn), dzb = 0.5f * (c->dn + c->df);
+    g.dz_scale = dzs;
+    g.dz_bias = dzb;
+    g.width = W;
+    g.height = H;
+    g.tiles_x = tiles_x;
+    g.tiles_y = tiles_y;
+    g.lay = L;
+    g.window_input = window_input ? 1 : 0;
+    g.wverts = (const swr_rasterizer_vertex *)dwv;
+    g.wvert_count = nverts;
+    g.status = (unsigned long long *)c->status.p;
+    g.ranges = (uint32_t *)c->ranges.p;
+    g.tiles = tiles;
+    A.mode = mode;
+    A.prims = P;
+    A.iz = ps.interpolate_z ? 1 : 0;
+    A.iw = ps.interpolate_w ? 1 : 0;
+    A.count = (uint32_t *)c->app.p;
+    A.first = (const uint32_t *)c->app_ex.p;
+    A.fstride = fragment_stride(L);
+
+    record(c, 0);
+    HIP_TRY(c, launch_lines_count(A, c->stream));
+    record(c, 1);
+    HIP_TRY(c, scan_u32(A.count, (uint32_t *)c->app_ex.p, P, (uint32_t *)c->scan_tmp.p, c->stream));
+    record(c, 2);
+    HIP_TRY(c, hipMemcpyAsync(c->status_host, c->status.p, ST_WORDS * 8, hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->status_host[ST_ERROR] & 4ull)
+        return fail(c, SWR_ERR_CAPACITY, "a line spans more than %d steps", SWR_MAX_LINE_STEPS);
+    rc = check_status(c);
+    if (rc)
+        return rc;
+    const uint64_t F = c->status_host[ST_PAIRS];
+    if (F > (1ull << 31))
+        return fail(c, SWR_ERR_CAPACITY, "too many fragments (%llu)", (unsigned long long)F);
+
+    if (F) {
+        c->pair_cap = std::max<uint64_t>(c->pair_cap, F);
+        HIP_TRY(c, ensure(c, c->prims, F * (size_t)A.fstride * 4));
+        HIP_TRY(c, ensure(c, c->pk_a, c->pair_cap * 4));
+        HIP_TRY(c, ensure(c, c->pv_a, c->pair_cap * 4));
+        HIP_TRY(c, ensure(c, c->pk_b, c->pair_cap * 4));
+        HIP_TRY(c, ensure(c, c->pv_b, c->pair_cap * 4));
+        uint32_t G = (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(1, c->pair_cap / 4096));
+        HIP_TRY(c, ensure(c, c->hist, (size_t)256 * G * 4));
+        HIP_TRY(c, ensure(c, c->hist_ex, (size_t)256 * G * 4));
+        HIP_TRY(c, ensure(c, c->hist_tmp, scan_tmp_elems((uint64_t)256 * G) * 4));
+        A.frags = (float *)c->prims.p;
+        A.keys = (uint32_t *)c->pk_a.p;
+        A.vals = (uint32_t *)c->pv_a.p;
+        HIP_TRY(c, launch_lines_emit(A, c->stream));
+        record(c, 3);
+        SortBuffers sb;
+        sb.keys_a = (uint32_t *)c->pk_a.p;
+        sb.vals_a = (uint32_t *)c->pv_a.p;
+        sb.keys_b = (uint32_t *)c->pk_b.p;
+        sb.vals_b = (uint32_t *)c->pv_b.p;
+        sb.hist = (uint32_t *)c->hist.p;
+        sb.hist_ex = (uint32_t *)c->hist_ex.p;
+        sb.scan_tmp = (uint32_t *)c->hist_tmp.p;
+        sb.ranges = (uint32_t *)c->ranges.p;
+        sb.blocks = G;
+        HIP_TRY(c, sort_pairs(sb, g.status, tiles, c->stream));
+        record(c, 4);
+        HIP_TRY(c, launch_ranges(sb, g.status, tiles, c->stream));
+        record(c, 5);
+
+        RasterArgs r;
+        std::memset(&r, 0, sizeof r);
+        r.prims = (const float *)c->prims.p;
+        r.stride = A.fstride;
+        r.ranges = (const uint32_t *)c->ranges.p;
+        r.vals = sb.sorted_vals;
+        r.status = g.status;
+        r.tiles = (int)tiles;
+        r.tiles_x = tiles_x;
+        r.width = W;
+        r.height = H;
+        r.pitch = ps.pitch;
+        r.sc_minx = g.sc_minx;
+        r.sc_maxx = g.sc_maxx;
+        r.na_set = L.set_na;
+        r.np_set = L.set_np;
+        r.iz = A.iz;
+        r.iw = A.iw;
+        r.t0 = tm.dev[0];
+        r.t1 = tm.dev[1];
+        r.t2 = tm.dev[2];
+        r.log = tm.log;
+        r.log_cap = ps.log_capacity;
+        r.log_count = tm.logcount;
+        r.checker_cells = ps.checker_cells;
+        HIP_TRY(c, launch_lines_raster(ps, L, r, c->stream));
+        record(c, 6);
+        HIP_TRY(c, hipMemcpyAsync(c->status_host, c->status.p, ST_WORDS * 8,
+                                  hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        rc = check_status(c);
+        if (rc)
+            return rc;
+    }
+    rc = unmap_targets(c, tm);
+    if (rc)
+        return rc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->stats.raster_primitives = c->status_host[ST_PRIMS];
+    c->stats.tile_pairs = F;
+    if (c->profiling && F)
+        collect_stats(c);
     return SWR_OK;
 }
 
@@ -782,12 +984,11 @@ int swr_draw_elements(swr_context *c, int mode, size_t count, const int32_t *ind
 {
     if (!c || (count && !indices))
         return fail(c, SWR_ERR_INVALID_ARG, "null indices");
-    if (mode != SWR_DRAW_TRIANGLE) {
-        if (mode == SWR_DRAW_POINT || mode == SWR_DRAW_LINE)
-            return fail(c, SWR_ERR_UNSUPPORTED, "point and line drawing is not built yet");
+    if (mode != SWR_DRAW_TRIANGLE && mode != SWR_DRAW_POINT && mode != SWR_DRAW_LINE)
         return fail(c, SWR_ERR_INVALID_ARG, "bad draw mode %d", mode);
-    }
     (void)hipSetDevice(c->device);
+    if (mode != SWR_DRAW_TRIANGLE)
+        return draw_lines_points(c, mode, indices, count, false, nullptr, 0);
     return draw_triangles(c, indices, count, false, nullptr, 0, true);
 }
 
@@ -819,16 +1020,22 @@ int swr_draw_triangle_list(swr_context *c, const swr_rasterizer_vertex *v, size_
     return draw_triangles(c, idx, n, true, v, nv, true);
 }
 
-int swr_draw_line_list(swr_context *c, const swr_rasterizer_vertex *, size_t, const int32_t *,
-                       size_t)
+int swr_draw_line_list(swr_context *c, const swr_rasterizer_vertex *v, size_t nv,
+                       const int32_t *idx, size_t n)
 {
-    return fail(c, SWR_ERR_UNSUPPORTED, "line lists are not built yet");
+    if (!c || (n && (!v || !idx)))
+        return fail(c, SWR_ERR_INVALID_ARG, "null vertices or indices");
+    (void)hipSetDevice(c->device);
+    return draw_lines_points(c, SWR_DRAW_LINE, idx, n, true, v, nv);
 }
 
-int swr_draw_point_list(swr_context *c, const swr_rasterizer_vertex *, size_t, const int32_t *,
-                        size_t)
+int swr_draw_point_list(swr_context *c, const swr_rasterizer_vertex *v, size_t nv,
+                        const int32_t *idx, size_t n)
 {
-    return fail(c, SWR_ERR_UNSUPPORTED, "point lists are not built yet");
+    if (!c || (n && (!v || !idx)))
+        return fail(c, SWR_ERR_INVALID_ARG, "null vertices or indices");
+    (void)hipSetDevice(c->device);
+    return draw_lines_points(c, SWR_DRAW_POINT, idx, n, true, v, nv);
 }
 
 int swr_batch_create(swr_context *c, const swr_scene *sc, size_t n, swr_batch **out)

@@ -19,6 +19,27 @@ hipError_t launch_geom_count(const GeomArgs &a, hipStream_t s);
 hipError_t launch_geom_emit(const GeomArgs &a, hipStream_t s);
 hipError_t launch_expand(const GeomArgs &a, hipStream_t s, int chunk);
 
+int emit_class(int nvary);  // varyings carried per vertex: 0, 3, 5, 8 or 32
+
+// points and lines (swr_lines.hip)
+struct LineArgs {
+    GeomArgs g;             // vertex input, VertexProcessor / Rasterizer state, layout, status,
+                            // ranges, tiles
+    int mode;               // SWR_DRAW_POINT or SWR_DRAW_LINE
+    uint64_t prims;         // primitives: index pairs (lines) or indices (points)
+    int iz, iw;             // InterpolateZ / InterpolateW: which values the DDA steps
+    uint32_t *count;        // [prims] fragments of each primitive (count pass)
+    const uint32_t *first;  // [prims] exclusive scan of count
+    float *frags;           // fragment records (emit pass)
+    int fstride;            // words per fragment record
+    uint32_t *keys, *vals;  // (tile, fragment) pairs in fragment order
+};
+int fragment_stride(const ParamLayout &L);
+hipError_t launch_lines_count(const LineArgs &a, hipStream_t s);
+hipError_t launch_lines_emit(const LineArgs &a, hipStream_t s);
+hipError_t launch_lines_raster(const swr_pixel_shader_desc &ps, const ParamLayout &L,
+                               const RasterArgs &a, hipStream_t s);
+
 // scans and sort (swr_sort.hip)
 uint64_t scan_tmp_elems(uint64_t n);
 hipError_t scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *tmp, hipStream_t s);

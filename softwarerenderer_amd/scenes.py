@@ -257,6 +257,40 @@ def clip_scene(ntri: int = 2000, seed: int = 3, width: int = 256, height: int = 
                  {"name": "clip", "ntri": ntri, "seed": seed})
 
 
+# RasterizerVertex, IRasterizer.h:42-53 (x, y, z, w, avar[16], pvar[16])
+RASTERIZER_VERTEX = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("w", "<f4"),
+                              ("avar", "<f4", (16,)), ("pvar", "<f4", (16,))])
+
+
+def edge_indices(tri_indices: np.ndarray) -> np.ndarray:
+    """Line-list indices of every triangle's three edges (v0 v1, v1 v2, v2 v0), in triangle
+    order: the wireframe of a triangle list."""
+    t = np.asarray(tri_indices, np.int32).reshape(-1, 3)
+    return np.ascontiguousarray(np.stack([t[:, [0, 1]], t[:, [1, 2]], t[:, [2, 0]]], 1).reshape(-1))
+
+
+def window_lines(n: int, seed: int, width: int, height: int, margin: float = 12.0,
+                 long_frac: float = 0.2, grid: bool = False) -> np.ndarray:
+    """2n window-space RasterizerVertex endpoints (x, y, z, w, 16 avar, 16 pvar): short lines
+    around random points plus a fraction crossing the whole target, endpoints up to `margin`
+    pixels outside it; `grid` puts endpoints on half-pixel positions (truncation ties)."""
+    rng = np.random.default_rng(seed)
+    v = np.zeros(2 * n, RASTERIZER_VERTEX)
+    c = rng.uniform([-margin, -margin], [width + margin, height + margin], (n, 2))
+    d = rng.normal(0.0, 6.0, (n, 2))
+    far = rng.random(n) < long_frac
+    d[far] = rng.uniform([-width, -height], [width, height], (int(far.sum()), 2))
+    p = np.stack([c, c + d], 1).reshape(-1, 2)
+    if grid:
+        p = np.round(p * 2) / 2
+    v["x"], v["y"] = p[:, 0], p[:, 1]
+    v["z"] = rng.random(2 * n)
+    v["w"] = rng.uniform(0.5, 2.0, 2 * n)
+    v["avar"] = rng.random((2 * n, 16))
+    v["pvar"] = rng.random((2 * n, 16))
+    return v
+
+
 def icosphere(subdivisions: int = 6):
     """Unit icosphere: (vertices [N, 3] float64, triangles [T, 3] int32), outward faces
     counter-clockwise.  Subdivision s gives 10 * 4^s + 2 vertices and 20 * 4^s triangles

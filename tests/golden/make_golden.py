@@ -219,6 +219,46 @@ def make_logs(ref):
     return out
 
 
+# ---- points and lines ------------------------------------------------------------------------
+def line_cases():
+    """(name, draw kwargs, pipeline) of the point / line fixtures; shared with the tests."""
+    sph = scenes.sphere_scene()
+    clip = scenes.clip_scene(ntri=1500, seed=31, width=160, height=120)
+    wl = scenes.window_lines(500, seed=7, width=96, height=64)
+    widx = np.arange(wl.size, dtype=np.int32)
+    widx[::8] = -1  # a line is skipped by its first index only
+    cases = []
+    for mode, tag in ((O.DRAW_LINE, "line"), (O.DRAW_POINT, "point")):
+        idx = scenes.edge_indices(sph.indices) if mode == O.DRAW_LINE else sph.indices
+        cases.append((f"wire_sphere_{tag}", dict(scene=sph, mode=mode, indices=idx),
+                      O.Pipeline(640, 480, mvp=sph.mvp, ps_program=O.PS_DEPTH_CHECKER,
+                                 viewport=(-20, 10, 640, 480), scissor=(5, 3, 600, 470))))
+        idx = scenes.edge_indices(clip.indices) if mode == O.DRAW_LINE else clip.indices
+        cases.append((f"clip_{tag}_log", dict(scene=clip, mode=mode, indices=idx),
+                      O.Pipeline(160, 120, vs_program=O.VS_RAW, vs_counts=(1, 3, 2),
+                                 ps_program=O.PS_LOG)))
+        cases.append((f"list_{tag}_log", dict(vertices=wl, indices=widx, prim=tag),
+                      O.Pipeline(96, 64, scissor=(-8, -8, 120, 90), ps_program=O.PS_LOG,
+                                 ps_a=4, ps_p=3)))
+    inputs = {"sphere": scene_digest(sph), "clip": scene_digest(clip), "window": digest(wl)}
+    return cases, inputs
+
+
+def make_lines(ref):
+    out = {}
+    cases, inputs = line_cases()
+    for k, v in inputs.items():
+        out["input_" + k] = np.array(v)
+    for name, kw, pipe in cases:
+        if name.endswith("_log"):
+            rec = sorted_log(run_cpu(ref, pipe, log_capacity=300_000, **kw))
+            compact(out, name, rec)
+        else:
+            t = run_cpu(ref, pipe, **kw)
+            out[name] = np.array(digest(t.t0, t.t1))
+    return out
+
+
 def main():
     if not O.ref_available():
         O.build()
@@ -229,6 +269,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "scenes.npz"),
                         **{k: np.array(v) for k, v in sc.items()})
     np.savez_compressed(os.path.join(HERE, "logs.npz"), **make_logs(ref))
+    np.savez_compressed(os.path.join(HERE, "lines.npz"), **make_lines(ref))
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -180,3 +180,38 @@ def test_gpu_invocation_logs(gpu):
     d, cases = _log_cases()
     for name, kw, pipe in cases:
         _check_log(d, name, _draw_log(run_gpu, kw, pipe))
+
+
+# ---- points and lines (tests/golden/lines.npz) ------------------------------------------------
+
+def _line_cases():
+    import sys
+    sys.path.insert(0, GOLDEN)
+    from make_golden import line_cases
+    d = load("lines.npz")
+    cases, inputs = line_cases()
+    for k, v in inputs.items():
+        assert v == str(d["input_" + k]), f"{k} input generator changed"
+    return d, cases
+
+
+def _check_line_case(d, name, draw, kw, pipe):
+    if name.endswith("_log"):
+        _check_log(d, name, sorted_log(draw(pipe, log_capacity=300_000, **kw)))
+    else:
+        t = draw(pipe, **kw)
+        assert digest(t.t0, t.t1) == str(d[name]), name
+
+
+def test_line_point_fixtures(cpu_oracle):
+    d, cases = _line_cases()
+    for name, kw, pipe in cases:
+        _check_line_case(d, name, lambda *a, **k: run_cpu(cpu_oracle, *a, **k), kw, pipe)
+    assert int(d["clip_line_log_n"]) > 100_000 and int(d["list_point_log_n"]) > 400
+
+
+@pytest.mark.gpu
+def test_gpu_line_point_fixtures(gpu):
+    d, cases = _line_cases()
+    for name, kw, pipe in cases:
+        _check_line_case(d, name, run_gpu, kw, pipe)

@@ -125,3 +125,57 @@ def test_soup(cpu_oracle, cpu_ref):
     for mode in MODES:
         pipe = O.Pipeline(1920, 1080, raster_mode=mode, mvp=sc.mvp)
         assert_same(run_cpu(cpu_ref, pipe, sc), run_cpu(cpu_oracle, pipe, sc), "soup")
+
+
+# ---- lines and points ---------------------------------------------------------------------
+# drawElements(Line / Point): LineClipper / clipPoints, transformVertices, then the DDA of
+# Rasterizer.h:175-222 and drawPointTemplate (:149-158); the list entry points directly.
+
+# (program, InterpolateZ, InterpolateW, AVarCount, PVarCount); the vertex program writes the
+# same counts (a clipped endpoint leaves the others undefined, PolyClipper.h:34-48)
+LP_PROGS = [(O.PS_LOG, True, True, 3, 2), (O.PS_LOG, False, False, 2, 2),
+            (O.PS_HASH, True, True, 4, 4), (O.PS_DEPTH_CHECKER, True, True, 3, 2)]
+
+
+def _lp_pipe(w, h, prog, z, wv, a, p, **kw):
+    return O.Pipeline(w, h, ps_program=prog, ps_z=z, ps_w=wv, ps_a=a, ps_p=p, **kw)
+
+
+@pytest.mark.parametrize("prog", LP_PROGS, ids=lambda t: f"p{t[0]}z{int(t[1])}w{int(t[2])}a{t[3]}p{t[4]}")
+@pytest.mark.parametrize("mode", [O.DRAW_LINE, O.DRAW_POINT])
+def test_lines_points_clip(cpu_oracle, cpu_ref, prog, mode):
+    sc = scenes.clip_scene(ntri=1500, seed=31, width=160, height=120, navar=prog[3],
+                           npvar=prog[4])
+    idx = scenes.edge_indices(sc.indices) if mode == O.DRAW_LINE else sc.indices
+    pipe = _lp_pipe(160, 120, *prog, vs_program=O.VS_RAW, vs_counts=(1, prog[3], prog[4]))
+    cap = 400_000 if prog[0] == O.PS_LOG else 0
+    a = run_cpu(cpu_ref, pipe, sc, mode=mode, indices=idx, log_capacity=cap)
+    b = run_cpu(cpu_oracle, pipe, sc, mode=mode, indices=idx, log_capacity=cap)
+    if prog[0] == O.PS_LOG:
+        assert int(a.log_count[0]) > 1000
+    assert_same(a, b, f"mode={mode} prog={prog}")
+
+
+@pytest.mark.parametrize("mode", [O.DRAW_LINE, O.DRAW_POINT])
+def test_wire_sphere(cpu_oracle, cpu_ref, sphere, mode):
+    idx = scenes.edge_indices(sphere.indices) if mode == O.DRAW_LINE else sphere.indices
+    for prog in (O.PS_DEPTH_CHECKER, O.PS_HASH):
+        pipe = O.Pipeline(640, 480, mvp=sphere.mvp, ps_program=prog,
+                          viewport=(-20, 10, 640, 480), scissor=(5, 3, 600, 470))
+        a = run_cpu(cpu_ref, pipe, sphere, mode=mode, indices=idx)
+        b = run_cpu(cpu_oracle, pipe, sphere, mode=mode, indices=idx)
+        assert_same(a, b, f"wire sphere mode={mode} prog={prog}")
+
+
+@pytest.mark.parametrize("grid", [False, True])
+@pytest.mark.parametrize("prim", ["line", "point"])
+def test_line_point_list(cpu_oracle, cpu_ref, grid, prim):
+    v = scenes.window_lines(500, seed=7 + grid, width=96, height=64, grid=grid)
+    idx = np.arange(v.size, dtype=np.int32)
+    idx[::8] = -1  # skipped primitives: a line by its first index (Rasterizer.h:119, 128)
+    for scissor in ((0, 0, 96, 64), (-8, -8, 120, 90), (10, 6, 50, 40)):
+        pipe = O.Pipeline(96, 64, scissor=scissor, ps_program=O.PS_LOG, ps_a=4, ps_p=3)
+        a = run_cpu(cpu_ref, pipe, vertices=v, indices=idx, prim=prim, log_capacity=200_000)
+        b = run_cpu(cpu_oracle, pipe, vertices=v, indices=idx, prim=prim, log_capacity=200_000)
+        assert int(a.log_count[0]) > 100
+        assert_same(a, b, f"{prim} list scissor={scissor}")
