@@ -74,6 +74,7 @@ struct swr_context {
     swr_draw_stats stats{};
     bool async_pending = false;
     swr_tuning tune{};
+    uint64_t diff_tag[8] = {};
 };
 
 // A batched draw's device tables (include/swr_hip.h, swr_batch_create).
@@ -789,6 +790,8 @@ hipStream_t context_stream(swr_context *c)
 int context_fail(swr_context *c, int code, const char *msg) { return fail(c, code, "%s", msg); }
 
 bool context_is_device_ptr(const void *p) { return is_device_ptr(p); }
+
+uint64_t *context_diff_tag(swr_context *c) { return c->diff_tag; }
 
 void *context_zeroed_scratch(swr_context *c, int slot, size_t bytes)
 {

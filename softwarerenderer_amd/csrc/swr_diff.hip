@@ -26,17 +26,27 @@
 //                  over a pixel centre, and the weights sum to one, so the sum does too.
 //                  Blends use the colours before antialiasing and add up.
 //
-// The backward pass recomputes the same quantities per pixel (gather form: every pixel
-// re-evaluates the pairs it belongs to), so the only per-pixel state kept from the forward
-// pass is the visibility buffer, the barycentrics and the colour.  Gradients reach the
-// vertex attributes through b_i, the clip coordinates through D_i (barycentrics) and
-// through t (the silhouette crossing), and are accumulated with float atomics.
+// Passes.  Forward: k_diff_interp (barycentrics, colour, image = colour, and the cheap part
+// of every pair's decision; pairs that need the silhouette walk go to a queue),
+// k_diff_march (one lane per queued pair: the walk, the stored decision, per-tile flags),
+// k_diff_blend (only tiles with an active pair: the antialiasing added to the image).
+// Backward: k_diff_backward<INTERIOR> (every pixel: its own triangle's interpolation under
+// dL/dimage) and k_diff_backward<EDGES> (flagged tiles only: what the antialiasing adds),
+// both reading the stored decisions; gradients reach the vertex attributes through b_i,
+// the clip coordinates through D_i (barycentrics) and through t (the silhouette crossing),
+// accumulated per tile in LDS and flushed with float atomics.  A backward whose buffers are
+// not those of the last forward on the context takes decisions again itself
+// (k_diff_backward<ALL>).  The walk is queued because few pairs need it (about 1% on a
+// 41k-vertex sphere at 512 x 512) and a wave that holds one waits on its chain of dependent
+// loads with all 64 lanes.
 //
 // Precision.  D_i is a cancellation-prone quantity: for a small or grazing triangle the
 // three clip vectors are nearly parallel, and fp32 loses up to 1e-4 of D.  The values
 // (D, b, t and every derivative) are therefore evaluated in fp64 from the fp32 inputs; the
 // kernels are bound by memory and atomics, not by the fp64 rate.  The discrete choices of
 // the antialiasing pass use fp32, so they are reproducible by a CPU restatement.
+#include <cstring>
+
 #include "swr_common.h"
 #include "swr_internal.h"
 
@@ -60,12 +70,23 @@ struct DiffArgs {
     float *gclip;
     float *gattr;  // per-view partial attribute gradients [V][N][C], summed afterwards
     float *gattr_out;
+    // antialiasing decisions of the forward pass, two pairs per pixel (to the right, below):
+    // flags (PairLds::flags | edge << 4) and {crossing t, triangle}; reused by the backward
+    uint8_t *pflag;
+    int2 *pdata;
+    uint32_t *tflag;   // per (view, tile): bit 0 some pair of the tile's pixels is active,
+                       // bit 1 some pair to the right of its last column, bit 2 some pair
+                       // below its last row
+    uint32_t *queue;   // pairs that need the silhouette walk (pair index 2 * pixel + dir)
+    uint32_t *qcount;
+    int pairs;     // 0: evaluate, 1: evaluate and store, 2: load the stored decisions
 };
 
 // Pixel of this thread: a workgroup of 256 threads covers one 16x16 tile of one view, so a
 // wave (16x4 pixels) reads its neighbours, triangles and vertices from a small area.
 struct PixelIdx {
     int view, px, py;
+    int tile;  // (view, tile) index of the workgroup
     size_t i;
 };
 
@@ -82,6 +103,7 @@ __device__ inline int xcd_order(unsigned b, unsigned n)
 __device__ inline bool pixel_of_thread(const DiffArgs &d, PixelIdx &q)
 {
     const int b = xcd_order(blockIdx.x, gridDim.x);
+    q.tile = b;
     q.view = b / d.tiles_per_view;
     const int t = b - q.view * d.tiles_per_view;
     const int ty = t / d.tiles_x, tx = t - ty * d.tiles_x;
@@ -153,6 +175,7 @@ struct Pair {
     bool horiz;       // horizontal pair
     float tf, s;      // fp32 crossing (decisions), orientation of the triangle
     int tri, vj, vk;  // triangle and the edge's vertices
+    int edge;         // the edge's index in the triangle (opposite vertex)
     int front_is_p;
 };
 
@@ -212,15 +235,17 @@ constexpr int MARCH_STEPS = 8;
 // triangle (no crossing: inactive), or MARCH_STEPS triangles were visited (inactive).  The
 // exit edge of a triangle is the one the segment crosses first while leaving its half-plane
 // (edge value < 0 at Q and falling from F to Q).
-__device__ inline Pair eval_pair(const DiffArgs &d, int view, int px, int py, int qx, int qy)
+// The cheap part of a pair's decision: false when no silhouette can separate p and q (same
+// triangle, no front triangle, or two visible triangles of a culled draw sharing a vertex);
+// otherwise the front pixel and its triangle, where the walk starts.
+__device__ inline bool pair_start(const DiffArgs &d, int view, int px, int py, int qx, int qy,
+                                  bool &pf_out, int &cur_out)
 {
-    Pair r;
-    r.active = false;
     const size_t base = (size_t)view * d.W * d.H;
     const size_t ip = base + (size_t)py * d.W + px, iq = base + (size_t)qy * d.W + qx;
     const uint32_t a = d.ids[ip], b = d.ids[iq];
     if (a == b)
-        return r;
+        return false;
     bool pf;
     if (a == 0)
         pf = false;
@@ -232,7 +257,7 @@ __device__ inline Pair eval_pair(const DiffArgs &d, int view, int px, int py, in
     }
     int cur = (int)(pf ? a : b) - 1;
     if (cur < 0 || cur >= d.T)
-        return r;
+        return false;
     if (d.culled && a != 0 && b != 0) {
         // two visible triangles of a culled draw that share a vertex both face the viewer:
         // no silhouette is taken to separate their pixels
@@ -244,9 +269,25 @@ __device__ inline Pair eval_pair(const DiffArgs &d, int view, int px, int py, in
             const bool share = c0 == o0 || c0 == o1 || c0 == o2 || c1 == o0 || c1 == o1 ||
                                c1 == o2 || c2 == o0 || c2 == o1 || c2 == o2;
             if (share)
-                return r;
+                return false;
         }
     }
+    pf_out = pf;
+    cur_out = cur;
+    return true;
+}
+
+__device__ inline Pair eval_pair(const DiffArgs &d, int view, int px, int py, int qx, int qy)
+{
+    Pair r;
+    r.active = r.p_blended = r.horiz = false;
+    r.tf = 0.0f;
+    r.s = 1.0f;
+    r.tri = r.vj = r.vk = r.edge = r.front_is_p = 0;
+    bool pf;
+    int cur;
+    if (!pair_start(d, view, px, py, qx, qy, pf, cur))
+        return r;
     const bool horiz = py == qy;
     const float2 F = pf ? make_float2(px + 0.5f, py + 0.5f) : make_float2(qx + 0.5f, qy + 0.5f);
     const float2 Q = pf ? make_float2(qx + 0.5f, qy + 0.5f) : make_float2(px + 0.5f, py + 0.5f);
@@ -296,6 +337,7 @@ __device__ inline Pair eval_pair(const DiffArgs &d, int view, int px, int py, in
             r.tf = best;
             r.s = s;
             r.tri = cur;
+            r.edge = be;
             r.vj = vid[j];
             r.vk = vid[k];
             r.front_is_p = pf;
@@ -397,9 +439,15 @@ __device__ inline void bary_color(const DiffArgs &d, const Bary &B, double *col)
 // pixels of a tile touch a few dozen vertices, so this replaces most global float atomics
 // (which serialise on shared vertices) by LDS atomics.  A vertex that finds no slot after
 // ACC_PROBES probes goes straight to global memory.
-constexpr int ACC_SLOTS = 512;   // power of two
+constexpr int ACC_SLOTS = 256;   // power of two
 constexpr int ACC_PROBES = 32;
 constexpr int ACC_VALS = 8;      // clip x, y, w, attributes 0..4
+// Each value is kept in four copies, picked by the pixel's (x, y) parity: the pixels of one
+// triangle are neighbours, and four copies cut the lanes of a wave that add into the same
+// LDS word (which serialise) about four times.  The flush sums the copies.
+constexpr int ACC_COPIES = 4;
+
+__device__ inline int acc_copy() { return (int)((threadIdx.x & 1u) | ((threadIdx.x >> 3) & 2u)); }
 
 struct VertexAcc {
     int *key;
@@ -408,7 +456,7 @@ struct VertexAcc {
 
 __device__ inline int acc_slot(const VertexAcc &A, int v)
 {
-    const unsigned h = ((unsigned)v * 2654435761u) >> 23;  // 9 bits: ACC_SLOTS
+    const unsigned h = ((unsigned)v * 2654435761u) >> 24;  // 8 bits: ACC_SLOTS
 #pragma unroll 1
     for (int probe = 0; probe < ACC_PROBES; ++probe) {
         const int sl = (int)((h + (unsigned)probe) & (ACC_SLOTS - 1));
@@ -429,9 +477,10 @@ __device__ inline void acc_clip(const DiffArgs &d, const VertexAcc &A, int view,
 {
     const int sl = acc_slot(A, v);
     if (sl >= 0) {
-        atomicAdd(&A.val[sl * ACC_VALS + 0], gx);
-        atomicAdd(&A.val[sl * ACC_VALS + 1], gy);
-        atomicAdd(&A.val[sl * ACC_VALS + 2], gw);
+        float *x = &A.val[sl * ACC_VALS * ACC_COPIES + acc_copy()];
+        atomicAdd(x + 0 * ACC_COPIES, gx);
+        atomicAdd(x + 1 * ACC_COPIES, gy);
+        atomicAdd(x + 2 * ACC_COPIES, gw);
     } else {
         float *g = d.gclip + ((size_t)view * d.N + v) * 4;
         atomicAdd(g + 0, gx);
@@ -447,13 +496,13 @@ __device__ inline void acc_vertex(const DiffArgs &d, const VertexAcc &A, int vie
 {
     const int sl = acc_slot(A, v);
     if (sl >= 0) {
-        float *x = &A.val[sl * ACC_VALS];
-        atomicAdd(x + 0, gx);
-        atomicAdd(x + 1, gy);
-        atomicAdd(x + 2, gw);
+        float *x = &A.val[sl * ACC_VALS * ACC_COPIES + acc_copy()];
+        atomicAdd(x + 0 * ACC_COPIES, gx);
+        atomicAdd(x + 1 * ACC_COPIES, gy);
+        atomicAdd(x + 2 * ACC_COPIES, gw);
 #pragma unroll
         for (int c = 0; c < C; ++c)
-            atomicAdd(x + 3 + c, ga[c]);
+            atomicAdd(x + (3 + c) * ACC_COPIES, ga[c]);
     } else {
         float *g = d.gclip + ((size_t)view * d.N + v) * 4;
         atomicAdd(g + 0, gx);
@@ -472,7 +521,12 @@ __device__ inline void acc_flush(const DiffArgs &d, const VertexAcc &A, int view
         const int v = A.key[sl];
         if (v < 0)
             continue;
-        const float *x = &A.val[sl * ACC_VALS];
+        float x[ACC_VALS];
+#pragma unroll
+        for (int k = 0; k < ACC_VALS; ++k) {
+            const float4 q = *reinterpret_cast<const float4 *>(&A.val[(sl * ACC_VALS + k) * ACC_COPIES]);
+            x[k] = (q.x + q.y) + (q.z + q.w);
+        }
         float *g = d.gclip + ((size_t)view * d.N + v) * 4;
         if (x[0] != 0.0f) atomicAdd(g + 0, x[0]);
         if (x[1] != 0.0f) atomicAdd(g + 1, x[1]);
@@ -542,9 +596,36 @@ __global__ void __launch_bounds__(256) k_diff_interp(DiffArgs d)
     }
     reinterpret_cast<float2 *>(d.bary)[i] = bb;
 #pragma unroll
-    for (int c = 0; c < C; ++c)
+    for (int c = 0; c < C; ++c) {
         d.color[i * C + c] = (float)col[c];
+        d.image[i * C + c] = (float)col[c];  // the blend pass adds the antialiasing
+    }
+    if (threadIdx.x == 0)
+        d.tflag[pi.tile] = 0u;
+    // this pixel's two pairs (to the right, below): decided here unless a silhouette walk
+    // is needed, which is queued for k_diff_march
+    const unsigned lane = threadIdx.x & 63u;
+#pragma unroll
+    for (int dir = 0; dir < 2; ++dir) {
+        const int qx = px + (dir == 0), qy = py + (dir == 1);
+        bool pf;
+        int cur;
+        const bool walk = qx < d.W && qy < d.H && pair_start(d, view, px, py, qx, qy, pf, cur);
+        d.pflag[2 * i + dir] = 0;
+        const unsigned long long m = __ballot(walk);
+        if (m) {
+            const unsigned leader = (unsigned)__builtin_ctzll(m);
+            uint32_t base = 0;
+            if (lane == leader)
+                base = atomicAdd(d.qcount, (uint32_t)__popcll(m));
+            base = __shfl(base, (int)leader, 64);
+            if (walk)
+                d.queue[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] =
+                    (uint32_t)(2 * i + dir);
+        }
+    }
 }
+
 
 __device__ inline bool neighbour(const DiffArgs &d, int px, int py, int k, int &qx, int &qy)
 {
@@ -560,16 +641,42 @@ __device__ inline bool neighbour(const DiffArgs &d, int px, int py, int k, int &
 // v[ly + 1][lx] the pair (lx, ly)-(lx, ly + 1), both seen from the left / upper pixel; the
 // other pixel reads it with the roles swapped.
 struct PairLds {
-    int flags;  // 1 active, 2 first pixel blended, 4 first pixel in front, 8 s < 0
+    int flags;  // 1 active, 2 first pixel blended, 4 first pixel in front, 8 s < 0,
+                // bits 4-5 the edge index
     float tf;
     int tri, vj, vk;
 };
 
+// Stored form of a pair: the flags byte, and {t, triangle} when active; the edge's vertices
+// are re-read from the triangle on load.
+__device__ inline void store_pair(const DiffArgs &d, size_t k, const PairLds &x)
+{
+    d.pflag[k] = (uint8_t)x.flags;
+    if (x.flags & 1)
+        d.pdata[k] = make_int2(__float_as_int(x.tf), x.tri);
+}
+
+__device__ inline PairLds load_pair(const DiffArgs &d, size_t k)
+{
+    PairLds x;
+    const int f = d.pflag[k];
+    x.flags = f;
+    if (f & 1) {
+        const int2 q = d.pdata[k];
+        const int e = f >> 4;
+        x.tf = __int_as_float(q.x);
+        x.tri = q.y;
+        x.vj = d.tris[3 * x.tri + (e + 1) % 3];
+        x.vk = d.tris[3 * x.tri + (e + 2) % 3];
+    }
+    return x;
+}
+
 __device__ inline PairLds pack_pair(const Pair &p)
 {
     PairLds x;
-    x.flags = (p.active ? 1 : 0) | (p.p_blended ? 2 : 0) | (p.front_is_p ? 4 : 0) |
-              (p.s < 0.0f ? 8 : 0);
+    x.flags = (p.active ? 1 | (p.edge << 4) : 0) | (p.p_blended ? 2 : 0) |
+              (p.front_is_p ? 4 : 0) | (p.s < 0.0f ? 8 : 0);
     x.tf = p.tf;
     x.tri = p.tri;
     x.vj = p.vj;
@@ -592,6 +699,50 @@ __device__ inline Pair unpack_pair(const PairLds &x, bool swap, bool horiz)
     return p;
 }
 
+__global__ void k_diff_clear(DiffArgs d)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        *d.qcount = 0u;
+}
+
+// The queued pairs, one lane each: the silhouette walk, the stored decision, and the tile
+// flags of active pairs.
+__global__ void __launch_bounds__(256) k_diff_march(DiffArgs d)
+{
+    const uint32_t n = *d.qcount;
+    const size_t HW = (size_t)d.W * d.H;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const uint32_t k = d.queue[j];
+        const size_t i = k >> 1;
+        const int dir = (int)(k & 1u);
+        const int view = (int)(i / HW);
+        const size_t rem = i - (size_t)view * HW;
+        const int py = (int)(rem / (size_t)d.W), px = (int)(rem - (size_t)py * d.W);
+        const Pair pr = eval_pair(d, view, px, py, px + (dir == 0), py + (dir == 1));
+        store_pair(d, k, pack_pair(pr));
+        if (pr.active) {
+            const int tx = px >> 4, ty = py >> 4;
+            const uint32_t bits = 1u | (dir == 0 && (px & 15) == 15 ? 2u : 0u) |
+                                  (dir == 1 && (py & 15) == 15 ? 4u : 0u);
+            atomicOr(&d.tflag[(size_t)view * d.tiles_per_view + ty * d.tiles_x + tx], bits);
+        }
+    }
+}
+
+// Whether (view, tile) b has an active pair: one of its pixels' own pairs, or a pair across
+// its left or top border (owned by the neighbouring tile's last column or row).
+__device__ inline bool tile_has_pairs(const DiffArgs &d, int b)
+{
+    const int t = b % d.tiles_per_view;
+    const int tx = t % d.tiles_x, ty = t / d.tiles_x;
+    uint32_t f = d.tflag[b] & 1u;
+    if (tx > 0)
+        f |= d.tflag[b - 1] & 2u;
+    if (ty > 0)
+        f |= d.tflag[b - d.tiles_x] & 4u;
+    return f != 0;
+}
+
 constexpr int TP_H = 16 * 17, TP_V = 17 * 16;
 
 __device__ inline PairLds eval_packed(const DiffArgs &d, int view, int ax, int ay, int bx, int by)
@@ -605,12 +756,34 @@ __device__ inline PairLds eval_packed(const DiffArgs &d, int view, int ax, int a
 }
 
 // Fills the tile's pair table; every thread of the workgroup calls it, then a barrier.
-__device__ inline void tile_pairs(const DiffArgs &d, const PixelIdx &pi, PairLds *h, PairLds *v)
+// d.pairs: 0 evaluate, 1 evaluate and store each pixel's own two pairs, 2 load them.
+template <bool LOAD = false>
+__device__ inline void tile_pairs(const DiffArgs &d, const PixelIdx &pi, bool valid, PairLds *h,
+                                  PairLds *v)
 {
     const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
     const int px = pi.px, py = pi.py, view = pi.view;
-    h[ly * 17 + lx + 1] = eval_packed(d, view, px, py, px + 1, py);
-    v[(ly + 1) * 16 + lx] = eval_packed(d, view, px, py, px, py + 1);
+    if (LOAD) {
+        PairLds none;
+        none.flags = 0;
+        const size_t i = ((size_t)view * d.H + py) * d.W + px;  // valid pixels only
+        h[ly * 17 + lx + 1] = valid ? load_pair(d, 2 * i) : none;
+        v[(ly + 1) * 16 + lx] = valid ? load_pair(d, 2 * i + 1) : none;
+        if (lx == 0)
+            h[ly * 17] = valid && px > 0 ? load_pair(d, 2 * (i - 1)) : none;
+        if (ly == 0)
+            v[lx] = valid && py > 0 ? load_pair(d, 2 * (i - d.W) + 1) : none;
+        return;
+    }
+    const PairLds ph = eval_packed(d, view, px, py, px + 1, py);
+    const PairLds pv = eval_packed(d, view, px, py, px, py + 1);
+    h[ly * 17 + lx + 1] = ph;
+    v[(ly + 1) * 16 + lx] = pv;
+    if (d.pairs == 1 && valid) {
+        const size_t i = ((size_t)view * d.H + py) * d.W + px;
+        store_pair(d, 2 * i, ph);
+        store_pair(d, 2 * i + 1, pv);
+    }
     if (lx == 0)
         h[ly * 17] = eval_packed(d, view, px - 1, py, px, py);
     if (ly == 0)
@@ -637,13 +810,15 @@ __device__ inline Pair tile_pair(const PairLds *h, const PairLds *v, int k)
 //                           so the image stays continuous; on the edge, E is a convex mix of
 //                           the edge's endpoint colours, so it stays bounded.
 template <int C>
-__global__ void __launch_bounds__(256) k_diff_aa(DiffArgs d)
+__global__ void __launch_bounds__(256) k_diff_blend(DiffArgs d)
 {
     __shared__ PairLds s_h[TP_H], s_v[TP_V];
     const size_t HW = (size_t)d.W * d.H;
     PixelIdx pi;
+    if (!tile_has_pairs(d, xcd_order(blockIdx.x, gridDim.x)))
+        return;  // the image is the colour here (uniform over the workgroup)
     const bool valid = pixel_of_thread(d, pi);
-    tile_pairs(d, pi, s_h, s_v);
+    tile_pairs<true>(d, pi, valid, s_h, s_v);
     __syncthreads();
     if (!valid)
         return;
@@ -694,7 +869,12 @@ __device__ inline void scatter_window_grad(const DiffArgs &d, const VertexAcc &A
              (float)((-gx * sx * h.x + gy * sy * h.y) * iw * iw));
 }
 
-template <int C>
+// Which part of a pixel's gradient backward_pixel computes: everything, only the
+// interpolation of its own triangle under dL/dimage, or only what the antialiasing adds
+// (the interpolation backward is linear in the colour gradient, so the two parts add up).
+enum BwdPart { BWD_ALL, BWD_INTERIOR, BWD_EDGES };
+
+template <int C, int PART>
 __device__ inline void backward_pixel(const DiffArgs &d, const VertexAcc &A, const PixelIdx &pi,
                                       const PairLds *s_h, const PairLds *s_v)
 {
@@ -704,12 +884,14 @@ __device__ inline void backward_pixel(const DiffArgs &d, const VertexAcc &A, con
     double gp[C], gc[C], cp[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-        gc[c] = gp[c] = d.gimage[i * C + c];
-        cp[c] = d.color[i * C + c];
+        gp[c] = d.gimage[i * C + c];
+        gc[c] = PART == BWD_EDGES ? 0.0 : gp[c];
+        cp[c] = PART == BWD_INTERIOR ? 0.0 : d.color[i * C + c];
     }
+    bool touched = PART != BWD_EDGES;
 
     // antialiasing: colour gradients (gather) and silhouette crossings (scatter)
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 4 && PART != BWD_INTERIOR; ++k) {
         int qx, qy;
         if (!neighbour(d, px, py, k, qx, qy))
             continue;
@@ -726,6 +908,7 @@ __device__ inline void backward_pixel(const DiffArgs &d, const VertexAcc &A, con
 #pragma unroll
                 for (int c = 0; c < C; ++c)
                     gc[c] += w * d.gimage[iq * C + c];
+                touched = true;
             }
             continue;
         }
@@ -760,6 +943,7 @@ __device__ inline void backward_pixel(const DiffArgs &d, const VertexAcc &A, con
             gc[c] -= w * gp[c];
             dot += gp[c] * (src[c] - cp[c]);
         }
+        touched = true;
         const double dldt = pv.a * (t_low ? -dot : dot) + dedt, dlda = wt * dot;
         scatter_window_grad(d, A, view, pr.vj, dldt * pv.tj.x + dlda * pv.aj.x,
                             dldt * pv.tj.y + dlda * pv.aj.y);
@@ -768,6 +952,8 @@ __device__ inline void backward_pixel(const DiffArgs &d, const VertexAcc &A, con
     }
 
     // barycentric interpolation of the pixel's own triangle
+    if (!touched)
+        return;
     const uint32_t id = d.ids[i];
     Bary B;
     if (id == 0 || (int)id > d.T || !bary_at(d, view, (int)id - 1, px, py, B))
@@ -777,25 +963,26 @@ __device__ inline void backward_pixel(const DiffArgs &d, const VertexAcc &A, con
 
 // Gradients of one 16x16 tile: pair decisions in LDS, vertex gradients accumulated in LDS
 // and flushed once per vertex.
-template <int C>
+template <int C, int PART>
 __global__ void __launch_bounds__(256) k_diff_backward(DiffArgs d)
 {
+    if (PART == BWD_EDGES && !tile_has_pairs(d, xcd_order(blockIdx.x, gridDim.x)))
+        return;  // no silhouette in this tile (uniform over the workgroup)
     __shared__ int s_key[ACC_SLOTS];
-    __shared__ float s_val[ACC_SLOTS * ACC_VALS];
+    __shared__ __align__(16) float s_val[ACC_SLOTS * ACC_VALS * ACC_COPIES];
     __shared__ PairLds s_h[TP_H], s_v[TP_V];
-    for (int sl = threadIdx.x; sl < ACC_SLOTS; sl += blockDim.x) {
+    for (int sl = threadIdx.x; sl < ACC_SLOTS; sl += blockDim.x)
         s_key[sl] = -1;
-#pragma unroll
-        for (int k = 0; k < ACC_VALS; ++k)
-            s_val[sl * ACC_VALS + k] = 0.0f;
-    }
+    for (int k = threadIdx.x; k < ACC_SLOTS * ACC_VALS * ACC_COPIES / 4; k += blockDim.x)
+        reinterpret_cast<float4 *>(s_val)[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     PixelIdx pi;
     const bool valid = pixel_of_thread(d, pi);
-    tile_pairs(d, pi, s_h, s_v);
+    if (PART != BWD_INTERIOR)
+        tile_pairs<PART == BWD_EDGES>(d, pi, valid, s_h, s_v);
     __syncthreads();
     const VertexAcc A{s_key, s_val};
     if (valid)  // every thread reaches the barrier below
-        backward_pixel<C>(d, A, pi, s_h, s_v);
+        backward_pixel<C, PART>(d, A, pi, s_h, s_v);
     __syncthreads();
     acc_flush<C>(d, A, pi.view);
 }
@@ -824,10 +1011,19 @@ static hipError_t launch_diff(const DiffArgs &d, int which, hipStream_t s)
     if (!blocks)
         return hipSuccess;
     if (which == 0) {
+        const size_t pairs = 2 * (size_t)d.V * d.W * d.H;
+        const uint32_t mb = (uint32_t)((pairs + 255) / 256 < 1024 ? (pairs + 255) / 256 : 1024);
+        hipLaunchKernelGGL(k_diff_clear, dim3(1), dim3(64), 0, s, d);
         hipLaunchKernelGGL(k_diff_interp<C>, dim3(blocks), dim3(256), 0, s, d);
-        hipLaunchKernelGGL(k_diff_aa<C>, dim3(blocks), dim3(256), 0, s, d);
+        hipLaunchKernelGGL(k_diff_march, dim3(mb), dim3(256), 0, s, d);
+        hipLaunchKernelGGL(k_diff_blend<C>, dim3(blocks), dim3(256), 0, s, d);
     } else {
-        hipLaunchKernelGGL(k_diff_backward<C>, dim3(blocks), dim3(256), 0, s, d);
+        if (d.pairs == 2) {
+            hipLaunchKernelGGL((k_diff_backward<C, BWD_INTERIOR>), dim3(blocks), dim3(256), 0, s, d);
+            hipLaunchKernelGGL((k_diff_backward<C, BWD_EDGES>), dim3(blocks), dim3(256), 0, s, d);
+        } else {
+            hipLaunchKernelGGL((k_diff_backward<C, BWD_ALL>), dim3(blocks), dim3(256), 0, s, d);
+        }
         const size_t n = (size_t)d.N * C;
         const uint32_t rb = (uint32_t)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096);
         if (rb)
@@ -890,6 +1086,36 @@ static int diff_run(swr_context *c, const swr_diff_desc *dd, const float *gimage
         d.gattr = (float *)context_zeroed_scratch(c, 0, (size_t)d.V * d.N * q.channels * 4);
         if (!d.gattr)
             return SWR_ERR_HIP;
+    }
+    // The forward stores its pair decisions; a backward over the same buffers and sizes
+    // reuses them instead of walking the silhouettes again.
+    // layout: queue count (256 B), flags [2 px] u8, data [2 px] int2, tile flags u32,
+    // queue [2 px] u32
+    const size_t tiles = (size_t)d.tiles_per_view * d.V, off_f = 256,
+                 off_d = (off_f + px * 2 + 15) & ~(size_t)15, off_t = off_d + px * 2 * sizeof(int2),
+                 off_q = (off_t + tiles * 4 + 15) & ~(size_t)15;
+    if (px * 2 > 0xFFFFFFFFull)
+        return context_fail(c, SWR_ERR_CAPACITY, "too many pixels for one swr_diff call");
+    uint8_t *pbuf = (uint8_t *)context_zeroed_scratch(c, 1, off_q + px * 2 * 4);
+    if (!pbuf)
+        return SWR_ERR_HIP;
+    d.qcount = reinterpret_cast<uint32_t *>(pbuf);
+    d.pflag = pbuf + off_f;
+    d.pdata = reinterpret_cast<int2 *>(pbuf + off_d);
+    d.tflag = reinterpret_cast<uint32_t *>(pbuf + off_t);
+    d.queue = reinterpret_cast<uint32_t *>(pbuf + off_q);
+    const uint64_t tag[8] = {(uint64_t)(uintptr_t)q.clip,      (uint64_t)(uintptr_t)q.triangles,
+                             (uint64_t)(uintptr_t)q.adjacency, (uint64_t)(uintptr_t)q.ids,
+                             (uint64_t)(uintptr_t)q.depth,     (uint64_t)(uintptr_t)pbuf,
+                             ((uint64_t)q.views << 32) | (uint32_t)q.width,
+                             ((uint64_t)q.height << 40) | ((uint64_t)(q.flags & 0xFF) << 32) |
+                                 (uint32_t)q.triangle_count};
+    uint64_t *held = context_diff_tag(c);
+    if (which == 0) {
+        d.pairs = 1;
+        std::memcpy(held, tag, sizeof tag);
+    } else {
+        d.pairs = std::memcmp(held, tag, sizeof tag) == 0 ? 2 : 0;
     }
     hipStream_t s = context_stream(c);
     hipError_t e;

@@ -18,6 +18,22 @@ def gpu_renderer(F, nverts, views, size, channels=3):
     return DiffRenderer(F, nverts, views, size, size, channels=channels)
 
 
+def backward_unstored(r, g):
+    """swr_diff_backward over copies of the visibility buffers: the decisions the forward
+    stored do not name them, so the backward walks the silhouettes itself (one kernel)."""
+    import ctypes as C
+    import torch
+    ids2, depth2 = r.ids.clone(), r.depth.clone()
+    d = r._desc(r._attr_keep)
+    d.ids, d.depth = ids2.data_ptr(), depth2.data_ptr()
+    gclip = torch.zeros((r.V, r.N, 4), dtype=torch.float32, device=r.device)
+    gattr = torch.zeros((r.N, r.C), dtype=torch.float32, device=r.device)
+    r.r._check(r.r._lib.swr_diff_backward(r.r._ctx, C.byref(d), g.data_ptr(), gclip.data_ptr(),
+                                          gattr.data_ptr()), "swr_diff_backward")
+    r.check()
+    return gclip.cpu().numpy(), gattr.cpu().numpy()
+
+
 def rel_err(a, b):
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-30))
 
@@ -45,16 +61,20 @@ def test_forward_and_gradients_match_fp64(gpu, subdiv, size, views):
     depth = r.depth.cpu().numpy()
     rng = np.random.default_rng(7)
     g = rng.normal(size=img.shape).astype(np.float32)
-    gclip, gattr = r.backward(torch.from_numpy(g).cuda())
+    gt = torch.from_numpy(g).cuda()
+    gclip, gattr = r.backward(gt)
     gclip, gattr = gclip.cpu().numpy(), gattr.cpu().numpy()
     adj = r.adj.cpu().numpy()
     ref_img, ref_gclip, ref_gattr = O.diff_reference(clip, F, adj, attr, ids, depth, g)
     assert np.abs(img - ref_img).max() <= 1e-5
-    assert rel_err(gattr, ref_gattr) <= 1e-4
-    assert rel_err(gclip, ref_gclip) <= 1e-4
-    # per element, beside the norm: relative 1e-4 of the element or of the largest one
-    tol = 1e-4 * np.maximum(np.abs(ref_gclip), 1e-2 * np.abs(ref_gclip).max())
-    assert (np.abs(gclip - ref_gclip) <= tol).mean() >= 0.999
+    # the backward reusing the forward's decisions (two kernels) and the one walking them
+    # again (one kernel) both match the reference
+    for gc_, ga_ in ((gclip, gattr), backward_unstored(r, gt)):
+        assert rel_err(ga_, ref_gattr) <= 1e-4
+        assert rel_err(gc_, ref_gclip) <= 1e-4
+        # per element, beside the norm: relative 1e-4 of the element or of the largest one
+        tol = 1e-4 * np.maximum(np.abs(ref_gclip), 1e-2 * np.abs(ref_gclip).max())
+        assert (np.abs(gc_ - ref_gclip) <= tol).mean() >= 0.999
     r.close()
 
 
