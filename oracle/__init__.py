@@ -19,6 +19,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libswr_ref.so")
+REF_OBJ_SO = os.path.join(HERE, "_ref", "libswr_refobj.so")
 REF_SRC = "/root/reference/src/renderer"
 
 MAX_AVARS = 16
@@ -214,6 +215,29 @@ class CpuRenderer:
         if n < 0:
             raise RuntimeError(f"clip failed {n}")
         return out[: min(n, max_out)], n
+
+
+def ref_obj_available() -> bool:
+    return os.path.exists(REF_OBJ_SO)
+
+
+def ref_obj_load(path: str):
+    """The reference's own OBJ loader (ObjData::loadFromFile + toVertexArray, compiled from
+    src/examples/ObjData.cpp): (vertices float32 [n, 8], indices int32)."""
+    lib = C.CDLL(REF_OBJ_SO)
+    f = lib.swr_ref_obj_load
+    f.argtypes = [C.c_char_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,
+                  C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
+    f.restype = C.c_int
+    nv, ni = C.c_size_t(), C.c_size_t()
+    f(path.encode(), None, 0, None, 0, C.byref(nv), C.byref(ni))
+    v = np.zeros((nv.value, 8), np.float32)
+    idx = np.zeros(ni.value, np.int32)
+    rc = f(path.encode(), v.ctypes.data, nv.value, idx.ctypes.data, ni.value, C.byref(nv),
+           C.byref(ni))
+    if rc != 0:
+        raise RuntimeError(f"swr_ref_obj_load failed: {rc}")
+    return v, idx
 
 
 def diff_reference(clip, tris, adj, attr, ids, depth, grad_image=None, culled=True):

@@ -257,6 +257,34 @@ def clip_scene(ntri: int = 2000, seed: int = 3, width: int = 256, height: int = 
                  {"name": "clip", "ntri": ntri, "seed": seed})
 
 
+def vertex_processor_test_scene() -> Scene:
+    """The triangle of the reference's VertexProcessorTest (VertexProcessorTest.cpp:86-118):
+    {x, y, z, r, g, b} vertices drawn with w = 1 into 640x480 through the viewport
+    (100, 100, 440, 280), cull mode None."""
+    v = np.array([[0.0, 0.5, 0.0, 1.0, 0.0, 0.0],
+                  [-1.5, -0.5, 0.0, 0.0, 1.0, 0.0],
+                  [1.5, -0.5, 0.0, 0.0, 0.0, 1.0]], np.float32)
+    return Scene(v, np.arange(3, dtype=np.int32), np.eye(4, dtype=np.float32), 640, 480,
+                 {"name": "vertex_processor_test"})
+
+
+def box_camera(angle: float, width: int = 640, height: int = 480) -> np.ndarray:
+    """The reference Box demo's camera (Box.cpp:182-187): eye (5 cos a, 2, 5 sin a) looking
+    at the origin, 60 degree fovy, near 0.1, far 10; MVP as float32."""
+    view = lookat((5.0 * math.cos(angle), 2.0, 5.0 * math.sin(angle)), (0, 0, 0), (0, 1, 0))
+    proj = perspective(60.0, width / height, 0.1, 10.0)
+    return (proj @ view).astype(np.float32)
+
+
+def obj_scene(path: str, angle: float = 0.6, width: int = 640, height: int = 480) -> Scene:
+    """An OBJ mesh (softwarerenderer_amd/obj.py) under the Box camera: vertices
+    {x, y, z, nx, ny, nz, u, v} (32 bytes, the SWR_VS_MVP_POS_UV layout)."""
+    from .obj import load_obj
+    v, idx = load_obj(path)
+    return Scene(v, idx, box_camera(angle, width, height), width, height,
+                 {"name": "obj", "path": path, "angle": angle})
+
+
 # RasterizerVertex, IRasterizer.h:42-53 (x, y, z, w, avar[16], pvar[16])
 RASTERIZER_VERTEX = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("w", "<f4"),
                               ("avar", "<f4", (16,)), ("pvar", "<f4", (16,))])

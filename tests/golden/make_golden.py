@@ -259,6 +259,17 @@ def make_lines(ref):
     return out
 
 
+# ---- OBJ input ---------------------------------------------------------------------------
+def make_obj():
+    """The reference's ObjData loader (oracle/_ref/libswr_refobj.so) on tests/golden/obj/."""
+    out = {}
+    for name in ("crate", "forms"):
+        v, i = O.ref_obj_load(os.path.join(HERE, "obj", name + ".obj"))
+        out[name + "_v"] = v
+        out[name + "_i"] = i
+    return out
+
+
 def main():
     if not O.ref_available():
         O.build()
@@ -270,6 +281,7 @@ def main():
                         **{k: np.array(v) for k, v in sc.items()})
     np.savez_compressed(os.path.join(HERE, "logs.npz"), **make_logs(ref))
     np.savez_compressed(os.path.join(HERE, "lines.npz"), **make_lines(ref))
+    np.savez_compressed(os.path.join(HERE, "obj.npz"), **make_obj())
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -179,3 +179,33 @@ def test_line_point_list(cpu_oracle, cpu_ref, grid, prim):
         b = run_cpu(cpu_oracle, pipe, vertices=v, indices=idx, prim=prim, log_capacity=200_000)
         assert int(a.log_count[0]) > 100
         assert_same(a, b, f"{prim} list scissor={scissor}")
+
+
+# ---- the reference's demos, headless ----------------------------------------------------------
+
+def test_vertex_processor_test_scene(cpu_oracle, cpu_ref):
+    """VertexProcessorTest.cpp:70-118: RGB shader without z / w, viewport (100, 100, 440,
+    280), cull None."""
+    sc = scenes.vertex_processor_test_scene()
+    for prog in (O.PS_RGB, O.PS_LOG):
+        pipe = O.Pipeline(640, 480, cull_mode=O.CULL_NONE, viewport=(100, 100, 440, 280),
+                          vs_program=O.VS_POS_RGB, vs_counts=(1, 3, 0), ps_program=prog,
+                          ps_z=False, ps_w=False, ps_a=3, ps_p=0)
+        a = run_cpu(cpu_ref, pipe, sc, log_capacity=200_000)
+        b = run_cpu(cpu_oracle, pipe, sc, log_capacity=200_000)
+        assert_same(a, b, f"VertexProcessorTest prog={prog}")
+
+
+@pytest.mark.parametrize("angle", [0.0, 0.6, 2.2])
+def test_box_scene(cpu_oracle, cpu_ref, angle):
+    """Box.cpp:152-196 with an OBJ crate: Span, cull CW, MVP vertex program with the texture
+    coordinate as 2 perspective varyings, InterpolateW (the texture program's inputs)."""
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "obj", "crate.obj")
+    sc = scenes.obj_scene(path, angle)
+    pipe = O.Pipeline(640, 480, mvp=sc.mvp, vs_program=O.VS_MVP_POS_UV, vs_counts=(1, 0, 2),
+                      ps_program=O.PS_LOG, ps_z=False, ps_w=True, ps_a=0, ps_p=2)
+    a = run_cpu(cpu_ref, pipe, sc, log_capacity=400_000)
+    b = run_cpu(cpu_oracle, pipe, sc, log_capacity=400_000)
+    assert int(a.log_count[0]) > 10_000
+    assert_same(a, b, f"box angle={angle}")
