@@ -124,6 +124,15 @@ typedef struct swr_vertex_shader_desc {
  *                         if (z < depth) { depth = z; id = triangle + 1 }, where triangle is
  *                         the input triangle (index triple) the primitive was clipped from;
  *                         the visibility buffer the differentiable path reads
+ *   SWR_PS_TEXTURE        target0 u32 0x00RRGGBB: the textured shader of the Box demo
+ *                         (Box.cpp:39-53, TextureExample.h:17-29): the derivatives of
+ *                         pvar[0..1] from the triangle's planes (PixelData.h:127-145), then
+ *                         Texture::sample (Texture.h:35-118: wrap, anisotropic footprint, up
+ *                         to max_anisotropy trilinear taps, Texture.h:121-200) of `texture`.
+ *                         Needs PVarCount >= 2 (u, v) and triangles (a point or line has no
+ *                         triangle planes: SWR_ERR_UNSUPPORTED).  log2 of the level of
+ *                         detail is correctly rounded (the reference calls the C library's
+ *                         log2f, which may round the other way by one ulp)
  * "key" identifies the primitive in draw order: (batch << 15) + slot, where slot is the
  * triangle's position in the batch's index list after clipping (VertexProcessor.cpp:254-261);
  * for swr_draw_triangle_list it is the triangle's position in the list.
@@ -135,6 +144,7 @@ typedef struct swr_vertex_shader_desc {
 #define SWR_PS_HASH            4
 #define SWR_PS_LOG             5
 #define SWR_PS_VISIBILITY      6
+#define SWR_PS_TEXTURE         7
 
 #define SWR_LOG_WORDS          (7 + SWR_MAX_AVARS + SWR_MAX_PVARS)
 #define SWR_HASH_SEED          2166136261u
@@ -155,6 +165,7 @@ typedef struct swr_pixel_shader_desc {
     uint64_t *log_count;       /* device or host u64            */
     float   checker_cells;     /* SWR_PS_DEPTH_CHECKER uniform  */
     int32_t reserved;
+    const void *texture;       /* SWR_PS_TEXTURE: a swr_texture */
 } swr_pixel_shader_desc;
 
 /* ---- context ------------------------------------------------------------------------ */
@@ -298,6 +309,18 @@ typedef struct swr_tuning {
 } swr_tuning;
 int  swr_set_tuning(swr_context *ctx, const swr_tuning *t);
 int  swr_get_tuning(swr_context *ctx, swr_tuning *t);
+
+/* ---- textures (Texture.h:13-33, 237-296) ---------------------------------------------
+ * A texture with its mip chain: level 0 is width x height texels 0x00RRGGBB (the top byte
+ * is ignored) read with a row pitch in texels from host or device memory; each further level
+ * halves the size (at least 1) with the 2x2 box filter of generateMipmaps (a missing
+ * neighbour repeats the top-left texel, channels averaged with >> 2), down to 1x1.
+ * max_anisotropy is clamped to 1..16 (Texture.h:15-16).  A texture is used by the context
+ * that created it; destroy it before that context. */
+typedef struct swr_texture swr_texture;
+int  swr_texture_create(swr_context *ctx, const uint32_t *texels, int width, int height,
+                        int pitch, int max_anisotropy, swr_texture **out);
+int  swr_texture_destroy(swr_texture *texture);
 
 /* ---- statistics of the last completed draw ------------------------------------------- */
 typedef struct swr_draw_stats {

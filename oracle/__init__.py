@@ -32,7 +32,7 @@ RASTER_SPAN, RASTER_BLOCK, RASTER_ADAPTIVE = 0, 1, 2
 DRAW_POINT, DRAW_LINE, DRAW_TRIANGLE = 0, 1, 2
 CULL_NONE, CULL_CCW, CULL_CW = 0, 1, 2
 VS_DUMMY, VS_RAW, VS_POS_RGB, VS_MVP_POS_RGB_UV, VS_MVP_POS_UV = 0, 1, 2, 3, 4
-PS_NULL, PS_DEPTH_CHECKER, PS_RGB, PS_COVERAGE, PS_HASH, PS_LOG, PS_VISIBILITY = range(7)
+PS_NULL, PS_DEPTH_CHECKER, PS_RGB, PS_COVERAGE, PS_HASH, PS_LOG, PS_VISIBILITY, PS_TEXTURE = range(8)
 
 
 class VertexShaderDesc(C.Structure):
@@ -48,7 +48,7 @@ class PixelShaderDesc(C.Structure):
                 ("pitch", C.c_int32), ("target0", C.c_void_p), ("target1", C.c_void_p),
                 ("target2", C.c_void_p), ("log", C.c_void_p), ("log_capacity", C.c_uint64),
                 ("log_count", C.c_void_p), ("checker_cells", C.c_float),
-                ("reserved", C.c_int32)]
+                ("reserved", C.c_int32), ("texture", C.c_void_p)]
 
 
 class PipelineState(C.Structure):
@@ -88,7 +88,7 @@ class Targets:
         if program == PS_DEPTH_CHECKER:
             self.t0 = np.zeros(shape, np.uint32)
             self.t1 = np.full(shape, depth_clear, np.float32)
-        elif program in (PS_RGB, PS_COVERAGE):
+        elif program in (PS_RGB, PS_COVERAGE, PS_TEXTURE):
             self.t0 = np.zeros(shape, np.uint32)
         elif program == PS_VISIBILITY:
             self.t0 = np.zeros(shape, np.uint32)
@@ -116,7 +116,8 @@ class Pipeline:
     def __init__(self, width=640, height=480, raster_mode=RASTER_SPAN, cull_mode=CULL_CW,
                  scissor=None, viewport=None, depth_range=(0.0, 1.0),
                  vs_program=VS_MVP_POS_RGB_UV, vs_counts=(1, 3, 2), mvp=None,
-                 ps_program=PS_HASH, ps_z=True, ps_w=True, ps_a=3, ps_p=2, checker_cells=8.0):
+                 ps_program=PS_HASH, ps_z=True, ps_w=True, ps_a=3, ps_p=2, checker_cells=8.0,
+                 texture=None):
         self.width, self.height = width, height
         self.raster_mode, self.cull_mode = raster_mode, cull_mode
         self.scissor = tuple(scissor) if scissor is not None else (0, 0, width, height)
@@ -126,6 +127,7 @@ class Pipeline:
         self.mvp = np.eye(4, dtype=np.float32) if mvp is None else np.asarray(mvp, np.float32)
         self.ps_program, self.ps_z, self.ps_w = ps_program, ps_z, ps_w
         self.ps_a, self.ps_p, self.checker_cells = ps_a, ps_p, checker_cells
+        self.texture = texture  # OracleTexture for PS_TEXTURE
 
     def state(self, targets: Targets, attribs=()):
         st = PipelineState()
@@ -147,6 +149,7 @@ class Pipeline:
         ps.log_capacity = targets.log_capacity
         ps.log_count = _ptr(targets.log_count)
         ps.checker_cells = self.checker_cells
+        ps.texture = self.texture.handle if self.texture is not None else None
         for i, (buf, stride) in enumerate(attribs):
             st.attrib[i] = buf.ctypes.data
             st.stride[i] = stride
@@ -154,6 +157,60 @@ class Pipeline:
 
     def targets(self, log_capacity=0) -> Targets:
         return Targets(self.ps_program, self.width, self.height, log_capacity)
+
+
+class OracleTexture:
+    """The oracle's texture for PS_TEXTURE: level 0 from `texels` (uint32 [h, w],
+    0x00RRGGBB) and its mip chain (Texture.h:237-296)."""
+
+    def __init__(self, texels, max_anisotropy: int = 8):
+        self.texels = np.ascontiguousarray(texels, np.uint32)
+        self.max_anisotropy = max_anisotropy
+        h, w = self.texels.shape
+        lib = C.CDLL(ORACLE_SO)
+        lib.swr_oracle_texture_create.restype = C.c_void_p
+        lib.swr_oracle_texture_create.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
+        lib.swr_oracle_texture_destroy.argtypes = [C.c_void_p]
+        self._lib = lib
+        self.handle = lib.swr_oracle_texture_create(self.texels.ctypes.data, w, h, w,
+                                                    max_anisotropy)
+        if not self.handle:
+            raise RuntimeError("swr_oracle_texture_create failed")
+        lib.swr_oracle_texture_sample.restype = C.c_uint32
+        lib.swr_oracle_texture_sample.argtypes = [C.c_void_p] + [C.c_float] * 6
+        lib.swr_oracle_texture_level.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                                 C.c_void_p]
+
+    def sample(self, u, v, dudx, dvdx, dudy, dvdy) -> int:
+        """Texture::sample (Texture.h:35-118): 0x00RRGGBB."""
+        return int(self._lib.swr_oracle_texture_sample(self.handle, u, v, dudx, dvdx, dudy,
+                                                       dvdy))
+
+    def level(self, k: int) -> np.ndarray:
+        p, w, h = C.c_void_p(), C.c_int(), C.c_int()
+        if self._lib.swr_oracle_texture_level(self.handle, k, C.byref(p), C.byref(w),
+                                              C.byref(h)) != 0:
+            raise IndexError(k)
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint32)),
+                                     (h.value, w.value)).copy()
+
+    @property
+    def levels(self) -> int:
+        k = 0
+        while True:
+            try:
+                self.level(k)
+            except IndexError:
+                return k
+            k += 1
+
+    def close(self):
+        if self.handle:
+            self._lib.swr_oracle_texture_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
 
 
 class CpuRenderer:
@@ -215,6 +272,14 @@ class CpuRenderer:
         if n < 0:
             raise RuntimeError(f"clip failed {n}")
         return out[: min(n, max_out)], n
+
+
+def log2_rounded(x: float) -> float:
+    """The oracle's level-of-detail log2 (one rounding to float)."""
+    lib = C.CDLL(ORACLE_SO)
+    lib.swr_oracle_log2.restype = C.c_float
+    lib.swr_oracle_log2.argtypes = [C.c_float]
+    return float(lib.swr_oracle_log2(x))
 
 
 def ref_obj_available() -> bool:

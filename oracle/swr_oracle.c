@@ -137,10 +137,12 @@ typedef struct {
     int x, y;
     float z, w, invw;
     float avar[SWR_MAX_AVARS], pvar[SWR_MAX_PVARS], pvt[SWR_MAX_PVARS];
+    const tri_t *eq;  /* the triangle's equations (PixelData.h:55-56); NULL for points / lines */
 } pix_t;
 
 static void pix_init(pix_t *p, const rctx *r, const tri_t *q, float x, float y)
 {
+    p->eq = q;
     if (r->iz)
         p->z = param_eval(&q->z, x, y);
     if (r->iw || r->np > 0) {
@@ -190,6 +192,254 @@ static uint32_t checker_rgba8(const pix_t *p, float cells)
     return r | (g << 8) | (b << 16) | 0xFF000000u;
 }
 
+/* ---------------------------------------------------------------- textures */
+
+/* Texture (Texture.h:11-33) with its mip chain (generateMipmaps, Texture.h:237-296). */
+struct swr_oracle_texture {
+    int levels, max_aniso;
+    int w[32], h[32];
+    uint32_t *px[32];
+};
+
+static uint32_t tex_r(uint32_t c) { return (c >> 16) & 0xFFu; }
+static uint32_t tex_g(uint32_t c) { return (c >> 8) & 0xFFu; }
+static uint32_t tex_b(uint32_t c) { return c & 0xFFu; }
+static uint32_t tex_pack(uint32_t r, uint32_t g, uint32_t b)
+{
+    return ((r & 0xFFu) << 16) | ((g & 0xFFu) << 8) | (b & 0xFFu);
+}
+
+swr_oracle_texture *swr_oracle_texture_create(const uint32_t *texels, int width, int height,
+                                              int pitch, int max_anisotropy)
+{
+    if (!texels || width < 1 || height < 1 || pitch < width)
+        return NULL;
+    swr_oracle_texture *t = (swr_oracle_texture *)calloc(1, sizeof *t);
+    if (!t)
+        return NULL;
+    /* std::clamp(maxAnisotropy, 1, MAX_ANISOTROPY), Texture.h:15-16 */
+    t->max_aniso = max_anisotropy < 1 ? 1 : (max_anisotropy > 16 ? 16 : max_anisotropy);
+    int w = width, h = height, lv = 0;
+    t->px[0] = (uint32_t *)malloc((size_t)w * h * 4);
+    if (!t->px[0]) {
+        free(t);
+        return NULL;
+    }
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x)
+            t->px[0][(size_t)y * w + x] = texels[(size_t)y * pitch + x] & 0xFFFFFFu;
+    t->w[0] = w;
+    t->h[0] = h;
+    while (w > 1 || h > 1) {
+        int nw = w / 2 > 1 ? w / 2 : 1, nh = h / 2 > 1 ? h / 2 : 1;
+        const uint32_t *src = t->px[lv];
+        uint32_t *dst = (uint32_t *)malloc((size_t)nw * nh * 4);
+        if (!dst)
+            break;
+        for (int y = 0; y < nh; ++y)
+            for (int x = 0; x < nw; ++x) {
+                uint32_t p00 = src[(size_t)(y * 2) * w + (x * 2)];
+                uint32_t p10 = x * 2 + 1 < w ? src[(size_t)(y * 2) * w + (x * 2 + 1)] : p00;
+                uint32_t p01 = y * 2 + 1 < h ? src[(size_t)(y * 2 + 1) * w + (x * 2)] : p00;
+                uint32_t p11 = (x * 2 + 1 < w && y * 2 + 1 < h)
+                                   ? src[(size_t)(y * 2 + 1) * w + (x * 2 + 1)]
+                                   : p00;
+                dst[(size_t)y * nw + x] =
+                    tex_pack((tex_r(p00) + tex_r(p10) + tex_r(p01) + tex_r(p11)) >> 2,
+                             (tex_g(p00) + tex_g(p10) + tex_g(p01) + tex_g(p11)) >> 2,
+                             (tex_b(p00) + tex_b(p10) + tex_b(p01) + tex_b(p11)) >> 2);
+            }
+        ++lv;
+        t->px[lv] = dst;
+        t->w[lv] = nw;
+        t->h[lv] = nh;
+        w = nw;
+        h = nh;
+    }
+    t->levels = lv + 1;
+    return t;
+}
+
+void swr_oracle_texture_destroy(swr_oracle_texture *t)
+{
+    if (!t)
+        return;
+    for (int i = 0; i < t->levels; ++i)
+        free(t->px[i]);
+    free(t);
+}
+
+/* log2 with one rounding to float: the C library's log2f (which Texture.h:124 calls through
+ * std::log2) may round the other way by one ulp, so the level of detail is evaluated here as
+ * a double series, then rounded; the device evaluates the same operations.  x = m 2^e with
+ * m in [sqrt(1/2), sqrt(2)); log(m) = 2 atanh(s) = 2 (s + s^3/3 + ...), s = (m-1)/(m+1). */
+static float log2_rounded(float xf)
+{
+    if (!(xf > 0.0f))
+        return xf == 0.0f ? -INFINITY : NAN;
+    if (isinf(xf))
+        return xf;
+    int e;
+    double m = frexp((double)xf, &e);
+    if (m < 0.70710678118654752440) {
+        m *= 2.0;
+        e -= 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0), s2 = s * s;
+    double q = 1.0 / 25.0;
+    q = q * s2 + 1.0 / 23.0;
+    q = q * s2 + 1.0 / 21.0;
+    q = q * s2 + 1.0 / 19.0;
+    q = q * s2 + 1.0 / 17.0;
+    q = q * s2 + 1.0 / 15.0;
+    q = q * s2 + 1.0 / 13.0;
+    q = q * s2 + 1.0 / 11.0;
+    q = q * s2 + 1.0 / 9.0;
+    q = q * s2 + 1.0 / 7.0;
+    q = q * s2 + 1.0 / 5.0;
+    q = q * s2 + 1.0 / 3.0;
+    q = q * s2 + 1.0;
+    const double ln_m = 2.0 * s * q;
+    return (float)((double)e + ln_m * 1.4426950408889634074);
+}
+
+/* std::clamp of the reference (argument order matters for NaN) */
+static float clamp_f(float v, float lo, float hi) { return (v < lo) ? lo : ((hi < v) ? hi : v); }
+
+/* Texture::sampleBilinear / bilinearLerpColors, Texture.h:148-228 */
+static uint32_t tex_bilinear(const swr_oracle_texture *t, int lv, float u, float v)
+{
+    const int w = t->w[lv], h = t->h[lv];
+    float px = u * (float)(w - 1);
+    float py = v * (float)(h - 1);
+    int x0 = max_i(0, f2i(floorf(px)));
+    int y0 = max_i(0, f2i(floorf(py)));
+    int x1 = min_i(w - 1, x0 + 1);
+    int y1 = min_i(h - 1, y0 + 1);
+    float fx = px - (float)x0;
+    float fy = py - (float)y0;
+    const uint32_t *p = t->px[lv];
+    uint32_t c00 = p[(size_t)y0 * w + x0], c10 = p[(size_t)y0 * w + x1];
+    uint32_t c01 = p[(size_t)y1 * w + x0], c11 = p[(size_t)y1 * w + x1];
+    uint32_t (*ch[3])(uint32_t) = {tex_r, tex_g, tex_b};
+    uint32_t out[3];
+    for (int k = 0; k < 3; ++k) {
+        float s = (float)ch[k](c00) * (1.0f - fx) * (1.0f - fy) + (float)ch[k](c10) * fx * (1.0f - fy) +
+                  (float)ch[k](c01) * (1.0f - fx) * fy + (float)ch[k](c11) * fx * fy;
+        out[k] = (uint32_t)(uint8_t)f2i(s);
+    }
+    return tex_pack(out[0], out[1], out[2]);
+}
+
+/* Texture::lerpColors, Texture.h:200-205: the channel difference times t converted to
+ * Uint8 (two's complement wrap for a negative difference, as x86-64 converts), added */
+static uint32_t tex_lerp(uint32_t c1, uint32_t c2, float t)
+{
+    uint32_t (*ch[3])(uint32_t) = {tex_r, tex_g, tex_b};
+    uint32_t out[3];
+    for (int k = 0; k < 3; ++k) {
+        int d = (int)ch[k](c2) - (int)ch[k](c1);
+        uint8_t step = (uint8_t)f2i((float)d * t);
+        out[k] = (uint8_t)(ch[k](c1) + step);
+    }
+    return tex_pack(out[0], out[1], out[2]);
+}
+
+/* Texture::sampleTrilinear, Texture.h:121-146 */
+static uint32_t tex_trilinear(const swr_oracle_texture *t, float u, float v, float rho)
+{
+    float lod = log2_rounded(rho < 1e-6f ? 1e-6f : rho);
+    lod = clamp_f(lod, 0.0f, (float)(t->levels - 1));
+    int base = f2i(floorf(lod));
+    int next = min_i(base + 1, t->levels - 1);
+    float frac = clamp_f(lod - (float)base, 0.0f, 1.0f);
+    uint32_t cb = tex_bilinear(t, base, u, v);
+    if (base != next)
+        return tex_lerp(cb, tex_bilinear(t, next, u, v), frac);
+    return cb;
+}
+
+static float wrap01(float u)
+{
+    u = fmodf(u, 1.0f);
+    if (u < 0)
+        u += 1.0f;
+    return u;
+}
+
+/* Texture::sample, Texture.h:35-118 */
+static uint32_t tex_sample(const swr_oracle_texture *t, float u, float v, float dudx, float dvdx,
+                           float dudy, float dvdy)
+{
+    u = wrap01(u);
+    v = wrap01(v);
+    const float W = (float)t->w[0], H = (float)t->h[0];
+    float dudx_s = dudx * W, dvdx_s = dvdx * H, dudy_s = dudy * W, dvdy_s = dvdy * H;
+    float dx_len = sqrtf(dudx_s * dudx_s + dvdx_s * dvdx_s);
+    float dy_len = sqrtf(dudy_s * dudy_s + dvdy_s * dvdy_s);
+    dx_len = max_f(dx_len, 1e-6f);
+    dy_len = max_f(dy_len, 1e-6f);
+    float major = max_f(dx_len, dy_len), minor = min_f(dx_len, dy_len);
+    float ratio = min_f(major / minor, (float)t->max_aniso);
+    int n = max_i(1, f2i(ceilf(ratio)));
+    if (n <= 1)
+        return tex_trilinear(t, u, v, major);
+    float mdu, mdv;
+    if (dx_len > dy_len) {
+        mdu = dudx / dx_len;
+        mdv = dvdx / dx_len;
+    } else {
+        mdu = dudy / dy_len;
+        mdv = dvdy / dy_len;
+    }
+    float r = 0, g = 0, b = 0;
+    float step = 1.0f / (float)n;
+    for (int i = 0; i < n; ++i) {
+        float tt = ((float)i + 0.5f) * step - 0.5f;
+        float su = wrap01(u + mdu * major * tt);
+        float sv = wrap01(v + mdv * major * tt);
+        uint32_t c = tex_trilinear(t, su, sv, minor);
+        r += (float)tex_r(c);
+        g += (float)tex_g(c);
+        b += (float)tex_b(c);
+    }
+    r = min_f(255.0f, r / (float)n);
+    g = min_f(255.0f, g / (float)n);
+    b = min_f(255.0f, b / (float)n);
+    return tex_pack((uint8_t)f2i(r), (uint8_t)f2i(g), (uint8_t)f2i(b));
+}
+
+/* exported probes of the sampler (tests pin them by closed forms) */
+uint32_t swr_oracle_texture_sample(const swr_oracle_texture *t, float u, float v, float dudx,
+                                   float dvdx, float dudy, float dvdy)
+{
+    return tex_sample(t, u, v, dudx, dvdx, dudy, dvdy);
+}
+
+float swr_oracle_log2(float x) { return log2_rounded(x); }
+
+int swr_oracle_texture_level(const swr_oracle_texture *t, int level, const uint32_t **texels,
+                             int *width, int *height)
+{
+    if (!t || level < 0 || level >= t->levels)
+        return SWR_ERR_INVALID_ARG;
+    *texels = t->px[level];
+    *width = t->w[level];
+    *height = t->h[level];
+    return SWR_OK;
+}
+
+/* PixelData::computePerspectiveDerivatives, PixelData.h:127-145 */
+static void persp_derivs(const pix_t *p, int k, float *ddx, float *ddy)
+{
+    const tri_t *q = p->eq;
+    float xf = (float)p->x + 0.5f, yf = (float)p->y + 0.5f;
+    float var = param_eval(&q->pvar[k], xf, yf);
+    float iw = param_eval(&q->invw, xf, yf);
+    *ddx = (iw * q->pvar[k].a - var * q->invw.a) / (iw * iw);
+    *ddy = (iw * q->pvar[k].b - var * q->invw.b) / (iw * iw);
+}
+
 /* The user drawPixel of PixelShaderBase.h:114-119, for the built-in programs. */
 static void draw_pixel(const rctx *r, const pix_t *p)
 {
@@ -217,6 +467,15 @@ static void draw_pixel(const rctx *r, const pix_t *p)
     case SWR_PS_COVERAGE:
         ((uint32_t *)ps->target0)[o] = 1u;
         break;
+    case SWR_PS_TEXTURE: {
+        /* Box.cpp:39-53 */
+        float dudx, dudy, dvdx, dvdy;
+        persp_derivs(p, 0, &dudx, &dudy);
+        persp_derivs(p, 1, &dvdx, &dvdy);
+        ((uint32_t *)ps->target0)[o] = tex_sample((const swr_oracle_texture *)ps->texture,
+                                                  p->pvar[0], p->pvar[1], dudx, dvdx, dudy, dvdy);
+        break;
+    }
     case SWR_PS_VISIBILITY: {
         float *depth = (float *)ps->target1;
         if (p->z < depth[o]) {
@@ -536,6 +795,7 @@ static int scissor_test(const rctx *r, float x, float y)
 /* Rasterizer::pixelDataFromVertex, Rasterizer.h:160-173 */
 static void pix_from_vertex(const rctx *r, const vtx *v, pix_t *p)
 {
+    p->eq = NULL;
     p->x = f2i(v->x);
     p->y = f2i(v->y);
     if (r->iz)
@@ -639,6 +899,17 @@ static void draw_point_list(rctx *r, uint32_t batch, const vtx *v, const int32_t
     }
 }
 
+/* The texture program needs its texture and a triangle's planes (points and lines have
+ * none: their PixelData::equations is null, PixelData.h:58). */
+static int check_texture(const swr_pipeline_state *st, int mode)
+{
+    if (st->ps.program != SWR_PS_TEXTURE)
+        return SWR_OK;
+    if (!st->ps.texture || st->ps.pvar_count < 2)
+        return SWR_ERR_INVALID_ARG;
+    return mode == SWR_DRAW_TRIANGLE ? SWR_OK : SWR_ERR_UNSUPPORTED;
+}
+
 static int check_list(const swr_pipeline_state *st, const vtx *v, size_t nv, const int32_t *idx,
                       size_t n)
 {
@@ -654,6 +925,8 @@ int swr_oracle_draw_triangle_list(const swr_pipeline_state *st, const vtx *v, si
                                   const int32_t *idx, size_t n)
 {
     int rc = check_list(st, v, nv, idx, n);
+    if (!rc)
+        rc = check_texture(st, SWR_DRAW_TRIANGLE);
     if (rc)
         return rc;
     rctx r;
@@ -666,6 +939,8 @@ int swr_oracle_draw_line_list(const swr_pipeline_state *st, const vtx *v, size_t
                               const int32_t *idx, size_t n)
 {
     int rc = check_list(st, v, nv, idx, n);
+    if (!rc)
+        rc = check_texture(st, SWR_DRAW_LINE);
     if (rc)
         return rc;
     rctx r;
@@ -678,6 +953,8 @@ int swr_oracle_draw_point_list(const swr_pipeline_state *st, const vtx *v, size_
                                const int32_t *idx, size_t n)
 {
     int rc = check_list(st, v, nv, idx, n);
+    if (!rc)
+        rc = check_texture(st, SWR_DRAW_POINT);
     if (rc)
         return rc;
     rctx r;
@@ -1106,6 +1383,9 @@ int swr_oracle_draw_elements(const swr_pipeline_state *st, int mode, size_t coun
         return SWR_ERR_INVALID_ARG;
     if (st->vs.program != SWR_VS_DUMMY && !st->attrib[0])
         return SWR_ERR_INVALID_ARG;
+    int rc0 = check_texture(st, mode);
+    if (rc0)
+        return rc0;
     for (size_t i = 0; i < count; ++i)
         if (indices[i] < 0)
             return SWR_ERR_INDEX_RANGE;

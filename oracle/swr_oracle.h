@@ -56,6 +56,21 @@ int swr_oracle_clip_triangle(const swr_rasterizer_vertex *v0, const swr_rasteriz
                              const swr_rasterizer_vertex *v2, int na, int np,
                              swr_rasterizer_vertex *out, int max_out);
 
+/* A texture for SWR_PS_TEXTURE (its pointer goes in ps.texture): level 0 copied from
+ * `texels` (0x00RRGGBB, row pitch in texels) and the mip chain of Texture::generateMipmaps
+ * (Texture.h:237-296); max_anisotropy clamped to 1..16. */
+typedef struct swr_oracle_texture swr_oracle_texture;
+swr_oracle_texture *swr_oracle_texture_create(const uint32_t *texels, int width, int height,
+                                              int pitch, int max_anisotropy);
+void swr_oracle_texture_destroy(swr_oracle_texture *t);
+/* Texture::sample of the texture (Texture.h:35-118), the level-of-detail log2, and mip level
+ * `level` (its texels stay owned by the texture). */
+uint32_t swr_oracle_texture_sample(const swr_oracle_texture *t, float u, float v, float dudx,
+                                   float dvdx, float dudy, float dvdy);
+float swr_oracle_log2(float x);
+int swr_oracle_texture_level(const swr_oracle_texture *t, int level, const uint32_t **texels,
+                             int *width, int *height);
+
 #ifdef __cplusplus
 }
 #endif

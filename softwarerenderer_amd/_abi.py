@@ -36,7 +36,7 @@ class PixelShaderDesc(C.Structure):
                 ("pitch", C.c_int32), ("target0", C.c_void_p), ("target1", C.c_void_p),
                 ("target2", C.c_void_p), ("log", C.c_void_p), ("log_capacity", C.c_uint64),
                 ("log_count", C.c_void_p), ("checker_cells", C.c_float),
-                ("reserved", C.c_int32)]
+                ("reserved", C.c_int32), ("texture", C.c_void_p)]
 
 
 class DrawStats(C.Structure):
@@ -105,6 +105,9 @@ SIGNATURES = {
     "swr_diff_forward": (_I, [_P, C.POINTER(DiffDesc)]),
     "swr_diff_backward": (_I, [_P, C.POINTER(DiffDesc), _P, _P, _P]),
     "swr_set_tuning": (_I, [_P, C.POINTER(Tuning)]),
+    "swr_texture_create": (_I, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_int,
+                                C.POINTER(C.c_void_p)]),
+    "swr_texture_destroy": (_I, [_P]),
     "swr_get_tuning": (_I, [_P, C.POINTER(Tuning)]),
 }
 

@@ -33,7 +33,7 @@ class CullMode:
 
 # program ids
 VS_DUMMY, VS_RAW, VS_POS_RGB, VS_MVP_POS_RGB_UV, VS_MVP_POS_UV = 0, 1, 2, 3, 4
-PS_NULL, PS_DEPTH_CHECKER, PS_RGB, PS_COVERAGE, PS_HASH, PS_LOG, PS_VISIBILITY = range(7)
+PS_NULL, PS_DEPTH_CHECKER, PS_RGB, PS_COVERAGE, PS_HASH, PS_LOG, PS_VISIBILITY, PS_TEXTURE = range(8)
 
 RASTERIZER_VERTEX = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("w", "<f4"),
                               ("avar", "<f4", (_abi.MAX_AVARS,)),
@@ -82,6 +82,8 @@ class PixelShaderBase:
         d.target0, d.target1, d.target2 = ptrs
         d.width, d.height, d.pitch = w, h, pitch
         d.checker_cells = float(getattr(self, "checker_cells", 8.0))
+        tex = getattr(self, "texture", None)
+        d.texture = tex.handle if tex is not None else None
         log = getattr(self, "log", None)
         if log is not None:
             p, nbytes, k = _buf(log)
@@ -181,6 +183,49 @@ class VisibilityShader(PixelShaderBase):
         if self.ids is None:
             return None, None, None, 0, 0, 0
         return (self.ids, self.depth, None) + _hw(self.ids)
+
+
+class Texture:
+    """A texture with its mip chain on the rasterizer's device (Texture.h:13-33, 237-296):
+    `texels` is uint32 [height, width] 0x00RRGGBB (numpy or torch, host or device)."""
+
+    def __init__(self, rasterizer: "Rasterizer", texels, max_anisotropy: int = 8):
+        h, w = texels.shape[:2]
+        p, _, keep = _buf(texels)
+        out = C.c_void_p()
+        self._r = rasterizer
+        rasterizer._check(rasterizer._lib.swr_texture_create(rasterizer._ctx, p, w, h, w,
+                                                             max_anisotropy, C.byref(out)),
+                          "Texture")
+        del keep
+        self.handle = out.value
+        self.width, self.height = w, h
+
+    def close(self):
+        if self.handle:
+            self._r._lib.swr_texture_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class TextureShader(PixelShaderBase):
+    """The textured shader of the Box demo (Box.cpp:27-53): color = texture.sample(u, v,
+    perspective derivatives of u and v), written to a u32 0x00RRGGBB target."""
+
+    InterpolateW = True
+    PVarCount = 2
+    Program = PS_TEXTURE
+
+    def __init__(self, color, texture: Texture):
+        self.color, self.texture = color, texture
+
+    def targets(self):
+        return (self.color, None, None) + _hw(self.color)
 
 
 class VertexShaderBase:

@@ -77,6 +77,13 @@ struct swr_context {
     uint64_t diff_tag[8] = {};
 };
 
+// A texture and its mip chain on the device (include/swr_hip.h, swr_texture_create).
+struct swr_texture {
+    swr_context *ctx = nullptr;
+    DevBuf texels;  // every level, level k at off[k]
+    DevBuf desc;    // TexDev
+};
+
 // A batched draw's device tables (include/swr_hip.h, swr_batch_create).
 struct swr_batch {
     swr_context *ctx = nullptr;
@@ -233,6 +240,7 @@ int targets_needed(int program, bool need[3])
     case SWR_PS_RGB:
     case SWR_PS_COVERAGE:
     case SWR_PS_LOG:
+    case SWR_PS_TEXTURE:
         need[0] = true;
         break;
     default:
@@ -558,6 +566,8 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
             r.nscenes = g.nscenes;
         }
         r.xcd = c->tune.xcd_order >= 0 ? 1 : 0;
+        if (ps.program == SWR_PS_TEXTURE)
+            r.tex = (const TexDev *)static_cast<const swr_texture *>(ps.texture)->desc.p;
         HIP_TRY(c, launch_raster(ps, L, r, c->stream, c->tune.raster_waves ? c->tune.raster_waves : 4,
                                  c->tune.raster_group));
         record(c, 6);
@@ -603,6 +613,9 @@ int draw_lines_points(swr_context *c, int mode, const int32_t *indices, size_t c
                      (uint64_t)((c->ps.height + TILE - 1) / TILE);
     if (P == 0 || c->ps.program == SWR_PS_NULL)
         return SWR_OK;
+    if (c->ps.program == SWR_PS_TEXTURE)
+        return fail(c, SWR_ERR_UNSUPPORTED,
+                    "the texture program needs a triangle's planes: points and lines have none");
     if (P > (size_t)(1u << 30))
         return fail(c, SWR_ERR_CAPACITY, "too many primitives (%zu)", P);
     ParamLayout L;
@@ -921,8 +934,13 @@ int swr_set_pixel_shader(swr_context *c, const swr_pixel_shader_desc *d)
 {
     if (!c || !d)
         return fail(c, SWR_ERR_INVALID_ARG, "null pixel shader");
-    if (d->program < SWR_PS_NULL || d->program > SWR_PS_VISIBILITY)
+    if (d->program < SWR_PS_NULL || d->program > SWR_PS_TEXTURE)
         return fail(c, SWR_ERR_UNSUPPORTED, "pixel program %d is not built", d->program);
+    if (d->program == SWR_PS_TEXTURE) {
+        const swr_texture *t = static_cast<const swr_texture *>(d->texture);
+        if (!t || t->ctx != c)
+            return fail(c, SWR_ERR_INVALID_ARG, "the texture program needs a texture of this context");
+    }
     ParamLayout L;
     if (d->program != SWR_PS_NULL && !raster_layout(*d, L))
         return fail(c, SWR_ERR_UNSUPPORTED, "pixel program %d cannot run with these counts",
@@ -1161,6 +1179,94 @@ int swr_batch_destroy(swr_batch *b)
     if (b->bmap.p)
         (void)hipFree(b->bmap.p);
     delete b;
+    return SWR_OK;
+}
+
+int swr_texture_create(swr_context *c, const uint32_t *texels, int width, int height, int pitch,
+                       int max_anisotropy, swr_texture **out)
+{
+    if (!c || !texels || !out || width < 1 || height < 1 || pitch < width)
+        return fail(c, SWR_ERR_INVALID_ARG, "bad texture arguments");
+    *out = nullptr;
+    (void)hipSetDevice(c->device);
+    // level 0 on the host (device texels are copied down: the chain is built once)
+    std::vector<uint32_t> base((size_t)width * height);
+    if (is_device_ptr(texels)) {
+        HIP_TRY(c, hipMemcpy2D(base.data(), (size_t)width * 4, texels, (size_t)pitch * 4,
+                               (size_t)width * 4, height, hipMemcpyDeviceToHost));
+    } else {
+        for (int y = 0; y < height; ++y)
+            std::memcpy(&base[(size_t)y * width], texels + (size_t)y * pitch, (size_t)width * 4);
+    }
+    TexDev td{};
+    // std::clamp(maxAnisotropy, 1, MAX_ANISOTROPY), Texture.h:15-16
+    td.max_aniso = max_anisotropy < 1 ? 1 : (max_anisotropy > 16 ? 16 : max_anisotropy);
+    std::vector<uint32_t> all;
+    for (uint32_t &p : base)
+        p &= 0xFFFFFFu;
+    std::vector<uint32_t> cur = base;
+    int w = width, h = height, lv = 0;
+    for (;;) {
+        td.w[lv] = w;
+        td.h[lv] = h;
+        td.off[lv] = all.size();
+        all.insert(all.end(), cur.begin(), cur.end());
+        if ((w <= 1 && h <= 1) || lv + 1 >= TEX_MAX_LEVELS)
+            break;
+        // Texture::generateMipmaps, Texture.h:268-287: 2x2 box filter, a missing neighbour
+        // repeats the top-left texel, each channel averaged with >> 2
+        const int nw = std::max(1, w / 2), nh = std::max(1, h / 2);
+        std::vector<uint32_t> next((size_t)nw * nh);
+        auto ch = [](uint32_t p, int k) { return (p >> (16 - 8 * k)) & 0xFFu; };
+        for (int y = 0; y < nh; ++y)
+            for (int x = 0; x < nw; ++x) {
+                const uint32_t p00 = cur[(size_t)(2 * y) * w + 2 * x];
+                const uint32_t p10 = 2 * x + 1 < w ? cur[(size_t)(2 * y) * w + 2 * x + 1] : p00;
+                const uint32_t p01 = 2 * y + 1 < h ? cur[(size_t)(2 * y + 1) * w + 2 * x] : p00;
+                const uint32_t p11 = (2 * x + 1 < w && 2 * y + 1 < h)
+                                         ? cur[(size_t)(2 * y + 1) * w + 2 * x + 1]
+                                         : p00;
+                uint32_t o = 0;
+                for (int k = 0; k < 3; ++k)
+                    o |= (((ch(p00, k) + ch(p10, k) + ch(p01, k) + ch(p11, k)) >> 2) & 0xFFu)
+                         << (16 - 8 * k);
+                next[(size_t)y * nw + x] = o;
+            }
+        cur.swap(next);
+        w = nw;
+        h = nh;
+        ++lv;
+    }
+    td.levels = lv + 1;
+    swr_texture *t = new swr_texture;
+    t->ctx = c;
+    hipError_t e = ensure(c, t->texels, all.size() * 4);
+    if (e == hipSuccess)
+        e = ensure(c, t->desc, sizeof(TexDev));
+    if (e == hipSuccess)
+        e = hipMemcpy(t->texels.p, all.data(), all.size() * 4, hipMemcpyHostToDevice);
+    td.texels = (const uint32_t *)t->texels.p;
+    if (e == hipSuccess)
+        e = hipMemcpy(t->desc.p, &td, sizeof td, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(t->texels.p);
+        (void)hipFree(t->desc.p);
+        delete t;
+        return fail(c, SWR_ERR_HIP, "texture upload failed: %s", hipGetErrorString(e));
+    }
+    *out = t;
+    return SWR_OK;
+}
+
+int swr_texture_destroy(swr_texture *t)
+{
+    if (!t)
+        return SWR_OK;
+    (void)hipSetDevice(t->ctx->device);
+    (void)hipStreamSynchronize(t->ctx->stream);
+    (void)hipFree(t->texels.p);
+    (void)hipFree(t->desc.p);
+    delete t;
     return SWR_OK;
 }
 

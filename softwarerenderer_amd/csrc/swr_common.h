@@ -227,6 +227,16 @@ struct GeomArgs {
     uint32_t nscenes;
 };
 
+// A texture and its mip chain in device memory (swr_texture_create): level k is
+// w[k] x h[k] texels 0x00RRGGBB at texels + off[k].
+constexpr int TEX_MAX_LEVELS = 32;
+struct TexDev {
+    const uint32_t *texels;
+    int levels, max_aniso;
+    int w[TEX_MAX_LEVELS], h[TEX_MAX_LEVELS];
+    uint64_t off[TEX_MAX_LEVELS];
+};
+
 struct RasterArgs {
     const float *prims;
     int stride;
@@ -246,6 +256,7 @@ struct RasterArgs {
     const SceneDev *scenes;     // batched draw (nscenes == 0: single draw)
     uint32_t nscenes;
     int xcd;                    // 1: tile blocks grouped per XCD
+    const TexDev *tex;          // SWR_PS_TEXTURE
 };
 
 // Scene-local state of a batched draw over the single-draw arguments.

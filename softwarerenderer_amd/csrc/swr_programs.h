@@ -31,6 +31,7 @@ __device__ inline float clamp01(float x) { return x > 0.0f ? (x < 1.0f ? x : 1.0
 template <class L>
 struct ChainPixel {
     const float *v;
+    const float *rec;  // the primitive's record (its planes), for programs with PLANES set
     __device__ float z() const { return v[L::Z]; }
     __device__ float invw() const { return v[L::IW]; }
     __device__ float w() const { return 1.0f / v[L::IW]; }
@@ -42,6 +43,7 @@ struct ChainPixel {
 template <class L>
 struct ProgCoverage {
     static constexpr int KEY_WORD = REC_KEY;
+    static constexpr bool PLANES = false;
     struct State { bool hit; };
     __device__ static void load(State &s, const RasterArgs &, size_t) { s.hit = false; }
     template <class PX>
@@ -60,6 +62,7 @@ struct ProgCoverage {
 template <class L>
 struct ProgRGB {
     static constexpr int KEY_WORD = REC_KEY;
+    static constexpr bool PLANES = false;
     struct State { uint32_t c; bool hit; };
     __device__ static void load(State &s, const RasterArgs &, size_t) { s.hit = false; s.c = 0; }
     template <class PX>
@@ -81,6 +84,7 @@ struct ProgRGB {
 template <class L>
 struct ProgDepthChecker {
     static constexpr int KEY_WORD = REC_KEY;
+    static constexpr bool PLANES = false;
     struct State { float depth; uint32_t color; bool hit; };
     __device__ static void load(State &s, const RasterArgs &a, size_t o)
     {
@@ -118,6 +122,7 @@ struct ProgDepthChecker {
 template <class L>
 struct ProgHash {
     static constexpr int KEY_WORD = REC_KEY;
+    static constexpr bool PLANES = false;
     struct State { uint32_t h, n, n0; };
     __device__ static void load(State &s, const RasterArgs &a, size_t o)
     {
@@ -159,6 +164,7 @@ struct ProgHash {
 template <class L>
 struct ProgLog {
     static constexpr int KEY_WORD = REC_KEY;
+    static constexpr bool PLANES = false;
     struct State { uint32_t n, n0; };
     __device__ static void load(State &s, const RasterArgs &a, size_t o)
     {
@@ -201,6 +207,7 @@ struct ProgLog {
 template <class L>
 struct ProgVisibility {
     static constexpr int KEY_WORD = REC_SRC;
+    static constexpr bool PLANES = false;
     struct State { float depth; uint32_t id; bool hit; };
     __device__ static void load(State &s, const RasterArgs &a, size_t o)
     {
@@ -225,6 +232,189 @@ struct ProgVisibility {
             static_cast<float *>(a.t1)[o] = s.depth;
             static_cast<uint32_t *>(a.t0)[o] = s.id;
         }
+    }
+};
+
+// ---- the texture program ----------------------------------------------------------------
+// Texture::sample and its helpers (Texture.h:35-228) over a TexDev, operation for operation;
+// the reference's std::min / std::max / std::clamp argument orders are kept (they decide
+// NaN cases) and its Uint8 conversions are (int) truncation then the low 8 bits, as x86-64
+// converts.  log2 of the level of detail is rounded once (see tex_log2).
+
+__device__ inline uint32_t tex_ch(uint32_t c, int k) { return (c >> (16 - 8 * k)) & 0xFFu; }
+__device__ inline uint32_t tex_pack(uint32_t r, uint32_t g, uint32_t b)
+{
+    return ((r & 0xFFu) << 16) | ((g & 0xFFu) << 8) | (b & 0xFFu);
+}
+__device__ inline int tex_imin(int a, int b) { return (b < a) ? b : a; }
+__device__ inline float tex_clamp(float v, float lo, float hi)
+{
+    return (v < lo) ? lo : ((hi < v) ? hi : v);
+}
+
+// fmodf(u, 1) then + 1 below zero (Texture.h:43-46): u - trunc(u) is exact for every finite
+// float (Sterbenz), and carries the sign of u as fmod does (also for a zero result).
+__device__ inline float tex_wrap(float u)
+{
+    float r = __builtin_copysignf(u - __builtin_truncf(u), u);
+    if (r < 0)
+        r += 1.0f;
+    return r;
+}
+
+// log2 rounded once to float: x = m 2^e, m in [sqrt(1/2), sqrt(2)); log(m) = 2 atanh(s),
+// s = (m - 1)/(m + 1), as a degree-25 odd series in double.  The C library's log2f (which the
+// reference calls) may round the other way by one ulp.
+__device__ inline float tex_log2(float xf)
+{
+    if (!(xf > 0.0f))
+        return xf == 0.0f ? -__builtin_inff() : __builtin_nanf("");
+    if (__builtin_isinf(xf))
+        return xf;
+    const unsigned long long bits = (unsigned long long)__double_as_longlong((double)xf);
+    int e = (int)((bits >> 52) & 0x7FFull) - 1022;
+    double m = __longlong_as_double((long long)((bits & ~(0x7FFull << 52)) | (1022ull << 52)));
+    if (m < 0.70710678118654752440) {
+        m *= 2.0;
+        e -= 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0), s2 = s * s;
+    double q = 1.0 / 25.0;
+    q = q * s2 + 1.0 / 23.0;
+    q = q * s2 + 1.0 / 21.0;
+    q = q * s2 + 1.0 / 19.0;
+    q = q * s2 + 1.0 / 17.0;
+    q = q * s2 + 1.0 / 15.0;
+    q = q * s2 + 1.0 / 13.0;
+    q = q * s2 + 1.0 / 11.0;
+    q = q * s2 + 1.0 / 9.0;
+    q = q * s2 + 1.0 / 7.0;
+    q = q * s2 + 1.0 / 5.0;
+    q = q * s2 + 1.0 / 3.0;
+    q = q * s2 + 1.0;
+    const double ln_m = 2.0 * s * q;
+    return (float)((double)e + ln_m * 1.4426950408889634074);
+}
+
+// sampleBilinear + bilinearLerpColors, Texture.h:148-228
+__device__ inline uint32_t tex_bilinear(const TexDev &t, int lv, float u, float v)
+{
+    const int w = t.w[lv], h = t.h[lv];
+    const float px = u * (float)(w - 1), py = v * (float)(h - 1);
+    const int x0 = imax(0, f2i(floorf(px))), y0 = imax(0, f2i(floorf(py)));
+    const int x1 = tex_imin(w - 1, x0 + 1), y1 = tex_imin(h - 1, y0 + 1);
+    const float fx = px - (float)x0, fy = py - (float)y0;
+    const uint32_t *p = t.texels + t.off[lv];
+    const uint32_t c00 = p[(size_t)y0 * w + x0], c10 = p[(size_t)y0 * w + x1];
+    const uint32_t c01 = p[(size_t)y1 * w + x0], c11 = p[(size_t)y1 * w + x1];
+    uint32_t out[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float sum = (float)tex_ch(c00, k) * (1.0f - fx) * (1.0f - fy) +
+                          (float)tex_ch(c10, k) * fx * (1.0f - fy) +
+                          (float)tex_ch(c01, k) * (1.0f - fx) * fy + (float)tex_ch(c11, k) * fx * fy;
+        out[k] = (uint32_t)(uint8_t)f2i(sum);
+    }
+    return tex_pack(out[0], out[1], out[2]);
+}
+
+// sampleTrilinear + lerpColors, Texture.h:121-146, 200-205
+__device__ inline uint32_t tex_trilinear(const TexDev &t, float u, float v, float rho)
+{
+    float lod = tex_log2(rho < 1e-6f ? 1e-6f : rho);
+    lod = tex_clamp(lod, 0.0f, (float)(t.levels - 1));
+    const int base = f2i(floorf(lod));
+    const int next = tex_imin(base + 1, t.levels - 1);
+    const float frac = tex_clamp(lod - (float)base, 0.0f, 1.0f);
+    const uint32_t cb = tex_bilinear(t, base, u, v);
+    if (base == next)
+        return cb;
+    const uint32_t cn = tex_bilinear(t, next, u, v);
+    uint32_t out[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int d = (int)tex_ch(cn, k) - (int)tex_ch(cb, k);
+        const uint8_t step = (uint8_t)f2i((float)d * frac);
+        out[k] = (uint8_t)(tex_ch(cb, k) + step);
+    }
+    return tex_pack(out[0], out[1], out[2]);
+}
+
+// Texture::sample, Texture.h:35-118
+__device__ inline uint32_t tex_sample(const TexDev &t, float u, float v, float dudx, float dvdx,
+                                      float dudy, float dvdy)
+{
+    u = tex_wrap(u);
+    v = tex_wrap(v);
+    const float W = (float)t.w[0], H = (float)t.h[0];
+    const float dudx_s = dudx * W, dvdx_s = dvdx * H, dudy_s = dudy * W, dvdy_s = dvdy * H;
+    float dx_len = sqrtf(dudx_s * dudx_s + dvdx_s * dvdx_s);
+    float dy_len = sqrtf(dudy_s * dudy_s + dvdy_s * dvdy_s);
+    dx_len = ref_max(dx_len, 1e-6f);
+    dy_len = ref_max(dy_len, 1e-6f);
+    const float major = ref_max(dx_len, dy_len), minor = ref_min(dx_len, dy_len);
+    const float ratio = ref_min(major / minor, (float)t.max_aniso);
+    const int n = imax(1, f2i(ceilf(ratio)));
+    if (n <= 1)
+        return tex_trilinear(t, u, v, major);
+    float mdu, mdv;
+    if (dx_len > dy_len) {
+        mdu = dudx / dx_len;
+        mdv = dvdx / dx_len;
+    } else {
+        mdu = dudy / dy_len;
+        mdv = dvdy / dy_len;
+    }
+    float r = 0, g = 0, b = 0;
+    const float step = 1.0f / (float)n;
+    for (int i = 0; i < n; ++i) {
+        const float tt = ((float)i + 0.5f) * step - 0.5f;
+        const float su = tex_wrap(u + mdu * major * tt);
+        const float sv = tex_wrap(v + mdv * major * tt);
+        const uint32_t c = tex_trilinear(t, su, sv, minor);
+        r += (float)tex_ch(c, 0);
+        g += (float)tex_ch(c, 1);
+        b += (float)tex_ch(c, 2);
+    }
+    r = ref_min(255.0f, r / (float)n);
+    g = ref_min(255.0f, g / (float)n);
+    b = ref_min(255.0f, b / (float)n);
+    return tex_pack((uint8_t)f2i(r), (uint8_t)f2i(g), (uint8_t)f2i(b));
+}
+
+// The Box demo's shader (Box.cpp:39-53): computePerspectiveDerivatives of pvar 0 and 1
+// (PixelData.h:127-145: the planes evaluated directly at the pixel centre), then sample.
+template <class L>
+struct ProgTexture {
+    static constexpr int KEY_WORD = REC_KEY;
+    static constexpr bool PLANES = true;
+    struct State { uint32_t c; bool hit; };
+    __device__ static void load(State &s, const RasterArgs &, size_t) { s.hit = false; s.c = 0; }
+    __device__ static void derivs(const float *pl, const float *iw, float xf, float yf,
+                                  float &ddx, float &ddy)
+    {
+        const float var = pl[0] * xf + pl[1] * yf + pl[2];
+        const float cur = iw[0] * xf + iw[1] * yf + iw[2];
+        ddx = (cur * pl[0] - var * iw[0]) / (cur * cur);
+        ddy = (cur * pl[1] - var * iw[1]) / (cur * cur);
+    }
+    template <class PX>
+    __device__ static void shade(State &s, const RasterArgs &a, const PX &v, uint32_t, int x, int y)
+    {
+        const float w = v.w();
+        const float u = v.pvar(0, w), t = v.pvar(1, w);
+        const float *P = v.rec + REC_PARAM;
+        const float xf = (float)x + 0.5f, yf = (float)y + 0.5f;
+        float dudx, dudy, dvdx, dvdy;
+        derivs(P + 3 * (L::P + 0), P + 3 * L::IW, xf, yf, dudx, dudy);
+        derivs(P + 3 * (L::P + 1), P + 3 * L::IW, xf, yf, dvdx, dvdy);
+        s.c = tex_sample(*a.tex, u, t, dudx, dvdx, dudy, dvdy);
+        s.hit = true;
+    }
+    __device__ static void store(const State &s, const RasterArgs &a, size_t o)
+    {
+        if (s.hit)
+            static_cast<uint32_t *>(a.t0)[o] = s.c;
     }
 };
 

@@ -212,8 +212,13 @@ __global__ void __launch_bounds__(512) k_raster(RasterArgs a0)
             if (s2 < cnt) {
                 const unsigned long long m = s_cov[wv][s2];
                 const uint32_t k2 = __builtin_amdgcn_readlane(key, s2 * 8);
+                const float *rec2 = nullptr;
+                if (Prog::PLANES) {
+                    const uint32_t pv2 = __builtin_amdgcn_readlane(pv, s2 * 8);
+                    rec2 = a.prims + (size_t)(pv2 & PAIR_PRIM_MASK) * (size_t)a.stride;
+                }
                 if (((m >> lane) & 1ull) && inside)
-                    Prog::shade(st, a, ChainPixel<L>{val[s2]}, k2, px, py);
+                    Prog::shade(st, a, ChainPixel<L>{val[s2], rec2}, k2, px, py);
             }
         }
         lds_sync();
@@ -273,6 +278,12 @@ bool raster_layout(const swr_pixel_shader_desc &ps, ParamLayout &L)
             return false;
         L.nz = 1;
         break;
+    case SWR_PS_TEXTURE:
+        if (ps.pvar_count < 2)
+            return false;
+        L.nw = 1;
+        L.np = L.set_np = 2;
+        break;
     case SWR_PS_HASH:
     case SWR_PS_LOG: {
         if (ps.avar_count < 0 || ps.avar_count > 16 || ps.pvar_count < 0 || ps.pvar_count > 16)
@@ -307,6 +318,8 @@ hipError_t launch_raster(const swr_pixel_shader_desc &ps, const ParamLayout &L,
         return launch_tuned<ProgDepthChecker, 1, 1, 3, 2>(a, s, waves, group);
     case SWR_PS_VISIBILITY:
         return launch_tuned<ProgVisibility, 1, 0, 0, 0>(a, s, waves, group);
+    case SWR_PS_TEXTURE:
+        return launch<ProgTexture, 0, 1, 0, 2>(a, s, waves);
     case SWR_PS_HASH:
         return L.na == 4 ? launch<ProgHash, 1, 1, 4, 4>(a, s, waves)
                          : launch<ProgHash, 1, 1, 16, 16>(a, s, waves);

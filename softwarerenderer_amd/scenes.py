@@ -285,6 +285,34 @@ def obj_scene(path: str, angle: float = 0.6, width: int = 640, height: int = 480
                  {"name": "obj", "path": path, "angle": angle})
 
 
+def floor_scene(width: int = 640, height: int = 480, extent: float = 20.0,
+                repeat: float = 16.0) -> Scene:
+    """A textured floor quad (y = -1, +-extent, uv 0..repeat) seen from just above it: grazing
+    footprints (anisotropic sampling, minification down the mip chain) and a near-plane clip.
+    Vertices in the {x, y, z, nx, ny, nz, u, v} layout of SWR_VS_MVP_POS_UV."""
+    e, r = extent, repeat
+    v = np.array([[-e, -1.0, -e, 0, 1, 0, 0.0, 0.0], [e, -1.0, -e, 0, 1, 0, r, 0.0],
+                  [e, -1.0, e, 0, 1, 0, r, r], [-e, -1.0, e, 0, 1, 0, 0.0, r]], np.float32)
+    idx = np.array([0, 2, 1, 0, 3, 2], np.int32)  # counter-clockwise seen from above
+    view = lookat((0.3, -0.4, 6.0), (0.0, -1.0, -10.0), (0, 1, 0))
+    mvp = (perspective(60.0, width / height, 0.1, 50.0) @ view).astype(np.float32)
+    return Scene(v, idx, mvp, width, height, {"name": "floor"})
+
+
+def procedural_texture(kind: str, width: int, height: int, seed: int = 0) -> np.ndarray:
+    """uint32 [height, width] texels 0x00RRGGBB: 'noise' (uniform random bytes), 'checker'
+    (8-texel cells with a colour gradient)."""
+    rng = np.random.default_rng(seed)
+    if kind == "noise":
+        return rng.integers(0, 1 << 24, (height, width), dtype=np.uint32)
+    yy, xx = np.mgrid[0:height, 0:width]
+    cell = ((xx // 8 + yy // 8) % 2).astype(np.uint32)
+    r = (xx * 255 // max(width - 1, 1)).astype(np.uint32)
+    g = (yy * 255 // max(height - 1, 1)).astype(np.uint32)
+    b = np.where(cell == 1, 230, 30).astype(np.uint32)
+    return (r << 16) | (g << 8) | b
+
+
 # RasterizerVertex, IRasterizer.h:42-53 (x, y, z, w, avar[16], pvar[16])
 RASTERIZER_VERTEX = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("w", "<f4"),
                               ("avar", "<f4", (16,)), ("pvar", "<f4", (16,))])

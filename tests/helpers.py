@@ -54,6 +54,10 @@ def run_gpu(pipe: O.Pipeline, scene=None, mode=O.DRAW_TRIANGLE, indices=None, ve
         ps.log, ps.log_count = log, log_count
 
     r = rasterizer or api.Rasterizer(0)
+    tex = None
+    if pipe.ps_program == O.PS_TEXTURE:
+        tex = api.Texture(r, pipe.texture.texels, pipe.texture.max_anisotropy)
+        ps.texture = tex
     r.setRasterMode(pipe.raster_mode)
     r.setScissorRect(*pipe.scissor)
     r.setPixelShader(ps)
@@ -83,6 +87,8 @@ def run_gpu(pipe: O.Pipeline, scene=None, mode=O.DRAW_TRIANGLE, indices=None, ve
     if log is not None:
         out.log = log.cpu().numpy().view(np.uint32).copy()
         out.log_count = log_count.cpu().numpy().view(np.uint64).copy()
+    if tex is not None:
+        tex.close()
     if rasterizer is None:
         r.close()
     return out

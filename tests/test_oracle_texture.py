@@ -1,0 +1,168 @@
+"""The oracle's texture sampler (Texture.h) pinned by what the mathematics fixes.
+
+The reference's Texture class needs SDL (Texture.h:3, SDL_Surface), which is absent here,
+so it cannot be compiled beside the renderer; these checks pin the restatement instead:
+the mip chain against hand-computed 2x2 box filters (odd sizes, edge repeats), bilinear
+weights at exact texel positions and midpoints, the trilinear blend between two levels,
+the wrap of texture coordinates, the anisotropic footprint (sample count, direction, and
+equality with the trilinear tap when isotropic), and the level-of-detail log2 against
+float64 log2 rounded once.  Each expected value is worked out from the formula's meaning,
+not by re-running the formula."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+def rgb(r, g, b):
+    return (r << 16) | (g << 8) | b
+
+
+def chans(c):
+    return (c >> 16) & 255, (c >> 8) & 255, c & 255
+
+
+def test_mip_chain_sizes_and_box_filter():
+    rng = np.random.default_rng(1)
+    tex = rng.integers(0, 1 << 24, (5, 7), dtype=np.uint32)
+    t = O.OracleTexture(tex)
+    sizes = [t.level(k).shape for k in range(t.levels)]
+    assert sizes == [(5, 7), (2, 3), (1, 1)]  # halve (floor), at least 1, down to 1x1
+    lv0, lv1 = t.level(0), t.level(1)
+    assert np.array_equal(lv0, tex & 0xFFFFFF)
+    for y in range(2):
+        for x in range(3):
+            block = [int(lv0[2 * y + dy, 2 * x + dx]) for dy in (0, 1) for dx in (0, 1)]
+            want = tuple(sum(chans(c)[k] for c in block) >> 2 for k in range(3))
+            assert chans(int(lv1[y, x])) == want
+    # the 1x1 level 2 averages texels (0,0), (1,0), (0,1), (1,1) of the 3x2 level 1
+    block = [int(lv1[dy, dx]) for dy in (0, 1) for dx in (0, 1)]
+    assert chans(int(t.level(2)[0, 0])) == tuple(sum(chans(c)[k] for c in block) >> 2
+                                                 for k in range(3))
+
+
+def test_mip_edge_repeat():
+    # 3x1: the last column has no right neighbour and no row below: p10 = p01 = p11 = p00
+    tex = np.array([[rgb(8, 16, 32), rgb(0, 0, 0), rgb(40, 80, 120)]], np.uint32)
+    t = O.OracleTexture(tex)
+    assert t.level(1).shape == (1, 1)
+    # level 1 texel 0 takes p00 = (8,16,32), p10 = (0,0,0), p01 = p11 = p00 (no row below)
+    assert chans(int(t.level(1)[0, 0])) == ((8 + 0 + 8 + 8) >> 2, (16 + 0 + 16 + 16) >> 2,
+                                            (32 + 0 + 32 + 32) >> 2)
+
+
+def test_constant_texture_everywhere():
+    # the four bilinear weights sum to 1 only up to rounding and the Uint8 conversion
+    # truncates, so a constant channel c comes back as c or c - 1 (never anything else)
+    c = (200, 100, 50)
+    t = O.OracleTexture(np.full((16, 32), rgb(*c), np.uint32))
+    rng = np.random.default_rng(2)
+    seen = set()
+    for _ in range(300):
+        u, v = rng.uniform(-3, 3, 2)
+        d = rng.normal(0, 0.2, 4)
+        got = chans(t.sample(u, v, *d))
+        for k in range(3):
+            assert got[k] in (c[k], c[k] - 1)
+        seen.add(got)
+    assert c in seen
+    # at exact texel positions the weights are exactly 1 and 0
+    assert t.sample(0.0, 0.0, 1e-7, 0, 0, 1e-7) == rgb(*c)
+
+
+def test_bilinear_weights_on_level_zero():
+    # a 3x2 texture; tiny derivatives keep lod 0 and one tap
+    tex = np.array([[rgb(0, 0, 0), rgb(255, 10, 0), rgb(20, 20, 20)],
+                    [rgb(100, 0, 0), rgb(0, 40, 0), rgb(0, 0, 90)]], np.uint32)
+    t = O.OracleTexture(tex)
+    eps = 1e-7
+    # exact texel positions: u = x / (w - 1), v = y / (h - 1)
+    assert t.sample(0.5, 0.0, eps, 0, 0, eps) == rgb(255, 10, 0)
+    # halfway between texels (0,0) and (1,0): px = 0.25 * 2 = 0.5
+    assert chans(t.sample(0.25, 0.0, eps, 0, 0, eps)) == (127, 5, 0)  # 127.5, 5.0 truncated
+    # centre of the 2x2 block (0..1, 0..1): px = 0.5, py = 0.5 -> the mean, truncated
+    got = chans(t.sample(0.25, 0.5, eps, 0, 0, eps))
+    assert got == ((0 + 255 + 100 + 0) // 4, (0 + 10 + 0 + 40) // 4, 0)
+
+
+def test_wrap_of_coordinates():
+    rng = np.random.default_rng(3)
+    t = O.OracleTexture(rng.integers(0, 1 << 24, (8, 8), dtype=np.uint32))
+    d = (0.01, 0.0, 0.0, 0.01)
+    for u, v in [(0.25, 0.625), (0.125, 0.5)]:
+        base = t.sample(u, v, *d)
+        assert t.sample(u + 1.0, v, *d) == base
+        assert t.sample(u - 1.0, v - 2.0, *d) == base
+        assert t.sample(u + 3.0, v + 1.0, *d) == base
+
+
+def test_trilinear_between_two_levels():
+    # texel checkerboard 0 / 255: level 1 is (0 + 255 + 255 + 0) >> 2 = 127 everywhere
+    n = 8
+    yy, xx = np.mgrid[0:n, 0:n]
+    tex = np.where((xx + yy) % 2 == 0, 0, rgb(255, 255, 255)).astype(np.uint32)
+    t = O.OracleTexture(tex, max_anisotropy=1)
+    assert set(np.unique(t.level(1))) == {rgb(127, 127, 127)}
+    # rho = footprint in texels = |d| * n; lod = log2(rho)
+    def at(rho):
+        d = rho / n
+        return chans(t.sample(0.0, 0.0, d, 0.0, 0.0, d))  # texel (0,0) = 0 on level 0
+    assert at(1.0) == (0, 0, 0)                             # lod 0: level 0 alone
+    assert at(2.0) == (127, 127, 127)                       # lod 1: level 1 alone
+    assert at(2.0 ** 0.3) == (38, 38, 38)                   # lod 0.3: 0 + (u8)(127 * 0.3)
+    assert at(0.25) == (0, 0, 0)                            # lod < 0 clamps to 0
+    # lod beyond the last level clamps to it (1x1 of the chain)
+    last = chans(int(t.level(t.levels - 1)[0, 0]))
+    assert at(4096.0) == last
+
+
+def test_lerp_downwards_wraps_like_x86():
+    # level 0 texel 255, level 1 texel 127: the difference is negative, the Uint8 conversion
+    # of (127 - 255) * t truncates toward zero and adds modulo 256
+    n = 8
+    yy, xx = np.mgrid[0:n, 0:n]
+    tex = np.where((xx + yy) % 2 == 0, rgb(255, 255, 255), 0).astype(np.uint32)
+    t = O.OracleTexture(tex, max_anisotropy=1)
+    d = (2.0 ** 0.3) / n
+    # lod 0.3: 255 + (u8)(-128 * 0.3 = -38.4 -> -38) = 255 - 38 = 217
+    assert chans(t.sample(0.0, 0.0, d, 0.0, 0.0, d)) == (217, 217, 217)
+
+
+def test_anisotropic_footprint():
+    # a texture varying along u only: taps spread along the major axis (u) average it
+    n = 16
+    col = np.arange(n, dtype=np.uint32) * 16
+    tex = np.tile(rgb(1, 1, 1) * col[None, :], (n, 1)).astype(np.uint32)
+    t = O.OracleTexture(tex, max_anisotropy=8)
+    u0, v0 = 0.5, 0.5
+    iso = t.sample(u0, v0, 1e-4, 0.0, 0.0, 1e-4)
+    # equal axes: ratio 1, one trilinear tap at the texel position u0 * 15 = 7.5 -> the
+    # midpoint of texels 7 and 8 (112 and 128), 120
+    assert chans(iso) == (120, 120, 120)
+    # any ratio above 1 rounds up to two taps, half a footprint apart along the major axis
+    two = chans(t.sample(u0, v0, 1e-4 * 1.001, 0.0, 0.0, 1e-4))
+    assert two[0] in (119, 120)
+    # a footprint 4 texels along u, 1 along v: 4 taps at u0 + (i + 1/2)/4 - 1/2 of 4 texels
+    du = 4.0 / n
+    got = chans(t.sample(u0, v0, du, 0.0, 0.0, 1.0 / n))
+    assert got[0] == got[1] == got[2]
+    taps = []
+    for i in range(4):
+        tt = (i + 0.5) / 4 - 0.5
+        su = u0 + (du / (du * n)) * (du * n) * tt
+        taps.append(chans(t.sample(su, v0, 1e-4, 0.0, 0.0, 1e-4))[0])
+    assert abs(got[0] - sum(taps) / 4) <= 1.0
+    # anisotropy limited: a 64:1 footprint takes max_anisotropy taps, stays within the texture
+    far = chans(t.sample(u0, v0, 64.0 / n, 0.0, 0.0, 1.0 / n))
+    assert 0 <= far[0] <= 255
+
+
+def test_log2_rounding():
+    rng = np.random.default_rng(4)
+    x = np.concatenate([rng.uniform(1e-6, 1e4, 4000), 2.0 ** np.arange(-19, 20, dtype=float),
+                        rng.lognormal(0, 5, 4000)]).astype(np.float32)
+    x = x[x >= np.float32(1e-6)]
+    want = np.log2(x.astype(np.float64)).astype(np.float32)
+    got = np.array([O.log2_rounded(float(v)) for v in x], np.float32)
+    assert np.array_equal(got, want)
+    assert O.log2_rounded(1.0) == 0.0 and O.log2_rounded(1024.0) == 10.0
