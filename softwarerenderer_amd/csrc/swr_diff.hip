@@ -79,6 +79,8 @@ struct DiffArgs {
                        // below its last row
     uint32_t *queue;   // pairs that need the silhouette walk (pair index 2 * pixel + dir)
     uint32_t *qcount;
+    uint32_t *elist;   // pixels with an active pair (the blend pass lists them)
+    uint32_t *ecount;
     int pairs;     // 0: evaluate, 1: evaluate and store, 2: load the stored decisions
 };
 
@@ -456,6 +458,8 @@ struct VertexAcc {
 
 __device__ inline int acc_slot(const VertexAcc &A, int v)
 {
+    if (!A.key)
+        return -1;  // no table: straight to global memory
     const unsigned h = ((unsigned)v * 2654435761u) >> 24;  // 8 bits: ACC_SLOTS
 #pragma unroll 1
     for (int probe = 0; probe < ACC_PROBES; ++probe) {
@@ -575,12 +579,38 @@ __device__ inline void bary_backward(const DiffArgs &d, const VertexAcc &A, int 
         acc_vertex<C>(d, A, view, B.v[m], (float)g[m].x, (float)g[m].y, (float)g[m].w, ga[m]);
 }
 
-template <int C>
-__global__ void __launch_bounds__(256) k_diff_interp(DiffArgs d)
+// Up to two values per thread appended with one global atomic per workgroup (a single
+// counter shared by every workgroup serialises its atomics); every thread of the
+// workgroup must call it.
+__device__ inline void block_append2(uint32_t *list, uint32_t *count, bool t0, uint32_t v0,
+                                     bool t1, uint32_t v1)
 {
-    PixelIdx pi;
-    if (!pixel_of_thread(d, pi))
-        return;
+    __shared__ uint32_t s_n, s_base;
+    const unsigned lane = threadIdx.x & 63u;
+    if (threadIdx.x == 0)
+        s_n = 0;
+    __syncthreads();
+    const unsigned long long m0 = __ballot(t0), m1 = __ballot(t1);
+    const uint32_t n = (uint32_t)(__popcll(m0) + __popcll(m1));
+    uint32_t woff = 0;
+    if (lane == 0 && n)
+        woff = atomicAdd(&s_n, n);
+    woff = __shfl(woff, 0, 64);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        s_base = s_n ? atomicAdd(count, s_n) : 0u;
+    __syncthreads();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const uint32_t base = s_base + woff;
+    if (t0)
+        list[base + (uint32_t)__popcll(m0 & below)] = v0;
+    if (t1)
+        list[base + (uint32_t)__popcll(m0) + (uint32_t)__popcll(m1 & below)] = v1;
+}
+
+template <int C>
+__device__ inline void interp_pixel(const DiffArgs &d, const PixelIdx &pi, bool *walk)
+{
     const size_t i = pi.i;
     const int view = pi.view, px = pi.px, py = pi.py;
     const uint32_t id = d.ids[i];
@@ -603,27 +633,27 @@ __global__ void __launch_bounds__(256) k_diff_interp(DiffArgs d)
     if (threadIdx.x == 0)
         d.tflag[pi.tile] = 0u;
     // this pixel's two pairs (to the right, below): decided here unless a silhouette walk
-    // is needed, which is queued for k_diff_march
-    const unsigned lane = threadIdx.x & 63u;
+    // is needed
 #pragma unroll
     for (int dir = 0; dir < 2; ++dir) {
         const int qx = px + (dir == 0), qy = py + (dir == 1);
         bool pf;
         int cur;
-        const bool walk = qx < d.W && qy < d.H && pair_start(d, view, px, py, qx, qy, pf, cur);
+        walk[dir] = qx < d.W && qy < d.H && pair_start(d, view, px, py, qx, qy, pf, cur);
         d.pflag[2 * i + dir] = 0;
-        const unsigned long long m = __ballot(walk);
-        if (m) {
-            const unsigned leader = (unsigned)__builtin_ctzll(m);
-            uint32_t base = 0;
-            if (lane == leader)
-                base = atomicAdd(d.qcount, (uint32_t)__popcll(m));
-            base = __shfl(base, (int)leader, 64);
-            if (walk)
-                d.queue[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] =
-                    (uint32_t)(2 * i + dir);
-        }
     }
+}
+
+template <int C>
+__global__ void __launch_bounds__(256) k_diff_interp(DiffArgs d)
+{
+    PixelIdx pi;
+    bool walk[2] = {false, false};
+    if (pixel_of_thread(d, pi))
+        interp_pixel<C>(d, pi, walk);
+    // the pairs that need a silhouette walk, queued for k_diff_march
+    const uint32_t k0 = (walk[0] || walk[1]) ? (uint32_t)(2 * pi.i) : 0u;
+    block_append2(d.queue, d.qcount, walk[0], k0, walk[1], k0 + 1u);
 }
 
 
@@ -701,9 +731,12 @@ __device__ inline Pair unpack_pair(const PairLds &x, bool swap, bool horiz)
 
 __global__ void k_diff_clear(DiffArgs d)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0)
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
         *d.qcount = 0u;
+        *d.ecount = 0u;
+    }
 }
+
 
 // The queued pairs, one lane each: the silhouette walk, the stored decision, and the tile
 // flags of active pairs.
@@ -791,6 +824,27 @@ __device__ inline void tile_pairs(const DiffArgs &d, const PixelIdx &pi, bool va
 }
 
 // The pair between this thread's pixel and neighbour k (0 right, 1 left, 2 below, 3 above).
+__device__ inline Pair tile_pair(const PairLds *h, const PairLds *v, int k);
+
+// Where a pixel's four pairs come from: the workgroup's LDS table, or the stored decisions.
+struct TilePairs {
+    const PairLds *h, *v;
+    __device__ Pair get(int k) const { return tile_pair(h, v, k); }
+};
+struct StoredPairs {
+    PairLds p[4];  // right, left, below, above, each seen from its left / upper pixel
+    __device__ StoredPairs(const DiffArgs &d, size_t i, int px, int py)
+    {
+        PairLds none;
+        none.flags = 0;
+        p[0] = px + 1 < d.W ? load_pair(d, 2 * i) : none;
+        p[1] = px > 0 ? load_pair(d, 2 * (i - 1)) : none;
+        p[2] = py + 1 < d.H ? load_pair(d, 2 * i + 1) : none;
+        p[3] = py > 0 ? load_pair(d, 2 * (i - d.W) + 1) : none;
+    }
+    __device__ Pair get(int k) const { return unpack_pair(p[k], k == 1 || k == 3, k < 2); }
+};
+
 __device__ inline Pair tile_pair(const PairLds *h, const PairLds *v, int k)
 {
     const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
@@ -820,6 +874,13 @@ __global__ void __launch_bounds__(256) k_diff_blend(DiffArgs d)
     const bool valid = pixel_of_thread(d, pi);
     tile_pairs<true>(d, pi, valid, s_h, s_v);
     __syncthreads();
+    // the backward pass's antialiasing work is per pixel with an active pair: list them
+    bool any = false;
+    for (int k = 0; k < 4 && valid; ++k) {
+        int qx, qy;
+        any = any || (neighbour(d, pi.px, pi.py, k, qx, qy) && tile_pair(s_h, s_v, k).active);
+    }
+    block_append2(d.elist, d.ecount, any, any ? (uint32_t)pi.i : 0u, false, 0u);
     if (!valid)
         return;
     const size_t i = pi.i;
@@ -874,9 +935,9 @@ __device__ inline void scatter_window_grad(const DiffArgs &d, const VertexAcc &A
 // (the interpolation backward is linear in the colour gradient, so the two parts add up).
 enum BwdPart { BWD_ALL, BWD_INTERIOR, BWD_EDGES };
 
-template <int C, int PART>
+template <int C, int PART, class PS>
 __device__ inline void backward_pixel(const DiffArgs &d, const VertexAcc &A, const PixelIdx &pi,
-                                      const PairLds *s_h, const PairLds *s_v)
+                                      const PS &pairs)
 {
     const size_t HW = (size_t)d.W * d.H;
     const size_t i = pi.i;
@@ -895,7 +956,7 @@ __device__ inline void backward_pixel(const DiffArgs &d, const VertexAcc &A, con
         int qx, qy;
         if (!neighbour(d, px, py, k, qx, qy))
             continue;
-        const Pair pr = tile_pair(s_h, s_v, k);
+        const Pair pr = pairs.get(k);
         if (!pr.active)
             continue;
         const size_t iq = (size_t)view * HW + (size_t)qy * d.W + qx;
@@ -982,9 +1043,30 @@ __global__ void __launch_bounds__(256) k_diff_backward(DiffArgs d)
     __syncthreads();
     const VertexAcc A{s_key, s_val};
     if (valid)  // every thread reaches the barrier below
-        backward_pixel<C, PART>(d, A, pi, s_h, s_v);
+        backward_pixel<C, PART>(d, A, pi, TilePairs{s_h, s_v});
     __syncthreads();
     acc_flush<C>(d, A, pi.view);
+}
+
+// What the antialiasing adds to the gradients, for the pixels the blend pass listed (one
+// lane each; the decisions come from the forward's store, gradients go straight to global
+// memory: the list is short).
+template <int C>
+__global__ void __launch_bounds__(256) k_diff_bwd_edges(DiffArgs d)
+{
+    const uint32_t n = *d.ecount;
+    const size_t HW = (size_t)d.W * d.H;
+    const VertexAcc A{nullptr, nullptr};
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        PixelIdx pi;
+        pi.i = d.elist[j];
+        pi.view = (int)(pi.i / HW);
+        const size_t rem = pi.i - (size_t)pi.view * HW;
+        pi.py = (int)(rem / (size_t)d.W);
+        pi.px = (int)(rem - (size_t)pi.py * d.W);
+        pi.tile = 0;
+        backward_pixel<C, BWD_EDGES>(d, A, pi, StoredPairs(d, pi.i, pi.px, pi.py));
+    }
 }
 
 // grad_attributes += sum over views of the partials; the partials are cleared for the next
@@ -1019,8 +1101,10 @@ static hipError_t launch_diff(const DiffArgs &d, int which, hipStream_t s)
         hipLaunchKernelGGL(k_diff_blend<C>, dim3(blocks), dim3(256), 0, s, d);
     } else {
         if (d.pairs == 2) {
+            const size_t px = (size_t)d.V * d.W * d.H;
+            const uint32_t eb = (uint32_t)((px + 255) / 256 < 1024 ? (px + 255) / 256 : 1024);
             hipLaunchKernelGGL((k_diff_backward<C, BWD_INTERIOR>), dim3(blocks), dim3(256), 0, s, d);
-            hipLaunchKernelGGL((k_diff_backward<C, BWD_EDGES>), dim3(blocks), dim3(256), 0, s, d);
+            hipLaunchKernelGGL(k_diff_bwd_edges<C>, dim3(eb), dim3(256), 0, s, d);
         } else {
             hipLaunchKernelGGL((k_diff_backward<C, BWD_ALL>), dim3(blocks), dim3(256), 0, s, d);
         }
@@ -1089,21 +1173,24 @@ static int diff_run(swr_context *c, const swr_diff_desc *dd, const float *gimage
     }
     // The forward stores its pair decisions; a backward over the same buffers and sizes
     // reuses them instead of walking the silhouettes again.
-    // layout: queue count (256 B), flags [2 px] u8, data [2 px] int2, tile flags u32,
-    // queue [2 px] u32
+    // layout: queue and list counts (256 B), flags [2 px] u8, data [2 px] int2, tile flags
+    // u32, queue [2 px] u32, list [px] u32
     const size_t tiles = (size_t)d.tiles_per_view * d.V, off_f = 256,
                  off_d = (off_f + px * 2 + 15) & ~(size_t)15, off_t = off_d + px * 2 * sizeof(int2),
                  off_q = (off_t + tiles * 4 + 15) & ~(size_t)15;
     if (px * 2 > 0xFFFFFFFFull)
         return context_fail(c, SWR_ERR_CAPACITY, "too many pixels for one swr_diff call");
-    uint8_t *pbuf = (uint8_t *)context_zeroed_scratch(c, 1, off_q + px * 2 * 4);
+    const size_t off_e = off_q + px * 2 * 4;
+    uint8_t *pbuf = (uint8_t *)context_zeroed_scratch(c, 1, off_e + px * 4);
     if (!pbuf)
         return SWR_ERR_HIP;
     d.qcount = reinterpret_cast<uint32_t *>(pbuf);
+    d.ecount = reinterpret_cast<uint32_t *>(pbuf) + 1;
     d.pflag = pbuf + off_f;
     d.pdata = reinterpret_cast<int2 *>(pbuf + off_d);
     d.tflag = reinterpret_cast<uint32_t *>(pbuf + off_t);
     d.queue = reinterpret_cast<uint32_t *>(pbuf + off_q);
+    d.elist = reinterpret_cast<uint32_t *>(pbuf + off_e);
     const uint64_t tag[8] = {(uint64_t)(uintptr_t)q.clip,      (uint64_t)(uintptr_t)q.triangles,
                              (uint64_t)(uintptr_t)q.adjacency, (uint64_t)(uintptr_t)q.ids,
                              (uint64_t)(uintptr_t)q.depth,     (uint64_t)(uintptr_t)pbuf,
