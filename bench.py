@@ -5,7 +5,7 @@ binning, raster, depth-tested checker shading into RGBA8 + fp32 depth) preceded 
 target clear, with vertices and indices resident in HBM.  Prints one JSON line.
 
 The line also carries the differentiation workload (64 views at 512x512 of a 40,962-vertex
-icosphere, forward + backward of an L2 image loss): views per second of the batched step
+icosphere, forward + backward of the image loss 1/2 |image - target|^2): views per second of the batched step
 (one visibility draw for all views, captured in a HIP graph) against the same work issued
 one view at a time.
 
@@ -88,7 +88,7 @@ def diff_bench(args, dev):
     def step(rr, m, tgt):
         clip = torch.einsum("vij,nj->vni", m, hom).contiguous()
         img = rr.forward(clip, col)
-        g = 2.0 * (img - tgt)
+        g = img - tgt  # dL/dimage of L = 1/2 |image - target|^2
         gclip, gattr = rr.backward(g)
         gpos = torch.einsum("vni,vij->nj", gclip, m)[:, :3]
         return gpos, gattr

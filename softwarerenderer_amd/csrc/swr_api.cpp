@@ -41,6 +41,8 @@ struct swr_context {
     int device = 0;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;               // the clip queue's emit (fork / join)
+    hipEvent_t fork = nullptr, join = nullptr;
     std::string err;
 
     // Rasterizer (Rasterizer.h:67-72 defaults)
@@ -518,7 +520,7 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
         HIP_TRY(c, scan_pairs(g.grp, (uint2 *)c->grp_ex.p, ngrp, (uint2 *)c->scan_tmp.p, g.status,
                               c->prim_cap, c->pair_cap, c->stream));
         record(c, 2);
-        HIP_TRY(c, launch_geom_emit(g, c->stream));
+        HIP_TRY(c, launch_geom_emit(g, c->stream, c->side, c->fork, c->join));
         HIP_TRY(c, launch_expand(g, c->stream, c->tune.expand_chunk));
         record(c, 3);
         SortBuffers sb;
@@ -855,6 +857,13 @@ int swr_create(swr_context **out, int device)
     c->stream = c->own;
     for (int i = 0; i <= S_COUNT; ++i)
         (void)hipEventCreate(&c->ev[i]);
+    // the side stream is optional: without it the clip queue's emit stays on the draw stream
+    if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        c->side = nullptr;
+    }
     *out = c;
     return SWR_OK;
 }
@@ -878,6 +887,14 @@ int swr_destroy(swr_context *c)
             (void)hipEventDestroy(c->ev[i]);
     if (c->status_host)
         (void)hipHostFree(c->status_host);
+    if (c->side) {
+        (void)hipStreamSynchronize(c->side);
+        (void)hipStreamDestroy(c->side);
+    }
+    if (c->fork)
+        (void)hipEventDestroy(c->fork);
+    if (c->join)
+        (void)hipEventDestroy(c->join);
     if (c->own)
         (void)hipStreamDestroy(c->own);
     delete c;

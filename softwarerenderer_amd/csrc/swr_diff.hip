@@ -518,6 +518,40 @@ __device__ inline void acc_vertex(const DiffArgs &d, const VertexAcc &A, int vie
     }
 }
 
+// The three vertices of one pixel's triangle: the first probes of the three lookups are
+// issued together (one LDS round trip instead of three in the common case that the vertex
+// already has its slot), then the adds.
+template <int C>
+__device__ inline void acc_triangle(const DiffArgs &d, const VertexAcc &A, int view,
+                                   const int (&v)[3], const float (&g)[3][3 + C])
+{
+    int sl[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        const int h = (int)((((unsigned)v[m] * 2654435761u) >> 24) & (ACC_SLOTS - 1));
+        sl[m] = __atomic_load_n(&A.key[h], __ATOMIC_RELAXED) == v[m] ? h : -1;
+    }
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        if (sl[m] < 0)
+            sl[m] = acc_slot(A, v[m]);
+        if (sl[m] >= 0) {
+            float *x = &A.val[sl[m] * ACC_VALS * ACC_COPIES + acc_copy()];
+#pragma unroll
+            for (int k = 0; k < 3 + C; ++k)
+                atomicAdd(x + k * ACC_COPIES, g[m][k]);
+        } else {
+            float *gc = d.gclip + ((size_t)view * d.N + v[m]) * 4;
+            atomicAdd(gc + 0, g[m][0]);
+            atomicAdd(gc + 1, g[m][1]);
+            atomicAdd(gc + 3, g[m][2]);
+#pragma unroll
+            for (int c = 0; c < C; ++c)
+                atomicAdd(d.gattr + ((size_t)view * d.N + v[m]) * C + c, g[m][3 + c]);
+        }
+    }
+}
+
 template <int C>
 __device__ inline void acc_flush(const DiffArgs &d, const VertexAcc &A, int view)
 {
@@ -544,19 +578,19 @@ __device__ inline void acc_flush(const DiffArgs &d, const VertexAcc &A, int view
 
 // Backward of col = sum_m b_m A_m given dL/dcol: adds to the attributes and, through
 // b_m = D_m / S with D_m = r . (h_j x h_k), to the clip coordinates of the three vertices.
+// The gradients bary_backward accumulates, per vertex m: {clip x, y, w, attribute 0..C).
 template <int C>
-__device__ inline void bary_backward(const DiffArgs &d, const VertexAcc &A, int view,
-                                     const Bary &B, const double *gcol)
+__device__ inline void bary_grads(const DiffArgs &d, const Bary &B, const double *gcol,
+                                  float (&out)[3][3 + C])
 {
     double u[3];
-    float ga[3][C];
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
         u[m] = 0.0;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             u[m] += gcol[c] * d.attr[(size_t)B.v[m] * C + c];
-            ga[m][c] = (float)(B.b[m] * gcol[c]);
+            out[m][3 + c] = (float)(B.b[m] * gcol[c]);
         }
     }
     const double ub = u[0] * B.b[0] + u[1] * B.b[1] + u[2] * B.b[2];
@@ -575,8 +609,22 @@ __device__ inline void bary_backward(const DiffArgs &d, const VertexAcc &A, int 
         g[k].w += gD * dk.w;
     }
 #pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        out[m][0] = (float)g[m].x;
+        out[m][1] = (float)g[m].y;
+        out[m][2] = (float)g[m].w;
+    }
+}
+
+template <int C>
+__device__ inline void bary_backward(const DiffArgs &d, const VertexAcc &A, int view,
+                                     const Bary &B, const double *gcol)
+{
+    float g[3][3 + C];
+    bary_grads<C>(d, B, gcol, g);
+#pragma unroll
     for (int m = 0; m < 3; ++m)
-        acc_vertex<C>(d, A, view, B.v[m], (float)g[m].x, (float)g[m].y, (float)g[m].w, ga[m]);
+        acc_vertex<C>(d, A, view, B.v[m], g[m][0], g[m][1], g[m][2], &g[m][3]);
 }
 
 // Up to two values per thread appended with one global atomic per workgroup (a single
@@ -1048,6 +1096,94 @@ __global__ void __launch_bounds__(256) k_diff_backward(DiffArgs d)
     acc_flush<C>(d, A, pi.view);
 }
 
+// DPP lane shifts inside rows of 16 lanes (one pixel row of the 16x16 tile): lane i reads
+// lane i + n; a read past the row end yields `edge`.
+template <int N>
+__device__ inline int row_next(int v, int edge)
+{
+    return __builtin_amdgcn_update_dpp(edge, v, 0x100 + N, 0xF, 0xF, false);
+}
+template <int N>
+__device__ inline float row_next(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x100 + N, 0xF, 0xF,
+                                                      false));
+}
+template <int N>
+__device__ inline int row_prev(int v, int edge)
+{
+    return __builtin_amdgcn_update_dpp(edge, v, 0x110 + N, 0xF, 0xF, false);
+}
+
+// Every pixel's interpolation under dL/dimage.  Neighbouring pixels mostly show the same
+// triangle (82% of horizontal pairs on the 41k-vertex sphere), so before the LDS adds the
+// lanes of a pixel row merge their 18 gradient values pairwise, then in fours and eights
+// (a lane keeps its values when its partner shows another triangle): about half the LDS
+// atomics, which serialise on the shared vertices, go away.
+template <int C>
+__global__ void __launch_bounds__(256) k_diff_bwd_interior(DiffArgs d)
+{
+    constexpr int NV = 3 + C;
+    __shared__ int s_key[ACC_SLOTS];
+    __shared__ __align__(16) float s_val[ACC_SLOTS * ACC_VALS * ACC_COPIES];
+    for (int sl = threadIdx.x; sl < ACC_SLOTS; sl += blockDim.x)
+        s_key[sl] = -1;
+    for (int k = threadIdx.x; k < ACC_SLOTS * ACC_VALS * ACC_COPIES / 4; k += blockDim.x)
+        reinterpret_cast<float4 *>(s_val)[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    PixelIdx pi;
+    const bool valid = pixel_of_thread(d, pi);
+    int tri = -1, vid[3] = {0, 0, 0};
+    float g[3][NV];
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            g[m][k] = 0.0f;
+    if (valid) {
+        const uint32_t id = d.ids[pi.i];
+        Bary B;
+        if (id != 0 && (int)id <= d.T && bary_at(d, pi.view, (int)id - 1, pi.px, pi.py, B)) {
+            double gc[C];
+#pragma unroll
+            for (int c = 0; c < C; ++c)
+                gc[c] = d.gimage[pi.i * C + c];
+            bary_grads<C>(d, B, gc, g);
+            tri = (int)id - 1;
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+                vid[m] = B.v[m];
+        }
+    }
+    // merge along the row: at stride s, lane i (i % 2s == 0) takes lane i + s's sums when
+    // both show the same triangle and lane i + s still holds its own (it merged nothing away)
+    const int lx = (int)(threadIdx.x & 15u);
+    bool keep = tri >= 0;
+#define SWR_MERGE(S)                                                                            \
+    {                                                                                          \
+        const int t_n = row_next<S>(keep ? tri : -2, -3);                                      \
+        const int t_p = row_prev<S>(keep ? tri : -2, -3);                                      \
+        const bool take = keep && (lx % (2 * S)) == 0 && t_n == tri;                           \
+        const bool give = keep && (lx % (2 * S)) == S && t_p == tri;                           \
+        _Pragma("unroll") for (int m = 0; m < 3; ++m)                                          \
+            _Pragma("unroll") for (int k = 0; k < NV; ++k)                                     \
+        {                                                                                      \
+            const float o = row_next<S>(g[m][k]);                                              \
+            g[m][k] = take ? g[m][k] + o : g[m][k];                                            \
+        }                                                                                      \
+        keep = keep && !give;                                                                  \
+    }
+    SWR_MERGE(1)
+    SWR_MERGE(2)
+    SWR_MERGE(4)
+#undef SWR_MERGE
+    __syncthreads();
+    const VertexAcc A{s_key, s_val};
+    if (keep)
+        acc_triangle<C>(d, A, pi.view, vid, g);
+    __syncthreads();
+    acc_flush<C>(d, A, pi.view);
+}
+
 // What the antialiasing adds to the gradients, for the pixels the blend pass listed (one
 // lane each; the decisions come from the forward's store, gradients go straight to global
 // memory: the list is short).
@@ -1103,7 +1239,7 @@ static hipError_t launch_diff(const DiffArgs &d, int which, hipStream_t s)
         if (d.pairs == 2) {
             const size_t px = (size_t)d.V * d.W * d.H;
             const uint32_t eb = (uint32_t)((px + 255) / 256 < 1024 ? (px + 255) / 256 : 1024);
-            hipLaunchKernelGGL((k_diff_backward<C, BWD_INTERIOR>), dim3(blocks), dim3(256), 0, s, d);
+            hipLaunchKernelGGL(k_diff_bwd_interior<C>, dim3(blocks), dim3(256), 0, s, d);
             hipLaunchKernelGGL(k_diff_bwd_edges<C>, dim3(eb), dim3(256), 0, s, d);
         } else {
             hipLaunchKernelGGL((k_diff_backward<C, BWD_ALL>), dim3(blocks), dim3(256), 0, s, d);

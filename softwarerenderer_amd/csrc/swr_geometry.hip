@@ -1139,33 +1139,46 @@ int emit_class(int nvary)
     return 32;
 }
 
-hipError_t launch_geom_emit(const GeomArgs &a, hipStream_t s)
+// The clipped triangles' emit (k_clip_emit) and the others' (k_geom_emit) write disjoint
+// records and pairs, so with a side stream the clip queue runs beside the main emit: its
+// few, long-latency waves fill what the main emit leaves idle.
+template <int NV>
+static void emit_kernels(const GeomArgs &a, hipStream_t s, hipStream_t side, bool q)
+{
+    dim3 g((a.tris + 255) / 256), blk(256), gq(CLIP_BLOCKS(a.tris));
+    if (q)
+        hipLaunchKernelGGL(k_clip_emit<NV == 0 ? 1 : NV>, gq, blk, 0, side ? side : s, a);
+    hipLaunchKernelGGL(k_geom_emit<NV>, g, blk, 0, s, a);
+}
+
+hipError_t launch_geom_emit(const GeomArgs &a, hipStream_t s, hipStream_t side, hipEvent_t fork,
+                            hipEvent_t join)
 {
     if (!a.tris)
         return hipSuccess;
-    dim3 g((a.tris + 255) / 256), blk(256), gq(CLIP_BLOCKS(a.tris));
     const bool q = !a.window_input;
+    const bool split = q && side && fork && join;
+    if (split) {
+        hipError_t e = hipEventRecord(fork, s);
+        if (e == hipSuccess)
+            e = hipStreamWaitEvent(side, fork, 0);
+        if (e != hipSuccess)
+            return e;
+    }
+    hipStream_t sd = split ? side : nullptr;
     switch (emit_class(a.lay.na + a.lay.np)) {
-    case 0:
-        hipLaunchKernelGGL(k_geom_emit<0>, g, blk, 0, s, a);
-        if (q) hipLaunchKernelGGL(k_clip_emit<1>, gq, blk, 0, s, a);
-        break;
-    case 3:
-        hipLaunchKernelGGL(k_geom_emit<3>, g, blk, 0, s, a);
-        if (q) hipLaunchKernelGGL(k_clip_emit<3>, gq, blk, 0, s, a);
-        break;
-    case 5:
-        hipLaunchKernelGGL(k_geom_emit<5>, g, blk, 0, s, a);
-        if (q) hipLaunchKernelGGL(k_clip_emit<5>, gq, blk, 0, s, a);
-        break;
-    case 8:
-        hipLaunchKernelGGL(k_geom_emit<8>, g, blk, 0, s, a);
-        if (q) hipLaunchKernelGGL(k_clip_emit<8>, gq, blk, 0, s, a);
-        break;
-    default:
-        hipLaunchKernelGGL(k_geom_emit<32>, g, blk, 0, s, a);
-        if (q) hipLaunchKernelGGL(k_clip_emit<32>, gq, blk, 0, s, a);
-        break;
+    case 0: emit_kernels<0>(a, s, sd, q); break;
+    case 3: emit_kernels<3>(a, s, sd, q); break;
+    case 5: emit_kernels<5>(a, s, sd, q); break;
+    case 8: emit_kernels<8>(a, s, sd, q); break;
+    default: emit_kernels<32>(a, s, sd, q); break;
+    }
+    if (split) {
+        hipError_t e = hipEventRecord(join, side);
+        if (e == hipSuccess)
+            e = hipStreamWaitEvent(s, join, 0);
+        if (e != hipSuccess)
+            return e;
     }
     return hipGetLastError();
 }
