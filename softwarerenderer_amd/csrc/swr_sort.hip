@@ -21,57 +21,124 @@ template <typename T> __device__ inline T szero();
 template <> __device__ inline uint32_t szero<uint32_t>() { return 0u; }
 template <> __device__ inline uint2 szero<uint2>() { return make_uint2(0u, 0u); }
 
+// Wave-wide inclusive scan (64 lanes, shuffles).
+__device__ inline uint32_t wave_incl(uint32_t v)
+{
+    const int lane = (int)(threadIdx.x & 63u);
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t o = __shfl_up(v, off, 64);
+        if (lane >= off)
+            v += o;
+    }
+    return v;
+}
+__device__ inline uint2 wave_incl(uint2 v) { return make_uint2(wave_incl(v.x), wave_incl(v.y)); }
+__device__ inline uint32_t ssub(uint32_t a, uint32_t b) { return a - b; }
+__device__ inline uint2 ssub(uint2 a, uint2 b) { return make_uint2(a.x - b.x, a.y - b.y); }
+
 // Block-wide exclusive scan of one value per thread; returns the block total in *total.
 template <typename T>
 __device__ T block_exclusive(T v, T *total)
 {
-    __shared__ T s[SCAN_THREADS];
-    const int tid = threadIdx.x;
-    s[tid] = v;
+    constexpr int NW = SCAN_THREADS / 64;
+    __shared__ T s_w[NW];
+    const int lane = (int)(threadIdx.x & 63u), w = (int)(threadIdx.x >> 6);
+    const T inc = wave_incl(v);
+    if (lane == 63)
+        s_w[w] = inc;
     __syncthreads();
-    for (int off = 1; off < SCAN_THREADS; off <<= 1) {
-        T add = tid >= off ? s[tid - off] : szero<T>();
-        __syncthreads();
-        s[tid] = sadd(s[tid], add);
-        __syncthreads();
+    T pre = szero<T>(), tot = szero<T>();
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+        const T x = s_w[k];
+        if (k < w)
+            pre = sadd(pre, x);
+        tot = sadd(tot, x);
     }
-    T incl = s[tid];
-    *total = s[SCAN_THREADS - 1];
-    __syncthreads();
-    T ex = tid ? s[tid - 1] : szero<T>();
-    __syncthreads();
-    (void)incl;
-    return ex;
+    __syncthreads();  // s_w is reused by the next call
+    *total = tot;
+    return sadd(pre, ssub(inc, v));
+}
+
+// SCAN_ITEMS consecutive elements from `base`: 16-byte loads when the run is whole and
+// aligned, element by element (zero past n) otherwise.
+template <typename T>
+__device__ inline void load_items(const T *in, uint64_t base, uint64_t n, T (&v)[SCAN_ITEMS])
+{
+    const T *p = in + base;
+    if (base + SCAN_ITEMS <= n && ((uintptr_t)p & 15u) == 0) {
+        constexpr int Q = SCAN_ITEMS * (int)sizeof(T) / 16;
+        uint4 q[Q];
+#pragma unroll
+        for (int k = 0; k < Q; ++k)
+            q[k] = reinterpret_cast<const uint4 *>(p)[k];
+        __builtin_memcpy(v, q, sizeof v);
+    } else {
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; ++i)
+            v[i] = base + i < n ? p[i] : szero<T>();
+    }
+}
+
+template <typename T>
+__device__ inline void store_items(T *out, uint64_t base, uint64_t n, const T (&v)[SCAN_ITEMS])
+{
+    T *p = out + base;
+    if (base + SCAN_ITEMS <= n && ((uintptr_t)p & 15u) == 0) {
+        constexpr int Q = SCAN_ITEMS * (int)sizeof(T) / 16;
+        uint4 q[Q];
+        __builtin_memcpy(q, v, sizeof q);
+#pragma unroll
+        for (int k = 0; k < Q; ++k)
+            reinterpret_cast<uint4 *>(p)[k] = q[k];
+    } else {
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; ++i)
+            if (base + i < n)
+                p[i] = v[i];
+    }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce(const T *in, uint64_t n, T *sums)
 {
-    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    T v[SCAN_ITEMS];
+    load_items<T>(in, base, n, v);
     T acc = szero<T>();
+#pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i)
-        if (base + i < n)
-            acc = sadd(acc, in[base + i]);
+        acc = sadd(acc, v[i]);
     T total;
     block_exclusive<T>(acc, &total);
     if (threadIdx.x == 0)
         sums[blockIdx.x] = total;
 }
 
-// One block scans the block sums in place (exclusive).
+// One block scans the block sums in place (exclusive), SCAN_TILE sums per step.
 template <typename T>
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_sums(T *sums, uint32_t nblocks)
 {
     T carry = szero<T>();
-    for (uint32_t base = 0; base < nblocks; base += SCAN_THREADS) {
-        uint32_t i = base + threadIdx.x;
-        T v = i < nblocks ? sums[i] : szero<T>();
+    for (uint64_t c0 = 0; c0 < nblocks; c0 += SCAN_TILE) {
+        const uint64_t base = c0 + (uint64_t)threadIdx.x * SCAN_ITEMS;
+        T v[SCAN_ITEMS];
+        load_items<T>(sums, base, nblocks, v);
+        T acc = szero<T>();
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; ++i)
+            acc = sadd(acc, v[i]);
         T total;
-        T ex = block_exclusive<T>(v, &total);
-        if (i < nblocks)
-            sums[i] = sadd(ex, carry);
+        T run = sadd(block_exclusive<T>(acc, &total), carry);
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; ++i) {
+            const T x = v[i];
+            v[i] = run;
+            run = sadd(run, x);
+        }
+        store_items<T>(sums, base, nblocks, v);
         carry = sadd(carry, total);
-        __syncthreads();
     }
 }
 
@@ -95,23 +162,24 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(const T *in, uint64_
                                                           unsigned long long *status,
                                                           uint64_t cap_x, uint64_t cap_y)
 {
-    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
     T vals[SCAN_ITEMS];
+    load_items<T>(in, base, n, vals);
     T acc = szero<T>();
-    for (int i = 0; i < SCAN_ITEMS; ++i) {
-        vals[i] = base + i < n ? in[base + i] : szero<T>();
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i)
         acc = sadd(acc, vals[i]);
-    }
     T total;
     T run = sadd(block_exclusive<T>(acc, &total), sums[blockIdx.x]);
+    T o[SCAN_ITEMS];
+#pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i) {
-        if (base + i < n) {
-            out[base + i] = run;
-            if (status && base + i == n - 1)
-                publish_totals(sadd(run, vals[i]), status, cap_x, cap_y);
-        }
+        o[i] = run;
+        if (status && base + i == n - 1)
+            publish_totals(sadd(run, vals[i]), status, cap_x, cap_y);
         run = sadd(run, vals[i]);
     }
+    store_items<T>(out, base, n, o);
 }
 
 template <typename T>
