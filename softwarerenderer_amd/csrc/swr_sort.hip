@@ -259,51 +259,80 @@ __global__ void k_kept_total(const uint32_t *hist, const uint32_t *hist_ex, uint
         status[ST_KEPT] = (unsigned long long)hist_ex[n - 1] + hist[n - 1];
 }
 
+// Stable scatter of one pass: RADIX_ITEMS chunks of RADIX_THREADS pairs per round, chunk j
+// element tid at base + j * RADIX_THREADS + tid.  A pair's position is the digit's running
+// offset plus the count of its digit in earlier chunks, in earlier waves of its chunk, and
+// in lower lanes of its wave (eight ballots find the lanes with the same digit).
+constexpr int RADIX_ITEMS = 4;
+
 __global__ void __launch_bounds__(RADIX_THREADS) k_radix_scatter(
     const uint32_t *keys, const uint32_t *vals, uint32_t *okeys, uint32_t *ovals,
     const unsigned long long *status, int word, int shift, uint32_t tiles, const uint32_t *hist_ex)
 {
+    constexpr int NW = RADIX_THREADS / 64;
     __shared__ uint32_t run[256];
-    __shared__ uint32_t wcnt[4][256];
+    __shared__ uint32_t wcnt[RADIX_ITEMS][NW][256];  // per (chunk, wave, digit) counts
+    __shared__ uint32_t off[RADIX_ITEMS][NW][256];   // their exclusive prefix from run
     const uint32_t G = gridDim.x, b = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     run[tid] = hist_ex[(size_t)tid * G + b];
-    for (int q = 0; q < 4; ++q)
-        wcnt[q][tid] = 0;
+#pragma unroll
+    for (int j = 0; j < RADIX_ITEMS; ++j)
+#pragma unroll
+        for (int q = 0; q < NW; ++q)
+            wcnt[j][q][tid] = 0;
     __syncthreads();
     uint64_t beg, end;
     chunk_of(status, word, G, b, beg, end);
     const uint64_t cap = pair_count(status, ST_PAIRS);
     const unsigned long long below = (1ull << lane) - 1ull;
-    for (uint64_t base = beg; base < end; base += RADIX_THREADS) {
-        uint64_t i = base + tid;
-        uint32_t k = i < end ? keys[i] : 0u;
-        bool valid = i < end && k < tiles;
-        uint32_t v = valid ? vals[i] : 0u;
-        uint32_t d = (k >> shift) & 255u;
-        unsigned long long m = __ballot(valid);
-        for (int bit = 0; bit < 8; ++bit) {
-            unsigned long long bb = __ballot((d >> bit) & 1u);
-            m &= ((d >> bit) & 1u) ? bb : ~bb;
+    for (uint64_t base = beg; base < end; base += (uint64_t)RADIX_THREADS * RADIX_ITEMS) {
+        uint32_t k[RADIX_ITEMS], v[RADIX_ITEMS], d[RADIX_ITEMS], rank[RADIX_ITEMS];
+        bool valid[RADIX_ITEMS];
+#pragma unroll
+        for (int j = 0; j < RADIX_ITEMS; ++j) {
+            const uint64_t i = base + (uint64_t)j * RADIX_THREADS + tid;
+            k[j] = i < end ? keys[i] : 0u;
+            valid[j] = i < end && k[j] < tiles;
+            v[j] = valid[j] ? vals[i] : 0u;
+            d[j] = (k[j] >> shift) & 255u;
         }
-        uint32_t rank = (uint32_t)__popcll(m & below);
-        if (valid && rank == 0)
-            wcnt[w][d] = (uint32_t)__popcll(m);
+#pragma unroll
+        for (int j = 0; j < RADIX_ITEMS; ++j) {
+            unsigned long long m = __ballot(valid[j]);
+#pragma unroll
+            for (int bit = 0; bit < 8; ++bit) {
+                const unsigned long long bb = __ballot((d[j] >> bit) & 1u);
+                m &= ((d[j] >> bit) & 1u) ? bb : ~bb;
+            }
+            rank[j] = (uint32_t)__popcll(m & below);
+            if (valid[j] && rank[j] == 0)
+                wcnt[j][w][d[j]] = (uint32_t)__popcll(m);
+        }
         __syncthreads();
-        if (valid) {
-            uint32_t pos = run[d] + rank;
-            for (int q = 0; q < w; ++q)
-                pos += wcnt[q][d];
+        {
+            uint32_t acc = run[tid];
+#pragma unroll
+            for (int j = 0; j < RADIX_ITEMS; ++j)
+#pragma unroll
+                for (int q = 0; q < NW; ++q) {
+                    off[j][q][tid] = acc;
+                    acc += wcnt[j][q][tid];
+                    wcnt[j][q][tid] = 0;
+                }
+            run[tid] = acc;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < RADIX_ITEMS; ++j) {
+            if (!valid[j])
+                continue;
+            const uint32_t pos = off[j][w][d[j]] + rank[j];
             if (pos < cap) {
-                okeys[pos] = k;
-                ovals[pos] = v;
+                okeys[pos] = k[j];
+                ovals[pos] = v[j];
             }
         }
-        __syncthreads();
-        run[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
-        for (int q = 0; q < 4; ++q)
-            wcnt[q][tid] = 0;
-        __syncthreads();
     }
 }
 
