@@ -77,10 +77,11 @@ struct DiffArgs {
     uint32_t *tflag;   // per (view, tile): bit 0 some pair of the tile's pixels is active,
                        // bit 1 some pair to the right of its last column, bit 2 some pair
                        // below its last row
-    uint32_t *queue;   // pairs that need the silhouette walk (pair index 2 * pixel + dir)
-    uint32_t *qcount;
-    uint32_t *elist;   // pixels with an active pair (the blend pass lists them)
-    uint32_t *ecount;
+    // per (view, tile) b: the pairs that need the silhouette walk (pair index 2 * pixel +
+    // dir) at queue[b * 512 ..) with qcnt[b] entries, and the pixels with an active pair (the
+    // blend pass lists them) at elist[b * 256 ..) with ecnt[b] entries
+    uint32_t *queue, *qcnt;
+    uint32_t *elist, *ecnt;
     int pairs;     // 0: evaluate, 1: evaluate and store, 2: load the stored decisions
 };
 
@@ -627,33 +628,31 @@ __device__ inline void bary_backward(const DiffArgs &d, const VertexAcc &A, int 
         acc_vertex<C>(d, A, view, B.v[m], g[m][0], g[m][1], g[m][2], &g[m][3]);
 }
 
-// Up to two values per thread appended with one global atomic per workgroup (a single
-// counter shared by every workgroup serialises its atomics); every thread of the
-// workgroup must call it.
-__device__ inline void block_append2(uint32_t *list, uint32_t *count, bool t0, uint32_t v0,
-                                     bool t1, uint32_t v1)
+// Up to two values per thread into the workgroup's own segment `seg`; thread 0 writes the
+// segment's length to *cnt.  Every thread of the workgroup calls it.  (A list shared by all
+// workgroups needs a global counter, whose atomics serialise.)
+__device__ inline void tile_append2(uint32_t *seg, uint32_t *cnt, bool t0, uint32_t v0, bool t1,
+                                    uint32_t v1)
 {
-    __shared__ uint32_t s_n, s_base;
+    __shared__ uint32_t s_n;
     const unsigned lane = threadIdx.x & 63u;
     if (threadIdx.x == 0)
         s_n = 0;
     __syncthreads();
     const unsigned long long m0 = __ballot(t0), m1 = __ballot(t1);
     const uint32_t n = (uint32_t)(__popcll(m0) + __popcll(m1));
-    uint32_t woff = 0;
+    uint32_t base = 0;
     if (lane == 0 && n)
-        woff = atomicAdd(&s_n, n);
-    woff = __shfl(woff, 0, 64);
+        base = atomicAdd(&s_n, n);
+    base = __shfl(base, 0, 64);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    if (t0)
+        seg[base + (uint32_t)__popcll(m0 & below)] = v0;
+    if (t1)
+        seg[base + (uint32_t)__popcll(m0) + (uint32_t)__popcll(m1 & below)] = v1;
     __syncthreads();
     if (threadIdx.x == 0)
-        s_base = s_n ? atomicAdd(count, s_n) : 0u;
-    __syncthreads();
-    const unsigned long long below = (1ull << lane) - 1ull;
-    const uint32_t base = s_base + woff;
-    if (t0)
-        list[base + (uint32_t)__popcll(m0 & below)] = v0;
-    if (t1)
-        list[base + (uint32_t)__popcll(m0) + (uint32_t)__popcll(m1 & below)] = v1;
+        *cnt = s_n;
 }
 
 template <int C>
@@ -678,8 +677,10 @@ __device__ inline void interp_pixel(const DiffArgs &d, const PixelIdx &pi, bool 
         d.color[i * C + c] = (float)col[c];
         d.image[i * C + c] = (float)col[c];  // the blend pass adds the antialiasing
     }
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
         d.tflag[pi.tile] = 0u;
+        d.ecnt[pi.tile] = 0u;  // the blend pass fills it for tiles with an active pair
+    }
     // this pixel's two pairs (to the right, below): decided here unless a silhouette walk
     // is needed
 #pragma unroll
@@ -701,7 +702,7 @@ __global__ void __launch_bounds__(256) k_diff_interp(DiffArgs d)
         interp_pixel<C>(d, pi, walk);
     // the pairs that need a silhouette walk, queued for k_diff_march
     const uint32_t k0 = (walk[0] || walk[1]) ? (uint32_t)(2 * pi.i) : 0u;
-    block_append2(d.queue, d.qcount, walk[0], k0, walk[1], k0 + 1u);
+    tile_append2(d.queue + (size_t)pi.tile * 512, d.qcnt + pi.tile, walk[0], k0, walk[1], k0 + 1u);
 }
 
 
@@ -777,23 +778,58 @@ __device__ inline Pair unpack_pair(const PairLds &x, bool swap, bool horiz)
     return p;
 }
 
-__global__ void k_diff_clear(DiffArgs d)
+
+
+// A workgroup's share of per-tile segments: TILES_PER_GROUP consecutive tiles, their
+// lengths scanned into s_off (s_off[k] = entries before tile k, s_off[64] = all), so the
+// workgroup's threads take the entries densely, one each.
+constexpr int TILES_PER_GROUP = 64;
+
+__device__ inline void group_segments(const uint32_t *cnt, uint32_t tiles, uint32_t t0,
+                                      uint32_t *s_off)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        *d.qcount = 0u;
-        *d.ecount = 0u;
+    if (threadIdx.x < 64) {
+        const uint32_t lane = threadIdx.x;
+        uint32_t v = t0 + lane < tiles ? cnt[t0 + lane] : 0u;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = __shfl_up(v, o, 64);
+            if ((int)lane >= o)
+                v += u;
+        }
+        s_off[lane + 1] = v;
+        if (lane == 0)
+            s_off[0] = 0;
     }
+    __syncthreads();
 }
 
+// The tile (within the group) of entry j: the largest k with s_off[k] <= j.
+__device__ inline uint32_t group_tile(const uint32_t *s_off, uint32_t j)
+{
+    uint32_t lo = 0, hi = TILES_PER_GROUP;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_off[mid] <= j)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
 
 // The queued pairs, one lane each: the silhouette walk, the stored decision, and the tile
 // flags of active pairs.
 __global__ void __launch_bounds__(256) k_diff_march(DiffArgs d)
 {
-    const uint32_t n = *d.qcount;
+    __shared__ uint32_t s_off[TILES_PER_GROUP + 1];
+    const uint32_t tiles = (uint32_t)(d.tiles_per_view * d.V), t0 = blockIdx.x * TILES_PER_GROUP;
+    group_segments(d.qcnt, tiles, t0, s_off);
+    const uint32_t n = s_off[TILES_PER_GROUP];
     const size_t HW = (size_t)d.W * d.H;
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-        const uint32_t k = d.queue[j];
+    for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
+        const uint32_t t = group_tile(s_off, j);
+        const uint32_t k = d.queue[(size_t)(t0 + t) * 512 + (j - s_off[t])];
         const size_t i = k >> 1;
         const int dir = (int)(k & 1u);
         const int view = (int)(i / HW);
@@ -928,7 +964,8 @@ __global__ void __launch_bounds__(256) k_diff_blend(DiffArgs d)
         int qx, qy;
         any = any || (neighbour(d, pi.px, pi.py, k, qx, qy) && tile_pair(s_h, s_v, k).active);
     }
-    block_append2(d.elist, d.ecount, any, any ? (uint32_t)pi.i : 0u, false, 0u);
+    const int b = xcd_order(blockIdx.x, gridDim.x);
+    tile_append2(d.elist + (size_t)b * 256, d.ecnt + b, any, any ? (uint32_t)pi.i : 0u, false, 0u);
     if (!valid)
         return;
     const size_t i = pi.i;
@@ -1190,12 +1227,16 @@ __global__ void __launch_bounds__(256) k_diff_bwd_interior(DiffArgs d)
 template <int C>
 __global__ void __launch_bounds__(256) k_diff_bwd_edges(DiffArgs d)
 {
-    const uint32_t n = *d.ecount;
+    __shared__ uint32_t s_off[TILES_PER_GROUP + 1];
+    const uint32_t tiles = (uint32_t)(d.tiles_per_view * d.V), t0 = blockIdx.x * TILES_PER_GROUP;
+    group_segments(d.ecnt, tiles, t0, s_off);
+    const uint32_t n = s_off[TILES_PER_GROUP];
     const size_t HW = (size_t)d.W * d.H;
     const VertexAcc A{nullptr, nullptr};
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
+        const uint32_t t = group_tile(s_off, j);
         PixelIdx pi;
-        pi.i = d.elist[j];
+        pi.i = d.elist[(size_t)(t0 + t) * 256 + (j - s_off[t])];
         pi.view = (int)(pi.i / HW);
         const size_t rem = pi.i - (size_t)pi.view * HW;
         pi.py = (int)(rem / (size_t)d.W);
@@ -1228,19 +1269,15 @@ static hipError_t launch_diff(const DiffArgs &d, int which, hipStream_t s)
     const uint32_t blocks = (uint32_t)((size_t)d.tiles_per_view * d.V);
     if (!blocks)
         return hipSuccess;
+    const uint32_t groups = (blocks + TILES_PER_GROUP - 1) / TILES_PER_GROUP;
     if (which == 0) {
-        const size_t pairs = 2 * (size_t)d.V * d.W * d.H;
-        const uint32_t mb = (uint32_t)((pairs + 255) / 256 < 1024 ? (pairs + 255) / 256 : 1024);
-        hipLaunchKernelGGL(k_diff_clear, dim3(1), dim3(64), 0, s, d);
         hipLaunchKernelGGL(k_diff_interp<C>, dim3(blocks), dim3(256), 0, s, d);
-        hipLaunchKernelGGL(k_diff_march, dim3(mb), dim3(256), 0, s, d);
+        hipLaunchKernelGGL(k_diff_march, dim3(groups), dim3(256), 0, s, d);
         hipLaunchKernelGGL(k_diff_blend<C>, dim3(blocks), dim3(256), 0, s, d);
     } else {
         if (d.pairs == 2) {
-            const size_t px = (size_t)d.V * d.W * d.H;
-            const uint32_t eb = (uint32_t)((px + 255) / 256 < 1024 ? (px + 255) / 256 : 1024);
             hipLaunchKernelGGL(k_diff_bwd_interior<C>, dim3(blocks), dim3(256), 0, s, d);
-            hipLaunchKernelGGL(k_diff_bwd_edges<C>, dim3(eb), dim3(256), 0, s, d);
+            hipLaunchKernelGGL(k_diff_bwd_edges<C>, dim3(groups), dim3(256), 0, s, d);
         } else {
             hipLaunchKernelGGL((k_diff_backward<C, BWD_ALL>), dim3(blocks), dim3(256), 0, s, d);
         }
@@ -1309,19 +1346,18 @@ static int diff_run(swr_context *c, const swr_diff_desc *dd, const float *gimage
     }
     // The forward stores its pair decisions; a backward over the same buffers and sizes
     // reuses them instead of walking the silhouettes again.
-    // layout: queue and list counts (256 B), flags [2 px] u8, data [2 px] int2, tile flags
-    // u32, queue [2 px] u32, list [px] u32
-    const size_t tiles = (size_t)d.tiles_per_view * d.V, off_f = 256,
+    // layout: flags [2 px] u8, data [2 px] int2, per tile: flags, queue and list lengths
+    // u32, queue [tiles][512] u32, list [tiles][256] u32
+    const size_t tiles = (size_t)d.tiles_per_view * d.V, off_f = 0,
                  off_d = (off_f + px * 2 + 15) & ~(size_t)15, off_t = off_d + px * 2 * sizeof(int2),
-                 off_q = (off_t + tiles * 4 + 15) & ~(size_t)15;
+                 off_q = (off_t + tiles * 12 + 15) & ~(size_t)15, off_e = off_q + tiles * 512 * 4;
     if (px * 2 > 0xFFFFFFFFull)
         return context_fail(c, SWR_ERR_CAPACITY, "too many pixels for one swr_diff call");
-    const size_t off_e = off_q + px * 2 * 4;
-    uint8_t *pbuf = (uint8_t *)context_zeroed_scratch(c, 1, off_e + px * 4);
+    uint8_t *pbuf = (uint8_t *)context_zeroed_scratch(c, 1, off_e + tiles * 256 * 4);
     if (!pbuf)
         return SWR_ERR_HIP;
-    d.qcount = reinterpret_cast<uint32_t *>(pbuf);
-    d.ecount = reinterpret_cast<uint32_t *>(pbuf) + 1;
+    d.qcnt = reinterpret_cast<uint32_t *>(pbuf + off_t) + tiles;
+    d.ecnt = reinterpret_cast<uint32_t *>(pbuf + off_t) + 2 * tiles;
     d.pflag = pbuf + off_f;
     d.pdata = reinterpret_cast<int2 *>(pbuf + off_d);
     d.tflag = reinterpret_cast<uint32_t *>(pbuf + off_t);
