@@ -7,9 +7,14 @@ target clear, with vertices and indices resident in HBM.  Prints one JSON line.
 The line also carries the differentiation workload (64 views at 512x512 of a 40,962-vertex
 icosphere, forward + backward of the image loss 1/2 |image - target|^2): views per second of the batched step
 (one visibility draw for all views, captured in a HIP graph) against the same work issued
-one view at a time.
+one view at a time; and the many-scene workload (256 UV spheres of 1k..200k triangles at
+256x256..1920x1080 in Span and Block mode): scenes per second of one batched pass against
+one draw per scene; and batched differentiation under load (32 fBm heightfields of 8k..2.1M
+triangles, 4 views each at 1024x1024): views per second of the batched forward + backward
+against one view at a time, and the HIP-graph Adam step.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-diff] [--copy-probe]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-diff] [--no-batch]
+                    [--no-terrain] [--copy-probe]
 """
 from __future__ import annotations
 
@@ -43,6 +48,14 @@ def parse():
     p.add_argument("--diff-views", type=int, default=64)
     p.add_argument("--diff-size", type=int, default=512)
     p.add_argument("--diff-subdiv", type=int, default=6)
+    p.add_argument("--no-batch", action="store_true", help="skip the many-scene workload")
+    p.add_argument("--batch-scenes", type=int, default=256)
+    p.add_argument("--no-terrain", action="store_true",
+                   help="skip the batched differentiation workload under load")
+    p.add_argument("--terrain-scenes", type=int, default=32)
+    p.add_argument("--terrain-size", type=int, default=1024)
+    p.add_argument("--terrain-reserve", type=float, default=8.0,
+                   help="tile-pair arena of the terrain workload, in multiples of the target")
     p.add_argument("--no-tune", action="store_true",
                    help="skip the launch-shape autotuner (built-in defaults)")
     return p.parse_args()
@@ -116,8 +129,9 @@ def diff_bench(args, dev):
 
     # the same work one view at a time: a one-view renderer, eager launches per view
     r1 = DiffRenderer(F, N, 1, S_, S_)
-    for v in range(2):
-        step(r1, mvps[v:v + 1], target[v:v + 1])
+    for v in range(V):  # synchronous draws size the arenas for every view
+        r1.forward(torch.einsum("vij,nj->vni", mvps[v:v + 1], hom).contiguous(), col, sync=True)
+    step(r1, mvps[:1], target[:1])
     torch.cuda.synchronize()
     e0.record()
     for v in range(V):
@@ -136,6 +150,249 @@ def diff_bench(args, dev):
             "batched_ms_per_step": round(t_batch, 4), "plain_ms_per_step": round(t_plain, 4),
             "batched": "one visibility draw for all views + forward + backward, HIP graph replay",
             "plain": "per view: visibility draw + forward + backward, eager launches"}
+
+
+def terrain_bench(args, dev):
+    """Batched differentiation under load: 32 fBm heightfields of 8k..2.1M triangles, 4
+    views each at 1024x1024, all in one SceneDiffRenderer.  Times (a) the forward + backward
+    step of all views from a HIP graph against the same work one view at a time (eager
+    launches), and (b) the graph-captured Adam step (lr 1e-2) on heights and colours, run
+    for the workload's 100 steps."""
+    import torch
+
+    from softwarerenderer_amd import scenes
+    from softwarerenderer_amd.diff import DiffRenderer, SceneDiffRenderer, render
+
+    views, size = 4, args.terrain_size
+    work = scenes.terrain_workload(args.terrain_scenes, seed=5, views=views, size=size)
+    sr = SceneDiffRenderer([(w["tris"], w["pos"].shape[0], views) for w in work], size, size)
+    mvps = [torch.from_numpy(w["mvps"]).float().to(dev) for w in work]
+    xz = [torch.from_numpy(w["pos"][:, [0, 2]]).float().to(dev) for w in work]
+    cuts = np.cumsum([0] + [w["pos"].shape[0] for w in work])
+    rng = np.random.default_rng(1)
+    h0 = torch.from_numpy(np.concatenate([w["pos"][:, 1] for w in work])).float().to(dev)
+    c0 = torch.from_numpy(np.concatenate([w["col"] for w in work])).float().to(dev)
+
+    def clip_scene(k, h):
+        hk = h[cuts[k]:cuts[k + 1]]
+        p = torch.stack([xz[k][:, 0], hk, xz[k][:, 1], torch.ones_like(hk)], 1)
+        return torch.matmul(mvps[k], p.t()).transpose(1, 2)
+
+    def clip_of(h):
+        return SceneDiffRenderer.clip_rows([clip_scene(k, h) for k in range(len(work))])
+
+    with torch.no_grad():
+        th = h0 + torch.from_numpy(rng.normal(0, 0.02, h0.shape)).float().to(dev)
+        tc = c0 + torch.from_numpy(rng.normal(0, 0.1, c0.shape)).float().to(dev)
+        target = sr.forward(clip_of(th), tc, sync=True).clone()
+        # headroom for the optimisation moving the terrain (captured draws cannot grow
+        # their arenas): Adam's steps roughen the heights, and tile pairs grow with the
+        # slivers; HBM is plentiful, so reserve several times the target's needs
+        st = sr.r.stats()
+        sr.reserve(primitives=int(st.input_primitives * 1.3),
+                   pairs=int(st.pairs_emitted * args.terrain_reserve))
+        sr.forward(clip_of(h0), c0, sync=True)
+    pairs_first = int(sr.r.stats().pairs_emitted)
+    sr.check()
+    s = torch.cuda.Stream(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def replay_ms(graph, reps):
+        graph.replay()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            graph.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    # (a) forward + backward of every view, HIP graph
+    clip0 = clip_of(h0).contiguous()
+
+    def fwd_bwd():
+        img = sr.forward(clip0, c0)
+        return sr.backward(img - target)
+
+    s.wait_stream(torch.cuda.current_stream(dev))
+    g_fb = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        fwd_bwd()
+        with torch.cuda.graph(g_fb, stream=s):
+            fwd_bwd()
+    torch.cuda.current_stream(dev).wait_stream(s)
+    t_fb = replay_ms(g_fb, 5)
+    sr.check()
+    del g_fb
+
+    # the same work one view at a time: a one-view renderer per scene, eager launches
+    plain = [DiffRenderer(w["tris"], w["pos"].shape[0], 1, size, size) for w in work]
+    with torch.no_grad():
+        clips = [clip_scene(k, h0).contiguous() for k in range(len(work))]
+    for k, r1 in enumerate(plain):  # synchronous draws size each renderer's arenas
+        for v in range(views):
+            r1.forward(clips[k][v:v + 1], c0[cuts[k]:cuts[k + 1]], sync=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for k, r1 in enumerate(plain):
+        ck = c0[cuts[k]:cuts[k + 1]]
+        for v in range(views):
+            img = r1.forward(clips[k][v:v + 1], ck)
+            r1.backward(img - target[k * views + v:k * views + v + 1])
+    e1.record()
+    torch.cuda.synchronize()
+    t_plain = e0.elapsed_time(e1)
+    for r1 in plain:
+        r1.check()
+        r1.close()
+
+    # (b) the Adam step, HIP graph, 100 steps
+    h = h0.clone().requires_grad_(True)
+    c = c0.clone().requires_grad_(True)
+    opt = torch.optim.Adam([h, c], lr=1e-2, capturable=True)
+
+    def adam_step():
+        opt.zero_grad(set_to_none=False)
+        img = render(sr, clip_of(h), c)
+        loss = 0.5 * ((img - target) ** 2).sum()
+        loss.backward()
+        opt.step()
+        return loss.detach()
+
+    s.wait_stream(torch.cuda.current_stream(dev))
+    with torch.cuda.stream(s):
+        first = float(adam_step())
+    torch.cuda.current_stream(dev).wait_stream(s)
+    torch.cuda.synchronize()
+    g_adam = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g_adam, stream=s):
+        out = adam_step()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(99):
+        g_adam.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    t_adam = e0.elapsed_time(e1) / 99
+    sr.check()
+    last = float(out)
+    with torch.no_grad():
+        sr.forward(clip_of(h), c, sync=True)
+    pairs_last = int(sr.r.stats().pairs_emitted)
+    V = sr.V
+    T = int(sum(w["tris"].shape[0] for w in work))
+    sr.close()
+    return {"workload": f"terrain-{len(work)}scenes-{views}views-{size}px-fwd+bwd",
+            "scenes": len(work), "views": V, "triangles": T, "size": size,
+            "vertices": int(cuts[-1]),
+            "views_per_s": round(V / (t_fb * 1e-3), 1),
+            "plain_views_per_s": round(V / (t_plain * 1e-3), 1),
+            "speedup": round(t_plain / t_fb, 2),
+            "batched_ms_per_step": round(t_fb, 3), "plain_ms_per_step": round(t_plain, 3),
+            "adam_ms_per_step": round(t_adam, 3), "adam_steps": 100,
+            "loss_first": round(first, 3), "loss_last": round(last, 3),
+            "tile_pairs_first": pairs_first, "tile_pairs_last": pairs_last,
+            "batched": "one visibility draw for all views + forward + backward, HIP graph replay",
+            "plain": "per view: visibility draw + forward + backward, eager launches",
+            "adam": "render + L2 loss + backward + Adam(lr 1e-2) on heights and colours, "
+                    "HIP graph replay"}
+
+
+def batch_bench(args, dev):
+    """The many-scene workload (256 UV spheres, 1k..200k triangles, 256x256..1920x1080,
+    Span / Block, depth-checker program): one batched pass (swr_batch_draw, enqueued
+    asynchronously) against one drawElements launch sequence per scene, same programs and
+    targets, targets cleared in every step."""
+    import torch
+
+    from softwarerenderer_amd import api, scenes
+    from softwarerenderer_amd.autotune import autotune
+
+    work = scenes.batch_workload(args.batch_scenes, seed=3)
+    px = np.concatenate([[0], np.cumsum([sc.width * sc.height for sc in work])])
+    color = torch.zeros(int(px[-1]), dtype=torch.int32, device=dev)
+    depth = torch.ones(int(px[-1]), dtype=torch.float32, device=dev)
+    verts = [torch.from_numpy(sc.vertices).to(dev) for sc in work]
+    idx = [torch.from_numpy(sc.indices).to(dev) for sc in work]
+    tg = [(color[px[i]:px[i + 1]].view(sc.height, sc.width),
+           depth[px[i]:px[i + 1]].view(sc.height, sc.width)) for i, sc in enumerate(work)]
+    modes = (api.RasterMode.Span, api.RasterMode.Block)
+    r = api.Rasterizer(0)
+    r.setStream(torch.cuda.current_stream(dev).cuda_stream)
+    vp = api.VertexProcessor(r)
+    r.setPixelShader(api.DepthCheckerShader(*tg[0]))
+    vp.setVertexShader(api.MvpVertexShader(work[0].mvp))
+    descs = [{"indices": idx[i], "vertices": verts[i], "stride": sc.stride, "mvp": sc.mvp,
+              "viewport": (0, 0, sc.width, sc.height), "scissor": (0, 0, sc.width, sc.height),
+              "cull_mode": api.CullMode.CW, "raster_mode": modes[sc.info["raster_mode"]],
+              "targets": tg[i], "size": (sc.height, sc.width)} for i, sc in enumerate(work)]
+    b = api.Batch(r, descs)
+
+    def clear():
+        color.zero_()
+        depth.fill_(1.0)
+
+    def step_batch():
+        clear()
+        b.draw(async_=True)
+
+    shaders = [api.DepthCheckerShader(*t) for t in tg]
+    vshaders = [api.MvpVertexShader(sc.mvp) for sc in work]
+
+    def step_plain(sync=False):
+        clear()
+        for i, sc in enumerate(work):
+            r.setRasterMode(modes[sc.info["raster_mode"]])
+            r.setScissorRect(0, 0, sc.width, sc.height)
+            r.setPixelShader(shaders[i])
+            vp.setViewport(0, 0, sc.width, sc.height)
+            vp.setCullMode(api.CullMode.CW)
+            vp.setVertexShader(vshaders[i])
+            vp.setVertexAttribPointer(0, sc.stride, verts[i])
+            if sync:
+                vp.drawElements(api.DrawMode.Triangle, idx[i])
+            else:
+                vp.drawElementsAsync(api.DrawMode.Triangle, idx[i])
+
+    b.draw()        # sizes the arenas for the batch
+    step_plain(True)  # and for every single scene
+    r.setPixelShader(api.DepthCheckerShader(*tg[0]))
+    tuning = None
+    if not args.no_tune:
+        tu = autotune(r, step_batch, reps=3, warmup=1, rounds=1)
+        tuning = {"chosen": tu["best"], "tuned_ms": round(tu["ms"], 4),
+                  "default_ms": round(tu["default_ms"], 4), "configs": len(tu["table"])}
+    reps = 5
+    s = torch.cuda.current_stream(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn):
+        fn()
+        r.synchronize()
+        e0.record(s)
+        for _ in range(reps):
+            fn()
+        e1.record(s)
+        r.synchronize()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    t_batch = timed(step_batch)
+    t_plain = timed(step_plain)
+    r.setTuning()
+    T = sum(sc.triangles for sc in work)
+    b.close()
+    r.close()
+    return {"workload": f"spheres-{len(work)}scenes-1k..200k-tris-256px..1080p-span/block",
+            "scenes": len(work), "triangles": T, "pixels": int(px[-1]),
+            "scenes_per_s": round(len(work) / (t_batch * 1e-3), 1),
+            "plain_scenes_per_s": round(len(work) / (t_plain * 1e-3), 1),
+            "speedup": round(t_plain / t_batch, 2),
+            "mtri_per_s": round(T / (t_batch * 1e-3) / 1e6, 1),
+            "batched_ms_per_step": round(t_batch, 4), "plain_ms_per_step": round(t_plain, 4),
+            "tuning": tuning,
+            "batched": "one swr_batch_draw of all scenes (targets cleared each step)",
+            "plain": "per scene: state + drawElementsAsync (targets cleared each step)"}
 
 
 def main():
@@ -275,6 +532,10 @@ def main():
 
     if not args.no_diff:
         out["diff"] = diff_bench(args, dev)
+    if not args.no_batch:
+        out["batch"] = batch_bench(args, dev)
+    if not args.no_terrain:
+        out["terrain"] = terrain_bench(args, dev)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_runs)
     print(json.dumps(out))

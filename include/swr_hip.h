@@ -270,7 +270,17 @@ int  swr_batch_destroy(swr_batch *batch);
  *             summed over views to grad_attributes ([vertex_count][channels]).
  * All buffers are device memory; both calls only enqueue kernels on the context stream
  * (they can be captured into a HIP graph).  The backward pass recomputes everything else
- * from ids, depth, bary and color. */
+ * from ids, depth, bary and color.
+ *
+ * Several meshes in one call (scenes > 1): scene s owns views scene_views[s] ..
+ * scene_views[s+1]-1, vertices scene_vertices[s] .. scene_vertices[s+1]-1 and triangles
+ * scene_triangles[s] .. scene_triangles[s+1]-1 (host arrays of scenes + 1 prefix sums, from
+ * 0 to views / vertex_count / triangle_count).  Triangles hold vertex numbers of their own
+ * scene in the global numbering, adjacency global triangle numbers; ids hold the scene's own
+ * triangle index + 1 (what a visibility draw of the scene's triangle list writes).  clip and
+ * grad_clip then hold, scene by scene, each of its views' rows for its own vertices only:
+ * sum over s of views(s) * vertices(s) rows of 4 floats.  The tables are uploaded when they
+ * change (a synchronous copy: not while the stream is being captured). */
 #define SWR_DIFF_CULLED 1          /* every visible triangle faces the same way (the visibility
                                       draw culled one winding): pixels of two triangles that
                                       share a vertex are never separated by a silhouette */
@@ -279,7 +289,7 @@ typedef struct swr_diff_desc {
     int32_t vertex_count, triangle_count;
     int32_t channels;              /* 1..4 */
     int32_t flags;                 /* SWR_DIFF_CULLED                                      */
-    int32_t reserved;
+    int32_t scenes;                /* 0 or 1: one mesh seen by every view; > 1: see above  */
     const float    *clip;          /* [views][vertex_count][4] clip-space x, y, z, w       */
     const int32_t  *triangles;     /* [triangle_count][3]                                  */
     const int32_t  *adjacency;     /* [triangle_count][3]: the triangle across the edge
@@ -290,6 +300,9 @@ typedef struct swr_diff_desc {
     float *bary;                   /* [views][height][width][2]: forward out, backward in  */
     float *color;                  /* [views][height][width][channels], before antialiasing */
     float *image;                  /* [views][height][width][channels]: forward out        */
+    const int32_t *scene_views;    /* scenes > 1: host [scenes + 1]                        */
+    const int32_t *scene_vertices; /* scenes > 1: host [scenes + 1]                        */
+    const int32_t *scene_triangles;/* scenes > 1: host [scenes + 1]                        */
 } swr_diff_desc;
 int  swr_diff_forward(swr_context *ctx, const swr_diff_desc *d);
 int  swr_diff_backward(swr_context *ctx, const swr_diff_desc *d, const float *grad_image,
@@ -331,6 +344,9 @@ typedef struct swr_draw_stats {
     float    stage_ms[8];        /* per-stage GPU time when profiling is enabled:   */
                                  /* 0 geometry, 1 scans, 2 emit, 3 sort, 4 ranges, 5 raster */
     uint32_t stage_launches[8];
+    uint64_t pairs_emitted;      /* pairs written by binning, before dropping the (tile, block)
+                                    pairs that cannot shade: what the pair arena must hold,
+                                    the figure to give swr_reserve                   */
 } swr_draw_stats;
 int  swr_get_draw_stats(swr_context *ctx, swr_draw_stats *out);
 /* Record HIP events around each stage (on the stream the stage runs on). */

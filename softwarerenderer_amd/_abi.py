@@ -42,7 +42,8 @@ class PixelShaderDesc(C.Structure):
 class DrawStats(C.Structure):
     _fields_ = [("input_primitives", C.c_uint64), ("raster_primitives", C.c_uint64),
                 ("tile_pairs", C.c_uint64), ("tiles", C.c_uint64),
-                ("stage_ms", C.c_float * 8), ("stage_launches", C.c_uint32 * 8)]
+                ("stage_ms", C.c_float * 8), ("stage_launches", C.c_uint32 * 8),
+                ("pairs_emitted", C.c_uint64)]
 
 
 class Scene(C.Structure):
@@ -61,11 +62,12 @@ class DiffDesc(C.Structure):
     """swr_diff_desc: differentiable shading over a visibility buffer."""
     _fields_ = [("views", C.c_int32), ("width", C.c_int32), ("height", C.c_int32),
                 ("vertex_count", C.c_int32), ("triangle_count", C.c_int32),
-                ("channels", C.c_int32), ("flags", C.c_int32), ("reserved", C.c_int32),
+                ("channels", C.c_int32), ("flags", C.c_int32), ("scenes", C.c_int32),
                 ("clip", C.c_void_p), ("triangles", C.c_void_p),
                 ("adjacency", C.c_void_p), ("attributes", C.c_void_p), ("ids", C.c_void_p),
                 ("depth", C.c_void_p), ("bary", C.c_void_p), ("color", C.c_void_p),
-                ("image", C.c_void_p)]
+                ("image", C.c_void_p), ("scene_views", C.c_void_p),
+                ("scene_vertices", C.c_void_p), ("scene_triangles", C.c_void_p)]
 
 
 class Tuning(C.Structure):

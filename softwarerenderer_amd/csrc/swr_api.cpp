@@ -76,7 +76,10 @@ struct swr_context {
     swr_draw_stats stats{};
     bool async_pending = false;
     swr_tuning tune{};
-    uint64_t diff_tag[8] = {};
+    uint64_t diff_tag[9] = {};
+    DevBuf diff_table;                        // scene table of the last multi-mesh swr_diff
+    std::vector<unsigned char> diff_table_host;
+    uint64_t diff_table_gen = 0;
 };
 
 // A texture and its mip chain on the device (include/swr_hip.h, swr_texture_create).
@@ -339,6 +342,7 @@ void collect_stats(swr_context *c)
 {
     c->stats.raster_primitives = c->status_host[ST_PRIMS];
     c->stats.tile_pairs = c->status_host[ST_KEPT];
+    c->stats.pairs_emitted = c->status_host[ST_PAIRS];
     if (c->profiling) {
         for (int i = 0; i < S_COUNT; ++i) {
             float ms = 0.0f;
@@ -787,6 +791,7 @@ int draw_lines_points(swr_context *c, int mode, const int32_t *indices, size_t c
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->stats.raster_primitives = c->status_host[ST_PRIMS];
     c->stats.tile_pairs = F;
+    c->stats.pairs_emitted = F;
     if (c->profiling && F)
         collect_stats(c);
     return SWR_OK;
@@ -807,6 +812,37 @@ int context_fail(swr_context *c, int code, const char *msg) { return fail(c, cod
 bool context_is_device_ptr(const void *p) { return is_device_ptr(p); }
 
 uint64_t *context_diff_tag(swr_context *c) { return c->diff_tag; }
+
+const void *context_diff_table(swr_context *c, const void *host, size_t bytes, uint64_t *gen)
+{
+    const unsigned char *h = static_cast<const unsigned char *>(host);
+    if (c->diff_table.p && c->diff_table_host.size() == bytes &&
+        std::memcmp(c->diff_table_host.data(), h, bytes) == 0) {
+        *gen = c->diff_table_gen;
+        return c->diff_table.p;
+    }
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(c->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+        fail(c, SWR_ERR_INVALID_ARG, "the swr_diff scene table changed while the stream is "
+                                     "being captured (run the call once before capturing)");
+        return nullptr;
+    }
+    // queued kernels may still read the old table: wait for them, then copy in stream order
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess)
+        e = ensure(c, c->diff_table, bytes);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->diff_table.p, h, bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        fail(c, SWR_ERR_HIP, "scene table upload failed: %s", hipGetErrorString(e));
+        return nullptr;
+    }
+    c->diff_table_host.assign(h, h + bytes);
+    *gen = ++c->diff_table_gen;
+    return c->diff_table.p;
+}
 
 void *context_zeroed_scratch(swr_context *c, int slot, size_t bytes)
 {
@@ -878,7 +914,7 @@ int swr_destroy(swr_context *c)
                       &c->scan_tmp, &c->prims, &c->pk_a, &c->pv_a, &c->pk_b, &c->pv_b,
                       &c->ranges, &c->hist, &c->hist_ex, &c->hist_tmp, &c->status, &c->tgt[0],
                       &c->tgt[1], &c->tgt[2], &c->log, &c->logcount, &c->bigq, &c->clipq,
-                      &c->clipq_cnt, &c->zscratch[0], &c->zscratch[1]};
+                      &c->clipq_cnt, &c->zscratch[0], &c->zscratch[1], &c->diff_table};
     for (DevBuf *b : bufs)
         if (b->p)
             (void)hipFree(b->p);

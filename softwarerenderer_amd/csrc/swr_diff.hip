@@ -46,6 +46,7 @@
 // kernels are bound by memory and atomics, not by the fp64 rate.  The discrete choices of
 // the antialiasing pass use fp32, so they are reproducible by a CPU restatement.
 #include <cstring>
+#include <vector>
 
 #include "swr_common.h"
 #include "swr_internal.h"
@@ -83,7 +84,30 @@ struct DiffArgs {
     uint32_t *queue, *qcnt;
     uint32_t *elist, *ecnt;
     int pairs;     // 0: evaluate, 1: evaluate and store, 2: load the stored decisions
+    // several meshes (null for one): per view, the clip row of global vertex g is
+    // vrow[view] + g and visibility id k names triangle vtri[view].x + k - 1 (k <= .y);
+    // per scene (S of them), its first vertex, first view and first row
+    const long long *vrow;
+    const int2 *vtri;
+    const int *svert, *sview;
+    int S;
 };
+
+// Clip / gradient row of vertex v (global number) as seen by `view`.
+__device__ inline size_t crow(const DiffArgs &d, int view, int v)
+{
+    return d.vrow ? (size_t)(d.vrow[view] + v) : (size_t)view * d.N + v;
+}
+
+// Triangle named by visibility id `id` in `view`; -1 for the background (0) or an id past
+// the view's triangles.
+__device__ inline int id_tri(const DiffArgs &d, int view, uint32_t id)
+{
+    if (!d.vtri)
+        return (id == 0 || id > (uint32_t)d.T) ? -1 : (int)id - 1;
+    const int2 t = d.vtri[view];
+    return (id == 0 || id > (uint32_t)t.y) ? -1 : t.x + (int)id - 1;
+}
 
 // Pixel of this thread: a workgroup of 256 threads covers one 16x16 tile of one view, so a
 // wave (16x4 pixels) reads its neighbours, triangles and vertices from a small area.
@@ -132,7 +156,7 @@ __device__ inline double dot3(const H3 &a, const H3 &b) { return a.x * b.x + a.y
 
 __device__ inline H3 load_h(const DiffArgs &d, int view, int v)
 {
-    const float4 c = d.clip[(size_t)view * d.N + v];
+    const float4 c = d.clip[crow(d, view, v)];
     return {(double)c.x, (double)c.y, (double)c.w};
 }
 
@@ -146,7 +170,7 @@ __device__ inline H3 pixel_ray(const DiffArgs &d, int px, int py)
 // Window position of a clip-space vertex (w > 0): fp32 for decisions, fp64 for values.
 __device__ inline float2 to_win_f(const DiffArgs &d, int view, int v)
 {
-    const float4 h = d.clip[(size_t)view * d.N + v];
+    const float4 h = d.clip[crow(d, view, v)];
     return make_float2(d.sx * (h.x / h.w) + d.sx, -d.sy * (h.y / h.w) + d.sy);
 }
 
@@ -168,7 +192,7 @@ __device__ inline double edge_d(double2 j, double2 k, double2 p)
 
 __device__ inline bool w_positive(const DiffArgs &d, int view, int v)
 {
-    return d.clip[(size_t)view * d.N + v].w > 0.0f;
+    return d.clip[crow(d, view, v)].w > 0.0f;
 }
 
 // One neighbour pair as the antialiasing pass sees it.
@@ -258,14 +282,14 @@ __device__ inline bool pair_start(const DiffArgs &d, int view, int px, int py, i
         const float dp = d.depth[ip], dq = d.depth[iq];
         pf = dp < dq || (dp == dq && ip < iq);
     }
-    int cur = (int)(pf ? a : b) - 1;
-    if (cur < 0 || cur >= d.T)
+    const int cur = id_tri(d, view, pf ? a : b);
+    if (cur < 0)
         return false;
     if (d.culled && a != 0 && b != 0) {
         // two visible triangles of a culled draw that share a vertex both face the viewer:
         // no silhouette is taken to separate their pixels
-        const int other = (int)(pf ? b : a) - 1;
-        if (other >= 0 && other < d.T) {
+        const int other = id_tri(d, view, pf ? b : a);
+        if (other >= 0) {
             const int c0 = d.tris[3 * cur], c1 = d.tris[3 * cur + 1], c2 = d.tris[3 * cur + 2];
             const int o0 = d.tris[3 * other], o1 = d.tris[3 * other + 1],
                       o2 = d.tris[3 * other + 2];
@@ -487,7 +511,7 @@ __device__ inline void acc_clip(const DiffArgs &d, const VertexAcc &A, int view,
         atomicAdd(x + 1 * ACC_COPIES, gy);
         atomicAdd(x + 2 * ACC_COPIES, gw);
     } else {
-        float *g = d.gclip + ((size_t)view * d.N + v) * 4;
+        float *g = d.gclip + crow(d, view, v) * 4;
         atomicAdd(g + 0, gx);
         atomicAdd(g + 1, gy);
         atomicAdd(g + 3, gw);
@@ -509,13 +533,13 @@ __device__ inline void acc_vertex(const DiffArgs &d, const VertexAcc &A, int vie
         for (int c = 0; c < C; ++c)
             atomicAdd(x + (3 + c) * ACC_COPIES, ga[c]);
     } else {
-        float *g = d.gclip + ((size_t)view * d.N + v) * 4;
+        float *g = d.gclip + crow(d, view, v) * 4;
         atomicAdd(g + 0, gx);
         atomicAdd(g + 1, gy);
         atomicAdd(g + 3, gw);
 #pragma unroll
         for (int c = 0; c < C; ++c)
-            atomicAdd(d.gattr + ((size_t)view * d.N + v) * C + c, ga[c]);
+            atomicAdd(d.gattr + crow(d, view, v) * C + c, ga[c]);
     }
 }
 
@@ -542,13 +566,13 @@ __device__ inline void acc_triangle(const DiffArgs &d, const VertexAcc &A, int v
             for (int k = 0; k < 3 + C; ++k)
                 atomicAdd(x + k * ACC_COPIES, g[m][k]);
         } else {
-            float *gc = d.gclip + ((size_t)view * d.N + v[m]) * 4;
+            float *gc = d.gclip + crow(d, view, v[m]) * 4;
             atomicAdd(gc + 0, g[m][0]);
             atomicAdd(gc + 1, g[m][1]);
             atomicAdd(gc + 3, g[m][2]);
 #pragma unroll
             for (int c = 0; c < C; ++c)
-                atomicAdd(d.gattr + ((size_t)view * d.N + v[m]) * C + c, g[m][3 + c]);
+                atomicAdd(d.gattr + crow(d, view, v[m]) * C + c, g[m][3 + c]);
         }
     }
 }
@@ -566,14 +590,14 @@ __device__ inline void acc_flush(const DiffArgs &d, const VertexAcc &A, int view
             const float4 q = *reinterpret_cast<const float4 *>(&A.val[(sl * ACC_VALS + k) * ACC_COPIES]);
             x[k] = (q.x + q.y) + (q.z + q.w);
         }
-        float *g = d.gclip + ((size_t)view * d.N + v) * 4;
+        float *g = d.gclip + crow(d, view, v) * 4;
         if (x[0] != 0.0f) atomicAdd(g + 0, x[0]);
         if (x[1] != 0.0f) atomicAdd(g + 1, x[1]);
         if (x[2] != 0.0f) atomicAdd(g + 3, x[2]);
 #pragma unroll
         for (int c = 0; c < C; ++c)
             if (x[3 + c] != 0.0f)
-                atomicAdd(d.gattr + ((size_t)view * d.N + v) * C + c, x[3 + c]);
+                atomicAdd(d.gattr + crow(d, view, v) * C + c, x[3 + c]);
     }
 }
 
@@ -667,7 +691,8 @@ __device__ inline void interp_pixel(const DiffArgs &d, const PixelIdx &pi, bool 
         col[c] = 0.0;
     float2 bb = make_float2(0.0f, 0.0f);
     Bary B;
-    if (id > 0 && (int)id <= d.T && bary_at(d, view, (int)id - 1, px, py, B)) {
+    const int tri = id_tri(d, view, id);
+    if (tri >= 0 && bary_at(d, view, tri, px, py, B)) {
         bary_color<C>(d, B, col);
         bb = make_float2((float)B.b[0], (float)B.b[1]);
     }
@@ -1102,7 +1127,8 @@ __device__ inline void backward_pixel(const DiffArgs &d, const VertexAcc &A, con
         return;
     const uint32_t id = d.ids[i];
     Bary B;
-    if (id == 0 || (int)id > d.T || !bary_at(d, view, (int)id - 1, px, py, B))
+    const int tri = id_tri(d, view, id);
+    if (tri < 0 || !bary_at(d, view, tri, px, py, B))
         return;
     bary_backward<C>(d, A, view, B, gc);
 }
@@ -1179,13 +1205,14 @@ __global__ void __launch_bounds__(256) k_diff_bwd_interior(DiffArgs d)
     if (valid) {
         const uint32_t id = d.ids[pi.i];
         Bary B;
-        if (id != 0 && (int)id <= d.T && bary_at(d, pi.view, (int)id - 1, pi.px, pi.py, B)) {
+        const int t = id_tri(d, pi.view, id);
+        if (t >= 0 && bary_at(d, pi.view, t, pi.px, pi.py, B)) {
             double gc[C];
 #pragma unroll
             for (int c = 0; c < C; ++c)
                 gc[c] = d.gimage[pi.i * C + c];
             bary_grads<C>(d, B, gc, g);
-            tri = (int)id - 1;
+            tri = t;
 #pragma unroll
             for (int m = 0; m < 3; ++m)
                 vid[m] = B.v[m];
@@ -1246,16 +1273,32 @@ __global__ void __launch_bounds__(256) k_diff_bwd_edges(DiffArgs d)
     }
 }
 
-// grad_attributes += sum over views of the partials; the partials are cleared for the next
-// call (they start at zero when the buffer is created).
+// grad_attributes += sum over the views that see the vertex (in view order) of the
+// partials; the partials are cleared for the next call (they start at zero when the buffer
+// is created).
 __global__ void __launch_bounds__(256) k_diff_reduce_attr(DiffArgs d, int C)
 {
     const size_t n = (size_t)d.N * C;
     for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
          k += (size_t)gridDim.x * blockDim.x) {
+        const int g = (int)(k / (size_t)C), c = (int)(k - (size_t)g * C);
+        int v0 = 0, v1 = d.V;
+        if (d.svert) {
+            // the scene owning vertex g: the last s with svert[s] <= g (svert[S] = N > g)
+            int lo = 0, hi = d.S - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (d.svert[mid] <= g)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            v0 = d.sview[lo];
+            v1 = d.sview[lo + 1];
+        }
         float s = 0.0f;
-        for (int v = 0; v < d.V; ++v) {
-            float *p = d.gattr + (size_t)v * n + k;
+        for (int v = v0; v < v1; ++v) {
+            float *p = d.gattr + crow(d, v, g) * C + c;
             s += *p;
             *p = 0.0f;
         }
@@ -1338,9 +1381,56 @@ static int diff_run(swr_context *c, const swr_diff_desc *dd, const float *gimage
     d.gclip = gclip;
     d.gattr = nullptr;
     d.gattr_out = gattr;
+    d.vrow = nullptr;
+    d.vtri = nullptr;
+    d.svert = d.sview = nullptr;
+    d.S = 1;
+    size_t rows = (size_t)d.V * d.N;  // clip / gradient rows
+    uint64_t table_gen = 0;
+    if (q.scenes > 1) {
+        const int S = q.scenes;
+        const int32_t *pv = q.scene_views, *pn = q.scene_vertices, *pt = q.scene_triangles;
+        if (!pv || !pn || !pt)
+            return context_fail(c, SWR_ERR_INVALID_ARG, "swr_diff scenes > 1 needs the scene tables");
+        if (pv[0] != 0 || pn[0] != 0 || pt[0] != 0 || pv[S] != q.views || pn[S] != q.vertex_count ||
+            pt[S] != q.triangle_count)
+            return context_fail(c, SWR_ERR_INVALID_ARG,
+                                "swr_diff scene tables must run from 0 to views / vertex_count / "
+                                "triangle_count");
+        for (int k = 0; k < S; ++k)
+            if (pv[k + 1] < pv[k] || pn[k + 1] < pn[k] || pt[k + 1] < pt[k])
+                return context_fail(c, SWR_ERR_INVALID_ARG, "swr_diff scene tables must not decrease");
+        // table: vrow [V] i64, vtri [V] int2, svert [S + 1] i32, sview [S + 1] i32
+        std::vector<unsigned char> tab((size_t)16 * d.V + (size_t)8 * (S + 1));
+        long long *vrow = reinterpret_cast<long long *>(tab.data());
+        int2 *vtri = reinterpret_cast<int2 *>(tab.data() + (size_t)8 * d.V);
+        int32_t *svert = reinterpret_cast<int32_t *>(tab.data() + (size_t)16 * d.V);
+        int32_t *sview = svert + (S + 1);
+        long long base = 0;
+        for (int k = 0; k < S; ++k) {
+            const long long nk = pn[k + 1] - pn[k];
+            for (int v = pv[k]; v < pv[k + 1]; ++v) {
+                vrow[v] = base + (long long)(v - pv[k]) * nk - pn[k];
+                vtri[v] = make_int2(pt[k], pt[k + 1] - pt[k]);
+            }
+            base += (long long)(pv[k + 1] - pv[k]) * nk;
+        }
+        std::memcpy(svert, pn, sizeof(int32_t) * (S + 1));
+        std::memcpy(sview, pv, sizeof(int32_t) * (S + 1));
+        rows = (size_t)base;
+        const unsigned char *dev =
+            static_cast<const unsigned char *>(context_diff_table(c, tab.data(), tab.size(), &table_gen));
+        if (!dev)
+            return SWR_ERR_HIP;
+        d.vrow = reinterpret_cast<const long long *>(dev);
+        d.vtri = reinterpret_cast<const int2 *>(dev + (size_t)8 * d.V);
+        d.svert = reinterpret_cast<const int *>(dev + (size_t)16 * d.V);
+        d.sview = d.svert + (S + 1);
+        d.S = S;
+    }
     if (which == 1) {
         // per-view partials: views do not contend on one vertex's attribute gradient
-        d.gattr = (float *)context_zeroed_scratch(c, 0, (size_t)d.V * d.N * q.channels * 4);
+        d.gattr = (float *)context_zeroed_scratch(c, 0, rows * q.channels * 4);
         if (!d.gattr)
             return SWR_ERR_HIP;
     }
@@ -1363,12 +1453,13 @@ static int diff_run(swr_context *c, const swr_diff_desc *dd, const float *gimage
     d.tflag = reinterpret_cast<uint32_t *>(pbuf + off_t);
     d.queue = reinterpret_cast<uint32_t *>(pbuf + off_q);
     d.elist = reinterpret_cast<uint32_t *>(pbuf + off_e);
-    const uint64_t tag[8] = {(uint64_t)(uintptr_t)q.clip,      (uint64_t)(uintptr_t)q.triangles,
+    const uint64_t tag[9] = {(uint64_t)(uintptr_t)q.clip,      (uint64_t)(uintptr_t)q.triangles,
                              (uint64_t)(uintptr_t)q.adjacency, (uint64_t)(uintptr_t)q.ids,
                              (uint64_t)(uintptr_t)q.depth,     (uint64_t)(uintptr_t)pbuf,
                              ((uint64_t)q.views << 32) | (uint32_t)q.width,
                              ((uint64_t)q.height << 40) | ((uint64_t)(q.flags & 0xFF) << 32) |
-                                 (uint32_t)q.triangle_count};
+                                 (uint32_t)q.triangle_count,
+                             q.scenes > 1 ? table_gen : 0};
     uint64_t *held = context_diff_tag(c);
     if (which == 0) {
         d.pairs = 1;

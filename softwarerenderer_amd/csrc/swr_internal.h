@@ -63,9 +63,14 @@ bool context_is_device_ptr(const void *p);
 // Device buffer `slot` (0 or 1) of at least `bytes` owned by the context, zero when first
 // handed out at this size.  nullptr (and the context's error set) on failure.
 void *context_zeroed_scratch(swr_context *c, int slot, size_t bytes);
-// Eight words the differentiation entry points keep in the context: what the last forward
+// Nine words the differentiation entry points keep in the context: what the last forward
 // stored its antialiasing decisions for.
 uint64_t *context_diff_tag(swr_context *c);
+// Device copy of a differentiation scene table (`bytes` of host data), uploaded only when
+// it differs from the last one (a synchronous, stream-ordered copy; refused while the
+// context stream is being captured).  *gen changes with every upload.  nullptr (and the
+// context's error set) on failure.
+const void *context_diff_table(swr_context *c, const void *host, size_t bytes, uint64_t *gen);
 
 // raster (swr_raster.hip)
 bool raster_layout(const swr_pixel_shader_desc &ps, ParamLayout &L);

@@ -52,18 +52,46 @@ class DiffRenderer:
     def __init__(self, triangles, num_vertices: int, views: int, width: int, height: int,
                  channels: int = 3, cull_mode: int = api.CullMode.CW,
                  raster_mode: int = api.RasterMode.Span, device: int = 0):
-        tris = np.ascontiguousarray(np.asarray(triangles, np.int32).reshape(-1, 3))
-        self.V, self.W, self.H, self.N, self.C = views, width, height, num_vertices, channels
-        self.T = tris.shape[0]
+        self._build([(triangles, num_vertices, views)], width, height, channels, cull_mode,
+                     raster_mode, device)
+        self._clip_shape = (views, num_vertices, 4)
+
+    def _build(self, scenes, width, height, channels, cull_mode, raster_mode, device):
+        """scenes: [(triangles [T_s, 3] in the scene's own vertex numbering, N_s, V_s)]."""
+        tl, ns, vs = [], [], []
+        for tris, n, v in scenes:
+            t = np.ascontiguousarray(np.asarray(tris, np.int32).reshape(-1, 3))
+            if t.size and (t.min() < 0 or t.max() >= n):
+                raise ValueError("triangle vertex numbers must lie in 0 .. vertices - 1")
+            if n < 0 or v < 0:
+                raise ValueError("negative vertex or view count")
+            tl.append(t)
+            ns.append(int(n))
+            vs.append(int(v))
+        S = len(tl)
+        pv = np.concatenate([[0], np.cumsum(vs)]).astype(np.int32)
+        pn = np.concatenate([[0], np.cumsum(ns)]).astype(np.int32)
+        pt = np.concatenate([[0], np.cumsum([t.shape[0] for t in tl])]).astype(np.int32)
+        self.V, self.W, self.H, self.C = int(pv[-1]), width, height, channels
+        self.N, self.T = int(pn[-1]), int(pt[-1])
+        self.S = S
+        self.rows = int(sum(v * n for v, n in zip(vs, ns)))
+        self._tables = [np.ascontiguousarray(a) for a in (pv, pn, pt)]
+        gtris = np.concatenate([t + pn[k] for k, t in enumerate(tl)]) if S else \
+            np.zeros((0, 3), np.int32)
+        gadj = np.concatenate([np.where(adjacency(t) >= 0, adjacency(t) + pt[k], -1)
+                               for k, t in enumerate(tl)]) if S else np.zeros((0, 3), np.int32)
         dev = torch.device("cuda", device)
         self.device = dev
-        self.tris = torch.from_numpy(tris).to(dev)
-        self.adj = torch.from_numpy(adjacency(tris)).to(dev)
-        self.clip = torch.zeros((views, num_vertices, 4), dtype=torch.float32, device=dev)
-        self.ids = torch.zeros((views, height, width), dtype=torch.int32, device=dev)
-        self.depth = torch.ones((views, height, width), dtype=torch.float32, device=dev)
-        self.bary = torch.zeros((views, height, width, 2), dtype=torch.float32, device=dev)
-        self.color = torch.zeros((views, height, width, channels), dtype=torch.float32, device=dev)
+        self.tris = torch.from_numpy(np.ascontiguousarray(gtris, np.int32)).to(dev)
+        self.adj = torch.from_numpy(np.ascontiguousarray(gadj, np.int32)).to(dev)
+        self._local = [torch.from_numpy(t.reshape(-1)).to(dev) for t in tl]
+        V = self.V
+        self.clip = torch.zeros((self.rows, 4), dtype=torch.float32, device=dev)
+        self.ids = torch.zeros((V, height, width), dtype=torch.int32, device=dev)
+        self.depth = torch.ones((V, height, width), dtype=torch.float32, device=dev)
+        self.bary = torch.zeros((V, height, width, 2), dtype=torch.float32, device=dev)
+        self.color = torch.zeros((V, height, width, channels), dtype=torch.float32, device=dev)
         self.image = torch.zeros_like(self.color)
         self.flags = 1 if cull_mode != api.CullMode.None_ else 0  # SWR_DIFF_CULLED
         self.r = api.Rasterizer(device)
@@ -71,14 +99,20 @@ class DiffRenderer:
         vp = api.VertexProcessor(self.r)
         vp.setVertexShader(api.RawVertexShader(0, 0))
         self._stream = None
-        scenes = []
-        for v in range(views):
-            scenes.append({"indices": self.tris.view(-1), "vertices": self.clip[v], "stride": 16,
-                           "viewport": (0, 0, width, height), "scissor": (0, 0, width, height),
-                           "cull_mode": cull_mode, "raster_mode": raster_mode,
-                           "targets": (self.ids[v], self.depth[v]), "size": (height, width)})
-        self.batch = api.Batch(self.r, scenes)
+        batch = []
+        row = 0
+        for k in range(S):
+            for j in range(vs[k]):
+                v = int(pv[k]) + j
+                batch.append({"indices": self._local[k], "vertices": self.clip[row:row + ns[k]],
+                              "stride": 16, "viewport": (0, 0, width, height),
+                              "scissor": (0, 0, width, height), "cull_mode": cull_mode,
+                              "raster_mode": raster_mode,
+                              "targets": (self.ids[v], self.depth[v]), "size": (height, width)})
+                row += ns[k]
+        self.batch = api.Batch(self.r, batch)
         self._generation = 0
+        self._drawn = False
         self._attr_keep: Optional[torch.Tensor] = None
         # one synchronous draw sizes the context's arenas, so later draws never allocate
         self.batch.draw()
@@ -100,17 +134,29 @@ class DiffRenderer:
         d.ids, d.depth = self.ids.data_ptr(), self.depth.data_ptr()
         d.bary, d.color, d.image = (self.bary.data_ptr(), self.color.data_ptr(),
                                     self.image.data_ptr())
+        if self.S > 1:
+            d.scenes = self.S
+            d.scene_views, d.scene_vertices, d.scene_triangles = (
+                t.ctypes.data for t in self._tables)
         return d
 
-    def forward(self, clip: torch.Tensor, attr: torch.Tensor, sync: bool = False) -> torch.Tensor:
-        """Fills the renderer's image buffer; returns it (no copy)."""
-        if clip.shape != (self.V, self.N, 4) or attr.shape != (self.N, self.C):
-            raise ValueError("clip must be [views, vertices, 4] and attr [vertices, channels]")
+    def forward(self, clip: torch.Tensor, attr: torch.Tensor,
+                sync: Optional[bool] = None) -> torch.Tensor:
+        """Fills the renderer's image buffer; returns it (no copy).  The visibility draw is
+        synchronous when `sync` is true, and by default on the first forward: a synchronous
+        draw grows the arenas to what the scene needs, an asynchronous one (which a HIP graph
+        can capture) reports SWR_ERR_CAPACITY at the next check() if they are too small."""
+        if tuple(clip.shape) != self._clip_shape or attr.shape != (self.N, self.C):
+            raise ValueError(f"clip must be {list(self._clip_shape)} and attr "
+                             f"[{self.N}, {self.C}]")
         self._use_current_stream()
-        self.clip.copy_(clip.detach())
+        self.clip.copy_(clip.detach().reshape(self.rows, 4))
         self.ids.zero_()
         self.depth.fill_(1.0)
+        if sync is None:
+            sync = not self._drawn
         self.batch.draw(async_=not sync)
+        self._drawn = True
         attr = attr.detach().contiguous().float()
         self._attr_keep = attr
         d = self._desc(attr)
@@ -119,16 +165,30 @@ class DiffRenderer:
         return self.image
 
     def backward(self, grad_image: torch.Tensor):
-        """(grad_clip [V, N, 4], grad_attr [N, C]) of the last forward."""
+        """(grad_clip shaped like the forward's clip, grad_attr [N, C]) of the last
+        forward."""
         self._use_current_stream()
         g = grad_image.detach().contiguous().float()
-        gclip = torch.zeros((self.V, self.N, 4), dtype=torch.float32, device=self.device)
+        gclip = torch.zeros(self._clip_shape, dtype=torch.float32, device=self.device)
         gattr = torch.zeros((self.N, self.C), dtype=torch.float32, device=self.device)
         d = self._desc(self._attr_keep)
         self.r._check(self.r._lib.swr_diff_backward(self.r._ctx, C.byref(d), g.data_ptr(),
                                                     gclip.data_ptr(), gattr.data_ptr()),
                       "swr_diff_backward")
         return gclip, gattr
+
+    def reserve(self, factor: float = 1.5, primitives: Optional[int] = None,
+                pairs: Optional[int] = None):
+        """Grows the arenas to `factor` times what the last synchronous visibility draw
+        needed, or to the given primitive and tile-pair counts (the next synchronous draw
+        allocates them), so that asynchronous or captured draws of changing geometry, such
+        as an optimisation moving the vertices, fit."""
+        st = self.r.stats()
+        if primitives is None:
+            primitives = int(st.raster_primitives * factor) + 1024
+        if pairs is None:
+            pairs = int(st.pairs_emitted * factor) + 4096
+        self.r.reserve(int(primitives), int(pairs))
 
     def check(self):
         """Waits for the queued work and raises if an asynchronous draw failed."""
@@ -137,6 +197,37 @@ class DiffRenderer:
     def close(self):
         self.batch.close()
         self.r.close()
+
+
+class SceneDiffRenderer(DiffRenderer):
+    """Several meshes (scenes), each seen from views of its own, rendered and differentiated
+    in one pass: ``scenes`` is a list of (triangles [T_s, 3] in the scene's own vertex
+    numbering, vertex count N_s, view count V_s).  Views are numbered scene by scene; all
+    are width x height.  Clip positions are one tensor of rows [sum V_s * N_s, 4]: scene by
+    scene, each of its views' rows for its own vertices (:meth:`clip_rows` builds it from
+    per-scene [V_s, N_s, 4] tensors); attributes are [sum N_s, C], scene by scene."""
+
+    def __init__(self, scenes, width: int, height: int, channels: int = 3,
+                 cull_mode: int = api.CullMode.CW, raster_mode: int = api.RasterMode.Span,
+                 device: int = 0):
+        self._build(list(scenes), width, height, channels, cull_mode, raster_mode, device)
+        self._clip_shape = (self.rows, 4)
+
+    @staticmethod
+    def clip_rows(per_scene) -> torch.Tensor:
+        """[sum V_s * N_s, 4] from per-scene clip tensors [V_s, N_s, 4] (differentiable)."""
+        return torch.cat([c.reshape(-1, 4) for c in per_scene], 0)
+
+    def scene_slices(self):
+        """Per scene: (view slice, vertex slice, clip-row slice)."""
+        pv, pn, _ = self._tables
+        out, row = [], 0
+        for k in range(self.S):
+            n = int((pv[k + 1] - pv[k]) * (pn[k + 1] - pn[k]))
+            out.append((slice(int(pv[k]), int(pv[k + 1])), slice(int(pn[k]), int(pn[k + 1])),
+                        slice(row, row + n)))
+            row += n
+        return out
 
 
 class _Render(torch.autograd.Function):
@@ -158,5 +249,6 @@ class _Render(torch.autograd.Function):
 
 def render(renderer: DiffRenderer, clip: torch.Tensor, attr: torch.Tensor) -> torch.Tensor:
     """Differentiable image [V, H, W, C] of the mesh under per-view clip positions
-    clip [V, N, 4] with per-vertex attributes attr [N, C]."""
+    clip [V, N, 4] (a SceneDiffRenderer: clip rows [sum V_s * N_s, 4]) with per-vertex
+    attributes attr [N, C]."""
     return _Render.apply(clip, attr, renderer)

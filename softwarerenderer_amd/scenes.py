@@ -90,18 +90,17 @@ def sphere_scene(stacks: int = 64, slices: int = 128, seed: int = 1, width: int 
     rgb = rng.random((pos.shape[0], 3))
     verts = np.concatenate([pos, rgb, uv], axis=1).astype(np.float32)
 
-    def vid(a, b):
-        return a * (slices + 1) + b
-
-    tris = []
-    for a in range(stacks):
-        for b in range(slices):
-            p00, p10, p11, p01 = vid(a, b), vid(a + 1, b), vid(a + 1, b + 1), vid(a, b + 1)
-            if a != 0:
-                tris.append((p00, p10, p01))
-            if a != stacks - 1:
-                tris.append((p10, p11, p01))
-    idx = np.asarray(tris, np.int32)
+    # per quad (stack a, slice b) in row-major order: (p00, p10, p01) unless a is the top
+    # row, then (p10, p11, p01) unless a is the bottom row
+    a = np.arange(stacks)[:, None]
+    b = np.arange(slices)[None, :]
+    p00, p10 = a * (slices + 1) + b, (a + 1) * (slices + 1) + b
+    p01, p11 = p00 + 1, p10 + 1
+    quad = np.stack([np.stack(np.broadcast_arrays(p00, p10, p01), -1),
+                     np.stack(np.broadcast_arrays(p10, p11, p01), -1)], 2)
+    keep = np.stack(np.broadcast_arrays(a != 0, a != stacks - 1), -1)
+    keep = np.broadcast_to(keep, (stacks, slices, 2))
+    idx = quad[keep].astype(np.int32).reshape(-1, 3)
     # outward-facing triangles are counter-clockwise seen from outside
     p = verts[:, :3].astype(np.float64)
     nrm = np.cross(p[idx[:, 1]] - p[idx[:, 0]], p[idx[:, 2]] - p[idx[:, 0]])
@@ -110,6 +109,30 @@ def sphere_scene(stacks: int = 64, slices: int = 128, seed: int = 1, width: int 
     mvp, _ = camera(width, height)
     return Scene(verts, idx.reshape(-1), mvp.astype(np.float32), width, height,
                  {"name": "sphere", "stacks": stacks, "slices": slices, "seed": seed})
+
+
+BATCH_SIZES = ((256, 256), (512, 512), (1024, 1024), (1920, 1080))
+
+
+def batch_workload(n: int = 256, seed: int = 3, tri_range=(1_000, 200_000)) -> list:
+    """The many-scene batching workload: ``n`` UV spheres built as :func:`sphere_scene`
+    (slices = 2 x stacks, so T = 4 stacks (stacks - 1)) with triangle counts log-uniform in
+    ``tri_range``, each at a resolution drawn uniformly from 256x256, 512x512, 1024x1024 and
+    1920x1080 (camera aspect to match) and drawn in Span or Block mode, drawn uniformly
+    (``info["raster_mode"]``: 0 Span, 1 Block).  About 9.6 M triangles and 221 M pixels in
+    all for n = 256."""
+    rng = np.random.default_rng(seed)
+    lo, hi = math.log(tri_range[0]), math.log(tri_range[1])
+    out = []
+    for i in range(n):
+        t = math.exp(rng.uniform(lo, hi))
+        stacks = max(2, int(round((1.0 + math.sqrt(1.0 + t)) / 2.0)))
+        w, h = BATCH_SIZES[int(rng.integers(len(BATCH_SIZES)))]
+        mode = int(rng.integers(2))
+        sc = sphere_scene(stacks, 2 * stacks, seed=seed * 1000 + i, width=w, height=h)
+        sc.info["raster_mode"] = mode
+        out.append(sc)
+    return out
 
 
 def soup_scene(ntri: int = 1_000_000, seed: int = 2, width: int = 1920, height: int = 1080,
@@ -393,4 +416,85 @@ def orbit_cameras(views: int, width: int, height: int, radius: float = 3.0, seed
         eye = radius * d[i]
         up = (0.0, 1.0, 0.0) if abs(d[i][1]) < 0.99 else (1.0, 0.0, 0.0)
         out[i] = proj @ lookat(eye, (0, 0, 0), up)
+    return out
+
+
+def fbm(nx: int, ny: int, seed: int, octaves: int = 6, persistence: float = 0.5,
+        base: int = 4) -> np.ndarray:
+    """Fractal value noise [ny, nx] over the unit square, zero-mean per octave: octave o
+    interpolates (smoothstep weights) a seeded lattice of (base 2^o + 1)^2 uniform values
+    and is weighted persistence^o."""
+    rng = np.random.default_rng(seed)
+    x = np.linspace(0.0, 1.0, nx)
+    y = np.linspace(0.0, 1.0, ny)
+    h = np.zeros((ny, nx))
+    amp = 1.0
+    for o in range(octaves):
+        f = base << o
+        lat = rng.random((f + 1, f + 1)) - 0.5
+        gx, gy = x * f, y * f
+        ix = np.minimum(gx.astype(np.int64), f - 1)
+        iy = np.minimum(gy.astype(np.int64), f - 1)
+        tx, ty = gx - ix, gy - iy
+        tx, ty = tx * tx * (3 - 2 * tx), ty * ty * (3 - 2 * ty)
+        top = lat[iy][:, ix] * (1 - tx) + lat[iy][:, ix + 1] * tx
+        bot = lat[iy + 1][:, ix] * (1 - tx) + lat[iy + 1][:, ix + 1] * tx
+        h += amp * (top * (1 - ty)[:, None] + bot * ty[:, None])
+        amp *= persistence
+    return h
+
+
+def heightfield(cells: int, seed: int, octaves: int = 6, persistence: float = 0.5,
+                relief: float = 1.2):
+    """fBm terrain over [-1, 1]^2 in x-z with y up: (positions [N, 3] float64, triangles
+    [T, 3] int32, colours [N, 3] float64).  cells x cells quads, (cells + 1)^2 vertices,
+    2 cells^2 triangles, counter-clockwise seen from above; colour follows height."""
+    n = cells + 1
+    h = relief * fbm(n, n, seed, octaves, persistence)
+    zs, xs = np.meshgrid(np.linspace(-1.0, 1.0, n), np.linspace(-1.0, 1.0, n), indexing="ij")
+    pos = np.stack([xs, h, zs], -1).reshape(-1, 3)
+    i = np.arange(cells)[:, None]
+    j = np.arange(cells)[None, :]
+    v00 = i * n + j
+    v01, v10 = v00 + 1, v00 + n
+    v11 = v10 + 1
+    tris = np.stack([np.stack(np.broadcast_arrays(v00, v10, v11), -1),
+                     np.stack(np.broadcast_arrays(v00, v11, v01), -1)], 2).reshape(-1, 3)
+    t = (h.reshape(-1) - h.min()) / max(float(np.ptp(h)), 1e-12)
+    rng = np.random.default_rng(seed + 7)
+    col = np.stack([0.2 + 0.6 * t, 0.5 + 0.3 * (1 - t), 0.3 * np.ones_like(t)], -1)
+    col = np.clip(col + rng.normal(0, 0.05, col.shape), 0.0, 1.0)
+    return pos, tris.astype(np.int32), col
+
+
+def terrain_cameras(views: int, width: int, height: int, seed: int = 0, radius: float = 2.6,
+                    elevation=(20.0, 35.0)):
+    """MVPs [views, 4, 4] (float64) of cameras around a terrain at the origin: azimuths
+    evenly spaced from a seeded start, elevations uniform in `elevation` degrees, looking at
+    the origin; 60-degree fovy, near 0.1, far 10."""
+    rng = np.random.default_rng(seed)
+    a0 = rng.uniform(0, 2 * np.pi)
+    proj = perspective(60.0, width / height, 0.1, 10.0)
+    out = np.zeros((views, 4, 4))
+    for k in range(views):
+        az = a0 + 2 * np.pi * k / views
+        el = np.radians(rng.uniform(*elevation))
+        eye = radius * np.array([np.cos(el) * np.cos(az), np.sin(el), np.cos(el) * np.sin(az)])
+        out[k] = proj @ lookat(eye, (0, 0, 0), (0, 1, 0))
+    return out
+
+
+def terrain_workload(n: int = 32, seed: int = 5, cells_range=(64, 1024), views: int = 4,
+                     size: int = 1024) -> list:
+    """The batched-differentiation workload: n fBm heightfields (6 octaves, persistence 0.5)
+    with cells per side log-uniform in `cells_range` (8k to 2.1M triangles), each seen by
+    `views` cameras at size x size.  Per scene: {"cells", "pos", "tris", "col", "mvps"}."""
+    rng = np.random.default_rng(seed)
+    lo, hi = math.log(cells_range[0]), math.log(cells_range[1])
+    out = []
+    for k in range(n):
+        cells = int(round(math.exp(rng.uniform(lo, hi))))
+        pos, tris, col = heightfield(cells, seed=seed * 1000 + k)
+        out.append({"cells": cells, "pos": pos, "tris": tris, "col": col,
+                    "mvps": terrain_cameras(views, size, size, seed=seed * 1000 + k)})
     return out

@@ -43,6 +43,9 @@ def _c_layout(code: str) -> dict:
 @pytest.mark.parametrize("cname,py", [("swr_pixel_shader_desc", _abi.PixelShaderDesc),
                                       ("swr_vertex_shader_desc", _abi.VertexShaderDesc),
                                       ("swr_draw_stats", _abi.DrawStats),
+                                      ("swr_diff_desc", _abi.DiffDesc),
+                                      ("swr_scene", _abi.Scene),
+                                      ("swr_tuning", _abi.Tuning),
                                       ("swr_pixel_shader_desc", O.PixelShaderDesc),
                                       ("swr_vertex_shader_desc", O.VertexShaderDesc),
                                       ("swr_pipeline_state", O.PipelineState)])

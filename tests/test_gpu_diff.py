@@ -26,7 +26,7 @@ def backward_unstored(r, g):
     ids2, depth2 = r.ids.clone(), r.depth.clone()
     d = r._desc(r._attr_keep)
     d.ids, d.depth = ids2.data_ptr(), depth2.data_ptr()
-    gclip = torch.zeros((r.V, r.N, 4), dtype=torch.float32, device=r.device)
+    gclip = torch.zeros(r._clip_shape, dtype=torch.float32, device=r.device)
     gattr = torch.zeros((r.N, r.C), dtype=torch.float32, device=r.device)
     r.r._check(r.r._lib.swr_diff_backward(r.r._ctx, C.byref(d), g.data_ptr(), gclip.data_ptr(),
                                           gattr.data_ptr()), "swr_diff_backward")
@@ -230,3 +230,140 @@ def test_graph_replay_matches_eager(gpu):
         assert torch.allclose(out["gattr"], gattr, rtol=1e-4, atol=1e-6 * gattr.abs().max().item())
         out["img"].zero_()  # eager work between replays
     r.close()
+
+
+def _terrain_case(cells, views, size, seed):
+    """A heightfield seen by `views` terrain cameras: (positions, triangles, clip, colours)."""
+    pos, F, col = S.heightfield(cells, seed=seed)
+    mvps = S.terrain_cameras(views, size, size, seed=seed)
+    hom = np.concatenate([pos, np.ones((pos.shape[0], 1))], axis=1)
+    clip = np.einsum("vij,nj->vni", mvps, hom).astype(np.float32)
+    return pos, F, clip, col.astype(np.float32)
+
+
+def test_scene_renderer_matches_per_scene(gpu):
+    """Several meshes in one pass (SceneDiffRenderer: one visibility batch, one forward and
+    one backward call for all) against each mesh on a renderer of its own (same images and
+    visibility, bit for bit) and against the FP64 reference per mesh (gradients 1e-4)."""
+    import torch
+    from softwarerenderer_amd.diff import DiffRenderer, SceneDiffRenderer
+    size = 64
+    cases = [make_case(views=2, size=size, subdiv=2, seed=1)[1:],
+             _terrain_case(12, 3, size, seed=3)[1:],
+             make_case(views=1, size=size, subdiv=3, seed=2)[1:],
+             _terrain_case(20, 2, size, seed=4)[1:]]
+    sr = SceneDiffRenderer([(F, c.shape[1], c.shape[0]) for F, c, _ in cases], size, size)
+    clip = SceneDiffRenderer.clip_rows([torch.from_numpy(c).cuda() for _, c, _ in cases])
+    attr = torch.from_numpy(np.concatenate([a for _, _, a in cases])).cuda()
+    img = sr.forward(clip, attr, sync=True).clone()
+    rng = np.random.default_rng(5)
+    g = torch.from_numpy(rng.normal(size=tuple(img.shape)).astype(np.float32)).cuda()
+    gclip, gattr = sr.backward(g)
+    unstored = backward_unstored(sr, g)
+    sr.check()
+    for k, ((F, c, a), (vs, ns, rs)) in enumerate(zip(cases, sr.scene_slices())):
+        r = DiffRenderer(F, c.shape[1], c.shape[0], size, size)
+        one = r.forward(torch.from_numpy(c).cuda(), torch.from_numpy(a).cuda(), sync=True)
+        assert torch.equal(one, img[vs]), f"scene {k}: image"
+        assert torch.equal(r.ids, sr.ids[vs]), f"scene {k}: visibility"
+        ids = r.ids.cpu().numpy().view(np.uint32)
+        depth = r.depth.cpu().numpy()
+        adj = r.adj.cpu().numpy()
+        gk = g[vs].cpu().numpy()
+        ref_img, ref_gclip, ref_gattr = O.diff_reference(c, F, adj, a, ids, depth, gk)
+        assert np.abs(img[vs].cpu().numpy() - ref_img).max() <= 1e-5
+        for gc_, ga_ in ((gclip.cpu().numpy(), gattr.cpu().numpy()), unstored):
+            gc_k = gc_[rs].reshape(c.shape)
+            assert rel_err(ga_[ns], ref_gattr) <= 1e-4, f"scene {k}: attributes"
+            assert rel_err(gc_k, ref_gclip) <= 1e-4, f"scene {k}: clip"
+        r.close()
+    sr.close()
+
+
+def test_terrain_adam_graph(gpu):
+    """A small instance of the batched-differentiation workload: four heightfields of uneven
+    size, four views each, heights and colours fitted with Adam (lr 1e-2) to the images of
+    perturbed terrains by a HIP-graph-captured step (render, L2 loss, backward, Adam).  The
+    graph's steps track eager steps from the same start, and the loss falls."""
+    import torch
+    from softwarerenderer_amd.diff import SceneDiffRenderer, render
+    size, views = 96, 4
+    work = S.terrain_workload(4, seed=9, cells_range=(10, 40), views=views, size=size)
+    sr = SceneDiffRenderer([(w["tris"], w["pos"].shape[0], views) for w in work], size, size)
+    mvps = [torch.from_numpy(w["mvps"]).float().cuda() for w in work]
+    xz = [torch.from_numpy(w["pos"][:, [0, 2]]).float().cuda() for w in work]
+    rng = np.random.default_rng(3)
+    h0 = torch.from_numpy(np.concatenate([w["pos"][:, 1] for w in work])).float().cuda()
+    c0 = torch.from_numpy(np.concatenate([w["col"] for w in work])).float().cuda()
+    cuts = np.cumsum([0] + [w["pos"].shape[0] for w in work])
+
+    def clip_of(h):
+        out = []
+        for k in range(len(work)):
+            hk = h[cuts[k]:cuts[k + 1]]
+            p = torch.stack([xz[k][:, 0], hk, xz[k][:, 1], torch.ones_like(hk)], 1)
+            out.append(torch.matmul(mvps[k], p.t()).transpose(1, 2))
+        return SceneDiffRenderer.clip_rows(out)
+
+    with torch.no_grad():
+        th = h0 + torch.from_numpy(rng.normal(0, 0.02, h0.shape)).float().cuda()
+        tc = c0 + torch.from_numpy(rng.normal(0, 0.1, c0.shape)).float().cuda()
+        target = render(sr, clip_of(th), tc).clone()
+        sr.reserve(6.0)
+        sr.forward(clip_of(h0), c0, sync=True)
+
+    def make():
+        h = h0.clone().requires_grad_(True)
+        c = c0.clone().requires_grad_(True)
+        return h, c, torch.optim.Adam([h, c], lr=1e-2, capturable=True)
+
+    def step(h, c, opt):
+        opt.zero_grad(set_to_none=False)
+        img = render(sr, clip_of(h), c)
+        loss = 0.5 * ((img - target) ** 2).sum()
+        loss.backward()
+        opt.step()
+        return loss.detach()
+
+    # eager reference
+    h, c, opt = make()
+    eager = [float(step(h, c, opt)) for _ in range(5)]
+    sr.check()
+    h_e, c_e = h.detach().clone(), c.detach().clone()
+    # captured: one warm-up step on the side stream, then reset and capture
+    h, c, opt = make()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        step(h, c, opt)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    with torch.no_grad():
+        h.copy_(h0)
+        c.copy_(c0)
+    for st in opt.state.values():
+        for v in st.values():
+            v.zero_()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=s):
+        out = step(h, c, opt)
+    torch.cuda.synchronize()
+    losses = []
+    for _ in range(5):
+        graph.replay()
+        torch.cuda.synchronize()
+        losses.append(float(out))
+    sr.check()
+    # float atomics add in another order, and Adam turns a gradient that is zero up to
+    # rounding into a full step of either sign: losses agree to rounding, parameters
+    # everywhere but at such points
+    assert np.allclose(losses, eager, rtol=1e-3), (losses, eager)
+    for x, y in ((h, h_e), (c, c_e)):
+        assert float(((x - y).abs() <= 1e-5).float().mean()) >= 0.99
+    for _ in range(95):
+        graph.replay()
+    torch.cuda.synchronize()
+    sr.check()
+    final = float(out)
+    assert final < 0.5 * eager[0], (eager[0], final)
+    sr.close()
