@@ -1249,17 +1249,32 @@ __global__ void __launch_bounds__(256) k_diff_bwd_interior(DiffArgs d)
 }
 
 // What the antialiasing adds to the gradients, for the pixels the blend pass listed (one
-// lane each; the decisions come from the forward's store, gradients go straight to global
-// memory: the list is short).
+// lane each; the decisions come from the forward's store).  Vertex gradients are summed in
+// an LDS table keyed by vertex when the group's tiles lie in one view (silhouette pixels of
+// a ridge share its vertices), else they go straight to global memory.
 template <int C>
 __global__ void __launch_bounds__(256) k_diff_bwd_edges(DiffArgs d)
 {
     __shared__ uint32_t s_off[TILES_PER_GROUP + 1];
+    __shared__ int s_key[ACC_SLOTS];
+    __shared__ __align__(16) float s_val[ACC_SLOTS * ACC_VALS * ACC_COPIES];
     const uint32_t tiles = (uint32_t)(d.tiles_per_view * d.V), t0 = blockIdx.x * TILES_PER_GROUP;
     group_segments(d.ecnt, tiles, t0, s_off);
     const uint32_t n = s_off[TILES_PER_GROUP];
+    if (n == 0)
+        return;  // n is the same for every thread of the group
+    const uint32_t t_last = (t0 + TILES_PER_GROUP < tiles ? t0 + TILES_PER_GROUP : tiles) - 1;
+    const int view0 = (int)(t0 / (uint32_t)d.tiles_per_view);
+    const bool one_view = (uint32_t)view0 == t_last / (uint32_t)d.tiles_per_view;
+    if (one_view) {
+        for (int sl = threadIdx.x; sl < ACC_SLOTS; sl += blockDim.x)
+            s_key[sl] = -1;
+        for (int k = threadIdx.x; k < ACC_SLOTS * ACC_VALS * ACC_COPIES / 4; k += blockDim.x)
+            reinterpret_cast<float4 *>(s_val)[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        __syncthreads();
+    }
     const size_t HW = (size_t)d.W * d.H;
-    const VertexAcc A{nullptr, nullptr};
+    const VertexAcc A = one_view ? VertexAcc{s_key, s_val} : VertexAcc{nullptr, nullptr};
     for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
         const uint32_t t = group_tile(s_off, j);
         PixelIdx pi;
@@ -1270,6 +1285,10 @@ __global__ void __launch_bounds__(256) k_diff_bwd_edges(DiffArgs d)
         pi.px = (int)(rem - (size_t)pi.py * d.W);
         pi.tile = 0;
         backward_pixel<C, BWD_EDGES>(d, A, pi, StoredPairs(d, pi.i, pi.px, pi.py));
+    }
+    if (one_view) {
+        __syncthreads();
+        acc_flush<C>(d, A, view0);
     }
 }
 
