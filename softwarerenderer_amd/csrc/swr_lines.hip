@@ -313,6 +313,10 @@ struct VertexPixel {
     __device__ float w() const { return wv; }
     __device__ float avar(int k) const { return av[k]; }
     __device__ float pvar(int k, float) const { return pv[k]; }
+    __device__ float w_raw() const { return wv; }
+    __device__ float p_raw(int k) const { return pv[k]; }
+    __device__ static float w_of(float raw) { return raw; }
+    __device__ static float pvar_of(float raw, float) { return raw; }
 };
 
 // One wave64 per 8x8 tile, lane = pixel: the tile's fragments, 64 at a time, in order.
@@ -369,7 +373,7 @@ __global__ void __launch_bounds__(256) k_lines_raster(RasterArgs a)
         }
     }
     if (inside)
-        Prog::store(st, a, o);
+        Prog::template store<VertexPixel<L>>(st, a, o);
 }
 
 // ---- launchers ------------------------------------------------------------------------------

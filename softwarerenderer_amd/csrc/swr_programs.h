@@ -37,6 +37,12 @@ struct ChainPixel {
     __device__ float w() const { return 1.0f / v[L::IW]; }
     __device__ float avar(int k) const { return v[L::A + k]; }
     __device__ float pvar(int k, float w) const { return v[L::P + k] * w; }
+    // the same values from the raw interpolants, for programs that keep a fragment's inputs
+    // and shade it later: w() == w_of(w_raw()), pvar(k, w) == pvar_of(p_raw(k), w)
+    __device__ float w_raw() const { return v[L::IW]; }
+    __device__ float p_raw(int k) const { return v[L::P + k]; }
+    __device__ static float w_of(float raw) { return 1.0f / raw; }
+    __device__ static float pvar_of(float raw, float w) { return raw * w; }
 };
 
 // ---- pixel programs (include/swr_hip.h) ---------------------------------------------------
@@ -51,6 +57,7 @@ struct ProgCoverage {
     {
         s.hit = true;
     }
+    template <class PX = void>
     __device__ static void store(const State &s, const RasterArgs &a, size_t o)
     {
         if (s.hit)
@@ -74,6 +81,7 @@ struct ProgRGB {
         s.c = (r << 16) | (g << 8) | b;
         s.hit = true;
     }
+    template <class PX = void>
     __device__ static void store(const State &s, const RasterArgs &a, size_t o)
     {
         if (s.hit)
@@ -81,40 +89,49 @@ struct ProgRGB {
     }
 };
 
+// Only the last depth-passing fragment of a pixel reaches the targets, and its colour is a
+// pure function of its interpolants: shade keeps that fragment's raw values, store computes
+// the colour once (the same operations on the same values as shading it on the spot).
 template <class L>
 struct ProgDepthChecker {
     static constexpr int KEY_WORD = REC_KEY;
     static constexpr bool PLANES = false;
-    struct State { float depth; uint32_t color; bool hit; };
+    struct State { float depth, wr, p0, p1, a0, a1, a2; bool hit; };
     __device__ static void load(State &s, const RasterArgs &a, size_t o)
     {
         s.depth = static_cast<const float *>(a.t1)[o];
-        s.color = 0;
+        s.wr = s.p0 = s.p1 = s.a0 = s.a1 = s.a2 = 0.0f;
         s.hit = false;
     }
     template <class PX>
-    __device__ static void shade(State &s, const RasterArgs &a, const PX &v, uint32_t, int, int)
+    __device__ static void shade(State &s, const RasterArgs &, const PX &v, uint32_t, int, int)
     {
         const float z = v.z();
         if (z < s.depth) {
             s.depth = z;
-            const float w = v.w();  // only the depth-passing pixels read pvar
-            const float u = v.pvar(0, w), t = v.pvar(1, w);
-            int cu = f2i(floorf(u * a.checker_cells));
-            int cv = f2i(floorf(t * a.checker_cells));
-            float k = (((uint32_t)cu + (uint32_t)cv) & 1u) ? 1.0f : 0.5f;
-            uint32_t r = (uint32_t)(clamp01(v.avar(0) * k) * 255.0f);
-            uint32_t g = (uint32_t)(clamp01(v.avar(1) * k) * 255.0f);
-            uint32_t b = (uint32_t)(clamp01(v.avar(2) * k) * 255.0f);
-            s.color = r | (g << 8) | (b << 16) | 0xFF000000u;
+            s.wr = v.w_raw();
+            s.p0 = v.p_raw(0);
+            s.p1 = v.p_raw(1);
+            s.a0 = v.avar(0);
+            s.a1 = v.avar(1);
+            s.a2 = v.avar(2);
             s.hit = true;
         }
     }
+    template <class PX>
     __device__ static void store(const State &s, const RasterArgs &a, size_t o)
     {
         if (s.hit) {
+            const float w = PX::w_of(s.wr);  // only the depth-passing pixels read pvar
+            const float u = PX::pvar_of(s.p0, w), t = PX::pvar_of(s.p1, w);
+            int cu = f2i(floorf(u * a.checker_cells));
+            int cv = f2i(floorf(t * a.checker_cells));
+            float k = (((uint32_t)cu + (uint32_t)cv) & 1u) ? 1.0f : 0.5f;
+            uint32_t r = (uint32_t)(clamp01(s.a0 * k) * 255.0f);
+            uint32_t g = (uint32_t)(clamp01(s.a1 * k) * 255.0f);
+            uint32_t b = (uint32_t)(clamp01(s.a2 * k) * 255.0f);
             static_cast<float *>(a.t1)[o] = s.depth;
-            static_cast<uint32_t *>(a.t0)[o] = s.color;
+            static_cast<uint32_t *>(a.t0)[o] = r | (g << 8) | (b << 16) | 0xFF000000u;
         }
     }
 };
@@ -152,6 +169,7 @@ struct ProgHash {
         s.h = h;
         s.n += 1;
     }
+    template <class PX = void>
     __device__ static void store(const State &s, const RasterArgs &a, size_t o)
     {
         if (s.n != s.n0) {
@@ -195,6 +213,7 @@ struct ProgLog {
                     (k < L::NP && k < a.np_set) ? fbits(v.pvar(k < L::NP ? k : 0, w)) : 0u;
         }
     }
+    template <class PX = void>
     __device__ static void store(const State &s, const RasterArgs &a, size_t o)
     {
         if (s.n != s.n0)
@@ -226,6 +245,7 @@ struct ProgVisibility {
             s.hit = true;
         }
     }
+    template <class PX = void>
     __device__ static void store(const State &s, const RasterArgs &a, size_t o)
     {
         if (s.hit) {
@@ -411,6 +431,7 @@ struct ProgTexture {
         s.c = tex_sample(*a.tex, u, t, dudx, dvdx, dudy, dvdy);
         s.hit = true;
     }
+    template <class PX = void>
     __device__ static void store(const State &s, const RasterArgs &a, size_t o)
     {
         if (s.hit)

@@ -224,7 +224,7 @@ __global__ void __launch_bounds__(512) k_raster(RasterArgs a0)
         lds_sync();
     }
     if (inside)
-        Prog::store(st, a, o);
+        Prog::template store<ChainPixel<L>>(st, a, o);
 }
 
 template <template <class> class ProgT, int NZ, int NW, int NA, int NP, int GSX = 0>
