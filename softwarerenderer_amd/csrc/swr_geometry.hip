@@ -478,21 +478,60 @@ __device__ __forceinline__ void fan_triangle(const GeomArgs &a, FanCtx &c, int k
     write_prim<NV>(a, pos, key, c.src, c.sid, g, w0, w1, w2, x0, v1, x2, pb);
 }
 
-// Clipped triangles: polygon nodes live in per-thread arrays (rare path, kept out of line).
+// Threads per workgroup of the clip-queue kernels.
+constexpr int CLIPQ_THREADS = 64;
+
+// Polygon nodes of the general clipper.  The walk indexes them with run-time values, which
+// per-thread arrays turn into scratch memory, a dependent global round trip per access, and
+// the few queued triangles that take this path are the queue kernels' long pole.  With few
+// varyings they live in LDS instead, one column per thread (node-major, so the lanes of a
+// wave touch consecutive words); wide varyings keep per-thread arrays.
+template <int NV, bool EMIT, bool LDS = (NV <= 8)>
+struct ClipNodes {
+    Pos4 P[MAX_NODES];
+    float V[EMIT ? MAX_NODES : 1][NV];
+    uint8_t poly[2][MAX_CLIP];
+    __device__ Pos4 &p(int i) { return P[i]; }
+    __device__ float &v(int i, int k) { return V[EMIT ? i : 0][k]; }
+    __device__ uint8_t &g(int b, int i) { return poly[b][i]; }
+};
+
+template <int NV, bool EMIT>
+struct ClipNodes<NV, EMIT, true> {
+    __device__ static Pos4 (&sp())[MAX_NODES][CLIPQ_THREADS]
+    {
+        __shared__ Pos4 s[MAX_NODES][CLIPQ_THREADS];
+        return s;
+    }
+    __device__ static float (&sv())[EMIT ? MAX_NODES : 1][NV > 0 ? NV : 1][CLIPQ_THREADS]
+    {
+        __shared__ float s[EMIT ? MAX_NODES : 1][NV > 0 ? NV : 1][CLIPQ_THREADS];
+        return s;
+    }
+    __device__ static uint8_t (&sg())[2][MAX_CLIP][CLIPQ_THREADS]
+    {
+        __shared__ uint8_t s[2][MAX_CLIP][CLIPQ_THREADS];
+        return s;
+    }
+    __device__ Pos4 &p(int i) { return sp()[i][threadIdx.x]; }
+    __device__ float &v(int i, int k) { return sv()[EMIT ? i : 0][k][threadIdx.x]; }
+    __device__ uint8_t &g(int b, int i) { return sg()[b][i][threadIdx.x]; }
+};
+
+// Clipped triangles that neither early-out nor fit the register paths (rare path, kept
+// out of line).  Called only from the clip-queue kernels (CLIPQ_THREADS threads each).
 template <int NV, bool EMIT>
 __device__ inline void clip_path(const GeomArgs &a, FanCtx &c, int cm, const Pos4 *in3,
                                        const float (*var3)[NV])
 {
-    Pos4 P[MAX_NODES];
-    float var[EMIT ? MAX_NODES : 1][NV];
-    uint8_t poly_a[MAX_CLIP], poly_b[MAX_CLIP];
-    uint8_t *poly = poly_a, *other = poly_b;
+    ClipNodes<NV, EMIT> N;
+    int cur = 0;  // node lists: g(cur, .) is the polygon, g(cur ^ 1, .) the next one
     for (int q = 0; q < 3; ++q) {
-        P[q] = in3[q];
+        N.p(q) = in3[q];
         if (EMIT)
             for (int k = 0; k < NV; ++k)
-                var[q][k] = var3[q][k];
-        poly[q] = (uint8_t)q;
+                N.v(q, k) = var3[q][k];
+        N.g(cur, q) = (uint8_t)q;
     }
     int n = 3, nn = 3;
     // VertexProcessor.cpp:233-250 with PolyClipper::clipToPlane (PolyClipper.cpp:45-81)
@@ -500,31 +539,34 @@ __device__ inline void clip_path(const GeomArgs &a, FanCtx &c, int cm, const Pos
         if (!(cm & (1 << p)) || n < 3)
             continue;
         int m = 0;
-        int iprev = poly[0];
-        float dprev = plane_dot(p, P[iprev]);
+        const int nxt = cur ^ 1;
+        int iprev = N.g(cur, 0);
+        float dprev = plane_dot(p, N.p(iprev));
         for (int i = 1; i <= n; ++i) {
-            int ic = (i == n) ? poly[0] : poly[i];
-            float dp = plane_dot(p, P[ic]);
+            int ic = (i == n) ? N.g(cur, 0) : N.g(cur, i);
+            float dp = plane_dot(p, N.p(ic));
             if (dprev >= 0) {
                 if (m >= MAX_CLIP) { m = -1; break; }
-                other[m++] = (uint8_t)iprev;
+                N.g(nxt, m++) = (uint8_t)iprev;
             }
             if (sgnf(dp) != sgnf(dprev)) {
                 if (m >= MAX_CLIP || nn >= MAX_NODES) { m = -1; break; }
                 float tt = dp < 0 ? dprev / (dprev - dp) : -dprev / (dp - dprev);
-                const Pos4 A = P[iprev], B = P[ic];
-                P[nn].x = lerp1(A.x, B.x, tt);
-                P[nn].y = lerp1(A.y, B.y, tt);
-                P[nn].z = lerp1(A.z, B.z, tt);
-                P[nn].w = lerp1(A.w, B.w, tt);
+                const Pos4 A = N.p(iprev), B = N.p(ic);
+                Pos4 R;
+                R.x = lerp1(A.x, B.x, tt);
+                R.y = lerp1(A.y, B.y, tt);
+                R.z = lerp1(A.z, B.z, tt);
+                R.w = lerp1(A.w, B.w, tt);
+                N.p(nn) = R;
                 if (EMIT) {
                     // Helper::interpolateVertex lerps the vertex shader's counts only
                     for (int k = 0; k < NV; ++k) {
                         bool lerped = k < a.lay.na ? (k < a.vs_na) : ((k - a.lay.na) < a.vs_np);
-                        var[nn][k] = lerped ? lerp1(var[iprev][k], var[ic][k], tt) : 0.0f;
+                        N.v(nn, k) = lerped ? lerp1(N.v(iprev, k), N.v(ic, k), tt) : 0.0f;
                     }
                 }
-                other[m++] = (uint8_t)nn++;
+                N.g(nxt, m++) = (uint8_t)nn++;
             }
             iprev = ic;
             dprev = dp;
@@ -534,18 +576,21 @@ __device__ inline void clip_path(const GeomArgs &a, FanCtx &c, int cm, const Pos
             n = 0;
             break;
         }
-        uint8_t *s = poly;
-        poly = other;
-        other = s;
+        cur = nxt;
         n = m;
     }
     if (!EMIT)
         a.app[c.t] = n > 3 ? (uint32_t)(n - 3) : 0u;
     for (int k = 0; k + 2 < n; ++k) {
-        int ia = poly[0], ib = poly[k + 1], ic = poly[k + 2];
-        fan_triangle<NV, EMIT>(a, c, k, to_window(a, P[ia]), to_window(a, P[ib]),
-                               to_window(a, P[ic]), var[EMIT ? ia : 0], var[EMIT ? ib : 0],
-                               var[EMIT ? ic : 0], true);
+        const int ia = N.g(cur, 0), ib = N.g(cur, k + 1), ic = N.g(cur, k + 2);
+        float va[NV > 0 ? NV : 1], vb[NV > 0 ? NV : 1], vc[NV > 0 ? NV : 1];
+        for (int q = 0; q < NV; ++q) {
+            va[q] = EMIT ? N.v(ia, q) : 0.0f;
+            vb[q] = EMIT ? N.v(ib, q) : 0.0f;
+            vc[q] = EMIT ? N.v(ic, q) : 0.0f;
+        }
+        fan_triangle<NV, EMIT>(a, c, k, to_window(a, N.p(ia)), to_window(a, N.p(ib)),
+                               to_window(a, N.p(ic)), va, vb, vc, true);
     }
 }
 
@@ -1073,13 +1118,13 @@ __device__ void run_clip_queue(const GeomArgs &a)
     }
 }
 
-__global__ void __launch_bounds__(256) k_clip_count(GeomArgs a)
+__global__ void __launch_bounds__(CLIPQ_THREADS) k_clip_count(GeomArgs a)
 {
     run_clip_queue<0, false>(a);
 }
 
 template <int NV>
-__global__ void __launch_bounds__(256) k_clip_emit(GeomArgs a)
+__global__ void __launch_bounds__(CLIPQ_THREADS) k_clip_emit(GeomArgs a)
 {
     if (a.status[ST_OVERFLOW] | a.status[ST_ERROR])
         return;
@@ -1087,8 +1132,13 @@ __global__ void __launch_bounds__(256) k_clip_emit(GeomArgs a)
 }
 
 // ---- host launchers -----------------------------------------------------------------------
-// queue kernels: enough threads for a tenth of the triangles, at least one block
-#define CLIP_BLOCKS(tris) ((uint32_t)(((tris) / 10 + 255) / 256 + 1))
+// queue kernels: enough threads for a tenth of the triangles (they loop beyond), at least
+// one block, at most 2048
+static uint32_t clip_blocks(uint64_t tris)
+{
+    const uint64_t b = (tris / 10 + CLIPQ_THREADS - 1) / CLIPQ_THREADS + 1;
+    return (uint32_t)(b < 2048 ? b : 2048);
+}
 hipError_t launch_begin(const GeomArgs &a, hipStream_t s)
 {
     uint32_t work = 2 * a.tiles > a.batches * 1024u ? 2 * a.tiles : a.batches * 1024u;
@@ -1105,7 +1155,7 @@ hipError_t launch_geom_count(const GeomArgs &a, hipStream_t s)
         return hipSuccess;
     hipLaunchKernelGGL(k_geom_count, dim3((a.tris + 255) / 256), dim3(256), 0, s, a);
     if (!a.window_input)
-        hipLaunchKernelGGL(k_clip_count, dim3(CLIP_BLOCKS(a.tris)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_clip_count, dim3(clip_blocks(a.tris)), dim3(CLIPQ_THREADS), 0, s, a);
     return hipGetLastError();
 }
 
@@ -1145,9 +1195,9 @@ int emit_class(int nvary)
 template <int NV>
 static void emit_kernels(const GeomArgs &a, hipStream_t s, hipStream_t side, bool q)
 {
-    dim3 g((a.tris + 255) / 256), blk(256), gq(CLIP_BLOCKS(a.tris));
+    dim3 g((a.tris + 255) / 256), blk(256), gq(clip_blocks(a.tris)), bq(CLIPQ_THREADS);
     if (q)
-        hipLaunchKernelGGL(k_clip_emit<NV == 0 ? 1 : NV>, gq, blk, 0, side ? side : s, a);
+        hipLaunchKernelGGL(k_clip_emit<NV == 0 ? 1 : NV>, gq, bq, 0, side ? side : s, a);
     hipLaunchKernelGGL(k_geom_emit<NV>, g, blk, 0, s, a);
 }
 
