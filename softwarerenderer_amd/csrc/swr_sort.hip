@@ -142,8 +142,15 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_sums(T *sums, uint32_t nb
     }
 }
 
-__device__ inline void publish_totals(uint32_t, unsigned long long *, uint64_t, uint64_t) {}
-__device__ inline void publish_totals(uint2 t, unsigned long long *status, uint64_t cap_x,
+// The total of the scan, published by the thread that owns the last element: a u32 scan
+// writes it to status[word]; the (prims, pairs) scan writes both totals and flags arena
+// overflow against the capacities.
+__device__ inline void publish_totals(uint32_t t, unsigned long long *status, int word, uint64_t,
+                                      uint64_t)
+{
+    status[word] = t;
+}
+__device__ inline void publish_totals(uint2 t, unsigned long long *status, int, uint64_t cap_x,
                                       uint64_t cap_y)
 {
     status[ST_PRIMS] = t.x;
@@ -154,14 +161,30 @@ __device__ inline void publish_totals(uint2 t, unsigned long long *status, uint6
     status[ST_OVERFLOW] = of;
 }
 
-// Writes the exclusive scan.  With `status` set (the (prims, pairs) scan) the thread that
-// owns the last element also publishes the totals and flags arena overflow.
+// Sum of sums[0 .. b) over the block (the carry into block b when no pass has scanned the
+// block sums): b / SCAN_THREADS loads per thread, for scans of few blocks.
 template <typename T>
+__device__ T block_carry(const T *sums, uint32_t b)
+{
+    T acc = szero<T>();
+    for (uint32_t k = threadIdx.x; k < b; k += SCAN_THREADS)
+        acc = sadd(acc, sums[k]);
+    T total;
+    block_exclusive<T>(acc, &total);
+    __syncthreads();  // block_exclusive's LDS is used again by the caller
+    return total;
+}
+
+// Writes the exclusive scan.  CARRY: the block adds up the block sums before it (the
+// two-kernel scan); else sums[] holds their exclusive scan.  With `status` set the thread
+// that owns the last element publishes the total (publish_totals).
+template <typename T, bool CARRY>
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(const T *in, uint64_t n,
                                                           const T *sums, T *out,
-                                                          unsigned long long *status,
+                                                          unsigned long long *status, int word,
                                                           uint64_t cap_x, uint64_t cap_y)
 {
+    const T carry = CARRY ? block_carry<T>(sums, blockIdx.x) : sums[blockIdx.x];
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
     T vals[SCAN_ITEMS];
     load_items<T>(in, base, n, vals);
@@ -170,29 +193,38 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(const T *in, uint64_
     for (int i = 0; i < SCAN_ITEMS; ++i)
         acc = sadd(acc, vals[i]);
     T total;
-    T run = sadd(block_exclusive<T>(acc, &total), sums[blockIdx.x]);
+    T run = sadd(block_exclusive<T>(acc, &total), carry);
     T o[SCAN_ITEMS];
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i) {
         o[i] = run;
         if (status && base + i == n - 1)
-            publish_totals(sadd(run, vals[i]), status, cap_x, cap_y);
+            publish_totals(sadd(run, vals[i]), status, word, cap_x, cap_y);
         run = sadd(run, vals[i]);
     }
     store_items<T>(out, base, n, o);
 }
 
+// Scans of up to this many blocks skip the pass over the block sums: each block of the
+// down pass adds up the sums before it (at most SCAN_CARRY_BLOCKS / SCAN_THREADS loads).
+constexpr uint32_t SCAN_CARRY_BLOCKS = 1024;
+
 template <typename T>
 hipError_t scan_impl(const T *in, T *out, uint64_t n, T *tmp, unsigned long long *status,
-                     uint64_t cap_x, uint64_t cap_y, hipStream_t s)
+                     int word, uint64_t cap_x, uint64_t cap_y, hipStream_t s)
 {
     if (!n)
         return hipSuccess;
     uint32_t nb = (uint32_t)((n + SCAN_TILE - 1) / SCAN_TILE);
     hipLaunchKernelGGL(k_scan_reduce<T>, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, tmp);
-    hipLaunchKernelGGL(k_scan_sums<T>, dim3(1), dim3(SCAN_THREADS), 0, s, tmp, nb);
-    hipLaunchKernelGGL(k_scan_down<T>, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, tmp, out, status,
-                       cap_x, cap_y);
+    if (nb <= SCAN_CARRY_BLOCKS) {
+        hipLaunchKernelGGL((k_scan_down<T, true>), dim3(nb), dim3(SCAN_THREADS), 0, s, in, n,
+                           tmp, out, status, word, cap_x, cap_y);
+    } else {
+        hipLaunchKernelGGL(k_scan_sums<T>, dim3(1), dim3(SCAN_THREADS), 0, s, tmp, nb);
+        hipLaunchKernelGGL((k_scan_down<T, false>), dim3(nb), dim3(SCAN_THREADS), 0, s, in, n,
+                           tmp, out, status, word, cap_x, cap_y);
+    }
     return hipGetLastError();
 }
 
@@ -200,14 +232,14 @@ uint64_t scan_tmp_elems(uint64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1
 
 hipError_t scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *tmp, hipStream_t s)
 {
-    return scan_impl<uint32_t>(in, out, n, tmp, nullptr, 0, 0, s);
+    return scan_impl<uint32_t>(in, out, n, tmp, nullptr, 0, 0, 0, s);
 }
 
 hipError_t scan_pairs(const uint2 *in, uint2 *out, uint64_t n, uint2 *tmp,
                       unsigned long long *status, uint64_t cap_prims, uint64_t cap_pairs,
                       hipStream_t s)
 {
-    return scan_impl<uint2>(in, out, n, tmp, status, cap_prims, cap_pairs, s);
+    return scan_impl<uint2>(in, out, n, tmp, status, 0, cap_prims, cap_pairs, s);
 }
 
 // ---- radix sort of (tile, primitive) pairs ---------------------------------------------
@@ -249,14 +281,6 @@ __global__ void __launch_bounds__(RADIX_THREADS) k_radix_hist(const uint32_t *ke
     }
     __syncthreads();
     hist[(size_t)threadIdx.x * G + b] = h[threadIdx.x];
-}
-
-// Number of kept pairs: the exclusive scan of the pass-0 histogram plus its last count.
-__global__ void k_kept_total(const uint32_t *hist, const uint32_t *hist_ex, uint32_t n,
-                             unsigned long long *status)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0)
-        status[ST_KEPT] = (unsigned long long)hist_ex[n - 1] + hist[n - 1];
 }
 
 // Stable scatter of one pass: RADIX_ITEMS chunks of RADIX_THREADS pairs per round, chunk j
@@ -374,12 +398,12 @@ hipError_t sort_pairs(SortBuffers &sb, const unsigned long long *status, uint32_
         const int word = p == 0 ? ST_PAIRS : ST_KEPT;
         hipLaunchKernelGGL(k_radix_hist, dim3(sb.blocks), dim3(RADIX_THREADS), 0, s, ka, status,
                            word, shift, tiles, sb.hist);
-        hipError_t e = scan_u32(sb.hist, sb.hist_ex, nh, sb.scan_tmp, s);
+        // pass 0's histogram total is the number of kept pairs (ST_KEPT)
+        hipError_t e = scan_impl<uint32_t>(sb.hist, sb.hist_ex, nh, sb.scan_tmp,
+                                           p == 0 ? (unsigned long long *)status : nullptr,
+                                           ST_KEPT, 0, 0, s);
         if (e != hipSuccess)
             return e;
-        if (p == 0)
-            hipLaunchKernelGGL(k_kept_total, dim3(1), dim3(64), 0, s, sb.hist, sb.hist_ex, nh,
-                               (unsigned long long *)status);
         hipLaunchKernelGGL(k_radix_scatter, dim3(sb.blocks), dim3(RADIX_THREADS), 0, s, ka, va, kb,
                            vb, status, word, shift, tiles, sb.hist_ex);
         uint32_t *t = ka;
