@@ -1133,13 +1133,12 @@ __device__ inline void backward_pixel(const DiffArgs &d, const VertexAcc &A, con
     bary_backward<C>(d, A, view, B, gc);
 }
 
-// Gradients of one 16x16 tile: pair decisions in LDS, vertex gradients accumulated in LDS
-// and flushed once per vertex.
-template <int C, int PART>
+// The whole backward of one 16x16 tile, for a backward whose buffers the last forward did
+// not store decisions for: pair decisions evaluated into LDS, vertex gradients accumulated
+// in LDS and flushed once per vertex.
+template <int C>
 __global__ void __launch_bounds__(256) k_diff_backward(DiffArgs d)
 {
-    if (PART == BWD_EDGES && !tile_has_pairs(d, xcd_order(blockIdx.x, gridDim.x)))
-        return;  // no silhouette in this tile (uniform over the workgroup)
     __shared__ int s_key[ACC_SLOTS];
     __shared__ __align__(16) float s_val[ACC_SLOTS * ACC_VALS * ACC_COPIES];
     __shared__ PairLds s_h[TP_H], s_v[TP_V];
@@ -1149,12 +1148,11 @@ __global__ void __launch_bounds__(256) k_diff_backward(DiffArgs d)
         reinterpret_cast<float4 *>(s_val)[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     PixelIdx pi;
     const bool valid = pixel_of_thread(d, pi);
-    if (PART != BWD_INTERIOR)
-        tile_pairs<PART == BWD_EDGES>(d, pi, valid, s_h, s_v);
+    tile_pairs<false>(d, pi, valid, s_h, s_v);
     __syncthreads();
     const VertexAcc A{s_key, s_val};
     if (valid)  // every thread reaches the barrier below
-        backward_pixel<C, PART>(d, A, pi, TilePairs{s_h, s_v});
+        backward_pixel<C, BWD_ALL>(d, A, pi, TilePairs{s_h, s_v});
     __syncthreads();
     acc_flush<C>(d, A, pi.view);
 }
@@ -1341,7 +1339,7 @@ static hipError_t launch_diff(const DiffArgs &d, int which, hipStream_t s)
             hipLaunchKernelGGL(k_diff_bwd_interior<C>, dim3(blocks), dim3(256), 0, s, d);
             hipLaunchKernelGGL(k_diff_bwd_edges<C>, dim3(groups), dim3(256), 0, s, d);
         } else {
-            hipLaunchKernelGGL((k_diff_backward<C, BWD_ALL>), dim3(blocks), dim3(256), 0, s, d);
+            hipLaunchKernelGGL(k_diff_backward<C>, dim3(blocks), dim3(256), 0, s, d);
         }
         const size_t n = (size_t)d.N * C;
         const uint32_t rb = (uint32_t)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096);

@@ -367,3 +367,36 @@ def test_terrain_adam_graph(gpu):
     final = float(out)
     assert final < 0.5 * eager[0], (eager[0], final)
     sr.close()
+
+
+def test_scene_tables_rejected(gpu):
+    """Scene tables that do not run from 0 to the totals, that decrease, or that are missing
+    are refused before any kernel runs (SWR_ERR_INVALID_ARG); a good call afterwards still
+    gives the same image."""
+    import ctypes as C
+    import torch
+    from softwarerenderer_amd.diff import SceneDiffRenderer
+    size = 32
+    cases = [make_case(views=1, size=size, subdiv=1, seed=1)[1:],
+             make_case(views=2, size=size, subdiv=1, seed=2)[1:]]
+    sr = SceneDiffRenderer([(F, c.shape[1], c.shape[0]) for F, c, _ in cases], size, size)
+    clip = SceneDiffRenderer.clip_rows([torch.from_numpy(c).cuda() for _, c, _ in cases])
+    attr = torch.from_numpy(np.concatenate([a for _, _, a in cases])).cuda()
+    img = sr.forward(clip, attr, sync=True).clone()
+    good = [t.copy() for t in sr._tables]
+    assert [t.tolist() for t in good] == [[0, 1, 3], [0, 42, 84], [0, 80, 160]]
+    bad = [(0, [1, 1, 3]), (0, [0, 1, 2]), (1, [0, 90, 84]), (2, [0, 80, 159]),
+           (0, [0, 4, 3])]
+    for k, t in bad:
+        tabs = [x.copy() for x in good]
+        tabs[k] = np.asarray(t, np.int32)
+        d = sr._desc(sr._attr_keep)
+        d.scene_views, d.scene_vertices, d.scene_triangles = (x.ctypes.data for x in tabs)
+        assert sr.r._lib.swr_diff_forward(sr.r._ctx, C.byref(d)) == -1, (k, t)
+    d = sr._desc(sr._attr_keep)
+    d.scene_triangles = None
+    assert sr.r._lib.swr_diff_forward(sr.r._ctx, C.byref(d)) == -1
+    again = sr.forward(clip, attr, sync=True)
+    sr.check()
+    assert torch.equal(again, img)
+    sr.close()
