@@ -282,6 +282,14 @@ def log2_rounded(x: float) -> float:
     return float(lib.swr_oracle_log2(x))
 
 
+def set_log2_libm(on: bool) -> None:
+    """Level of detail by the C library's log2f (the reference's call) when on, else by
+    log2 rounded once (the device's choice, the default); process-wide."""
+    lib = C.CDLL(ORACLE_SO)
+    lib.swr_oracle_set_log2_libm.argtypes = [C.c_int]
+    lib.swr_oracle_set_log2_libm(int(bool(on)))
+
+
 def ref_obj_available() -> bool:
     return os.path.exists(REF_OBJ_SO)
 

@@ -345,10 +345,17 @@ static uint32_t tex_lerp(uint32_t c1, uint32_t c2, float t)
     return tex_pack(out[0], out[1], out[2]);
 }
 
+/* Which log2 the level of detail uses: 0 (default) log2_rounded, as the device does; 1 the
+ * C library's log2f, as the reference's std::log2 does (swr_oracle_set_log2_libm).  Tests
+ * use mode 1 only to measure how far the device's choice moves the images. */
+static int g_log2_libm = 0;
+void swr_oracle_set_log2_libm(int on) { g_log2_libm = on != 0; }
+
 /* Texture::sampleTrilinear, Texture.h:121-146 */
 static uint32_t tex_trilinear(const swr_oracle_texture *t, float u, float v, float rho)
 {
-    float lod = log2_rounded(rho < 1e-6f ? 1e-6f : rho);
+    const float r = rho < 1e-6f ? 1e-6f : rho;
+    float lod = g_log2_libm ? log2f(r) : log2_rounded(r);
     lod = clamp_f(lod, 0.0f, (float)(t->levels - 1));
     int base = f2i(floorf(lod));
     int next = min_i(base + 1, t->levels - 1);

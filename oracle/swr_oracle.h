@@ -68,6 +68,9 @@ void swr_oracle_texture_destroy(swr_oracle_texture *t);
 uint32_t swr_oracle_texture_sample(const swr_oracle_texture *t, float u, float v, float dudx,
                                    float dvdx, float dudy, float dvdy);
 float swr_oracle_log2(float x);
+/* 1: the sampler's level of detail uses the C library's log2f (the reference's call) instead
+ * of log2 rounded once (the device's); process-wide, for measuring the difference. */
+void swr_oracle_set_log2_libm(int on);
 int swr_oracle_texture_level(const swr_oracle_texture *t, int level, const uint32_t **texels,
                              int *width, int *height);
 
