@@ -19,6 +19,16 @@
  *   - Block mode with PVarCount > 0 and InterpolateW == false chains 1/w as in Span mode
  *     (the reference reads an uninitialised invw there, PixelShaderBase.h:122-136);
  *   - negative indices are rejected (the reference reads before the attribute buffer).
+ *
+ * What pins each part (tests/): the triangle, line and point paths against the reference
+ * compiled from its own sources (oracle/_ref) and the golden fixtures it wrote; OBJ loading
+ * against the reference's ObjData.cpp; the texture sampler (Texture.h needs SDL, so the
+ * reference's class cannot be built) by closed forms, and its level-of-detail log2 against
+ * float64 and against the C library's log2f on textured scenes.  Parity unpinned: the
+ * varyings of Block / Adaptive draws with PVarCount > 0 and InterpolateW == false (the
+ * reference reads uninitialised memory there; coverage is pinned), and textured pixels
+ * whose trilinear blend lands on a rounding boundary where log2f and the log2 rounded once
+ * disagree (none on the test scenes).
  */
 #include "swr_oracle.h"
 
