@@ -468,7 +468,9 @@ __device__ inline void bary_color(const DiffArgs &d, const Bary &B, double *col)
 // ACC_PROBES probes goes straight to global memory.
 constexpr int ACC_SLOTS = 256;   // power of two
 constexpr int ACC_PROBES = 32;
-constexpr int ACC_VALS = 8;      // clip x, y, w, attributes 0..4
+constexpr int ACC_VALS = 8;      // at most: clip x, y, w, attributes 0..4
+// values per slot with C attribute channels: clip x, y, w and the C attributes
+template <int C> constexpr int acc_vals() { return 3 + C; }
 // Each value is kept in four copies, picked by the pixel's (x, y) parity: the pixels of one
 // triangle are neighbours, and four copies cut the lanes of a wave that add into the same
 // LDS word (which serialise) about four times.  The flush sums the copies.
@@ -501,12 +503,13 @@ __device__ inline int acc_slot(const VertexAcc &A, int v)
     return -1;
 }
 
+template <int C>
 __device__ inline void acc_clip(const DiffArgs &d, const VertexAcc &A, int view, int v, float gx,
                                 float gy, float gw)
 {
     const int sl = acc_slot(A, v);
     if (sl >= 0) {
-        float *x = &A.val[sl * ACC_VALS * ACC_COPIES + acc_copy()];
+        float *x = &A.val[sl * acc_vals<C>() * ACC_COPIES + acc_copy()];
         atomicAdd(x + 0 * ACC_COPIES, gx);
         atomicAdd(x + 1 * ACC_COPIES, gy);
         atomicAdd(x + 2 * ACC_COPIES, gw);
@@ -525,7 +528,7 @@ __device__ inline void acc_vertex(const DiffArgs &d, const VertexAcc &A, int vie
 {
     const int sl = acc_slot(A, v);
     if (sl >= 0) {
-        float *x = &A.val[sl * ACC_VALS * ACC_COPIES + acc_copy()];
+        float *x = &A.val[sl * acc_vals<C>() * ACC_COPIES + acc_copy()];
         atomicAdd(x + 0 * ACC_COPIES, gx);
         atomicAdd(x + 1 * ACC_COPIES, gy);
         atomicAdd(x + 2 * ACC_COPIES, gw);
@@ -561,7 +564,7 @@ __device__ inline void acc_triangle(const DiffArgs &d, const VertexAcc &A, int v
         if (sl[m] < 0)
             sl[m] = acc_slot(A, v[m]);
         if (sl[m] >= 0) {
-            float *x = &A.val[sl[m] * ACC_VALS * ACC_COPIES + acc_copy()];
+            float *x = &A.val[sl[m] * acc_vals<C>() * ACC_COPIES + acc_copy()];
 #pragma unroll
             for (int k = 0; k < 3 + C; ++k)
                 atomicAdd(x + k * ACC_COPIES, g[m][k]);
@@ -586,8 +589,8 @@ __device__ inline void acc_flush(const DiffArgs &d, const VertexAcc &A, int view
             continue;
         float x[ACC_VALS];
 #pragma unroll
-        for (int k = 0; k < ACC_VALS; ++k) {
-            const float4 q = *reinterpret_cast<const float4 *>(&A.val[(sl * ACC_VALS + k) * ACC_COPIES]);
+        for (int k = 0; k < acc_vals<C>(); ++k) {
+            const float4 q = *reinterpret_cast<const float4 *>(&A.val[(sl * acc_vals<C>() + k) * ACC_COPIES]);
             x[k] = (q.x + q.y) + (q.z + q.w);
         }
         float *g = d.gclip + crow(d, view, v) * 4;
@@ -1031,12 +1034,13 @@ __global__ void __launch_bounds__(256) k_diff_blend(DiffArgs d)
 }
 
 // d(window x, y) of a clip-space vertex -> d(clip x, y, w), accumulated.
+template <int C>
 __device__ inline void scatter_window_grad(const DiffArgs &d, const VertexAcc &A, int view, int v,
                                            double gx, double gy)
 {
     const H3 h = load_h(d, view, v);
     const double iw = 1.0 / h.w, sx = 0.5 * d.W, sy = 0.5 * d.H;
-    acc_clip(d, A, view, v, (float)(gx * sx * iw), (float)(-gy * sy * iw),
+    acc_clip<C>(d, A, view, v, (float)(gx * sx * iw), (float)(-gy * sy * iw),
              (float)((-gx * sx * h.x + gy * sy * h.y) * iw * iw));
 }
 
@@ -1116,9 +1120,9 @@ __device__ inline void backward_pixel(const DiffArgs &d, const VertexAcc &A, con
         }
         touched = true;
         const double dldt = pv.a * (t_low ? -dot : dot) + dedt, dlda = wt * dot;
-        scatter_window_grad(d, A, view, pr.vj, dldt * pv.tj.x + dlda * pv.aj.x,
+        scatter_window_grad<C>(d, A, view, pr.vj, dldt * pv.tj.x + dlda * pv.aj.x,
                             dldt * pv.tj.y + dlda * pv.aj.y);
-        scatter_window_grad(d, A, view, pr.vk, dldt * pv.tk.x + dlda * pv.ak.x,
+        scatter_window_grad<C>(d, A, view, pr.vk, dldt * pv.tk.x + dlda * pv.ak.x,
                             dldt * pv.tk.y + dlda * pv.ak.y);
     }
 
@@ -1140,11 +1144,11 @@ template <int C>
 __global__ void __launch_bounds__(256) k_diff_backward(DiffArgs d)
 {
     __shared__ int s_key[ACC_SLOTS];
-    __shared__ __align__(16) float s_val[ACC_SLOTS * ACC_VALS * ACC_COPIES];
+    __shared__ __align__(16) float s_val[ACC_SLOTS * acc_vals<C>() * ACC_COPIES];
     __shared__ PairLds s_h[TP_H], s_v[TP_V];
     for (int sl = threadIdx.x; sl < ACC_SLOTS; sl += blockDim.x)
         s_key[sl] = -1;
-    for (int k = threadIdx.x; k < ACC_SLOTS * ACC_VALS * ACC_COPIES / 4; k += blockDim.x)
+    for (int k = threadIdx.x; k < ACC_SLOTS * acc_vals<C>() * ACC_COPIES / 4; k += blockDim.x)
         reinterpret_cast<float4 *>(s_val)[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     PixelIdx pi;
     const bool valid = pixel_of_thread(d, pi);
@@ -1186,10 +1190,10 @@ __global__ void __launch_bounds__(256) k_diff_bwd_interior(DiffArgs d)
 {
     constexpr int NV = 3 + C;
     __shared__ int s_key[ACC_SLOTS];
-    __shared__ __align__(16) float s_val[ACC_SLOTS * ACC_VALS * ACC_COPIES];
+    __shared__ __align__(16) float s_val[ACC_SLOTS * acc_vals<C>() * ACC_COPIES];
     for (int sl = threadIdx.x; sl < ACC_SLOTS; sl += blockDim.x)
         s_key[sl] = -1;
-    for (int k = threadIdx.x; k < ACC_SLOTS * ACC_VALS * ACC_COPIES / 4; k += blockDim.x)
+    for (int k = threadIdx.x; k < ACC_SLOTS * acc_vals<C>() * ACC_COPIES / 4; k += blockDim.x)
         reinterpret_cast<float4 *>(s_val)[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     PixelIdx pi;
     const bool valid = pixel_of_thread(d, pi);
@@ -1255,7 +1259,7 @@ __global__ void __launch_bounds__(256) k_diff_bwd_edges(DiffArgs d)
 {
     __shared__ uint32_t s_off[TILES_PER_GROUP + 1];
     __shared__ int s_key[ACC_SLOTS];
-    __shared__ __align__(16) float s_val[ACC_SLOTS * ACC_VALS * ACC_COPIES];
+    __shared__ __align__(16) float s_val[ACC_SLOTS * acc_vals<C>() * ACC_COPIES];
     const uint32_t tiles = (uint32_t)(d.tiles_per_view * d.V), t0 = blockIdx.x * TILES_PER_GROUP;
     group_segments(d.ecnt, tiles, t0, s_off);
     const uint32_t n = s_off[TILES_PER_GROUP];
@@ -1267,7 +1271,7 @@ __global__ void __launch_bounds__(256) k_diff_bwd_edges(DiffArgs d)
     if (one_view) {
         for (int sl = threadIdx.x; sl < ACC_SLOTS; sl += blockDim.x)
             s_key[sl] = -1;
-        for (int k = threadIdx.x; k < ACC_SLOTS * ACC_VALS * ACC_COPIES / 4; k += blockDim.x)
+        for (int k = threadIdx.x; k < ACC_SLOTS * acc_vals<C>() * ACC_COPIES / 4; k += blockDim.x)
             reinterpret_cast<float4 *>(s_val)[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         __syncthreads();
     }
