@@ -317,6 +317,8 @@ struct VertexPixel {
     __device__ float p_raw(int k) const { return pv[k]; }
     __device__ static float w_of(float raw) { return raw; }
     __device__ static float pvar_of(float raw, float) { return raw; }
+    static constexpr bool REPLAY = false;  // no planes: programs keep the values
+    const float *rec = nullptr;
 };
 
 // One wave64 per 8x8 tile, lane = pixel: the tile's fragments, 64 at a time, in order.

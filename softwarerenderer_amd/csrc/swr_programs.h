@@ -43,7 +43,36 @@ struct ChainPixel {
     __device__ float p_raw(int k) const { return v[L::P + k]; }
     __device__ static float w_of(float raw) { return 1.0f / raw; }
     __device__ static float pvar_of(float raw, float w) { return raw * w; }
+    // a fragment's planes can be chained again at its pixel (chain_at)
+    static constexpr bool REPLAY = true;
 };
+
+// Plane p of a primitive record at pixel (x, y) by the chain the raster kernel runs, in the
+// same operations: Block (PixelShaderBase.h:55-94): init at the 8x8 block's first pixel
+// centre, +b per row, +a per column; Span (PixelShaderBase.h:96-112): init at the span's
+// first pixel xl of row y, +a per pixel after it.  (x, y) must be a pixel the primitive
+// covers (for Span, xl <= x).
+__device__ inline float chain_at(const float *rec, int p, int x, int y, int sc_minx, int sc_maxx)
+{
+    const float pa = rec[REC_PARAM + 3 * p], pb = rec[REC_PARAM + 3 * p + 1],
+                pc = rec[REC_PARAM + 3 * p + 2];
+    const uint32_t flags = __float_as_uint(rec[REC_FLAGS]);
+    if (flags & FLAG_BLOCK) {
+        const int bx = x & ~(TILE - 1), by = y & ~(TILE - 1);
+        float r = pa * ((float)bx + 0.5f) + pb * ((float)by + 0.5f) + pc;
+        for (int k = by; k < y; ++k)
+            r = r + pb;
+        for (int i = bx; i < x; ++i)
+            r = r + pa;
+        return r;
+    }
+    int xl = 0, xr = 0;
+    span_row_extent(rec, flags, y, sc_minx, sc_maxx, xl, xr);
+    float v = pa * ((float)xl + 0.5f) + pb * ((float)y + 0.5f) + pc;
+    for (int i = xl; i < x; ++i)
+        v = v + pa;
+    return v;
+}
 
 // ---- pixel programs (include/swr_hip.h) ---------------------------------------------------
 template <class L>
@@ -90,17 +119,21 @@ struct ProgRGB {
 };
 
 // Only the last depth-passing fragment of a pixel reaches the targets, and its colour is a
-// pure function of its interpolants: shade keeps that fragment's raw values, store computes
-// the colour once (the same operations on the same values as shading it on the spot).
+// pure function of its interpolants.  So shade keeps only that fragment: for triangles its
+// primitive record (the raster kernel chains z alone, CHAINED), and store chains its other
+// planes again at the pixel (chain_at: the same operations on the same values); for points
+// and lines its raw values.  Store computes the colour once.
 template <class L>
 struct ProgDepthChecker {
     static constexpr int KEY_WORD = REC_KEY;
-    static constexpr bool PLANES = false;
-    struct State { float depth, wr, p0, p1, a0, a1, a2; bool hit; };
+    static constexpr bool PLANES = true;
+    static constexpr int CHAINED = 1;  // z
+    struct State { float depth, wr, p0, p1, a0, a1, a2; const float *rec; bool hit; };
     __device__ static void load(State &s, const RasterArgs &a, size_t o)
     {
         s.depth = static_cast<const float *>(a.t1)[o];
         s.wr = s.p0 = s.p1 = s.a0 = s.a1 = s.a2 = 0.0f;
+        s.rec = nullptr;
         s.hit = false;
     }
     template <class PX>
@@ -109,19 +142,33 @@ struct ProgDepthChecker {
         const float z = v.z();
         if (z < s.depth) {
             s.depth = z;
-            s.wr = v.w_raw();
-            s.p0 = v.p_raw(0);
-            s.p1 = v.p_raw(1);
-            s.a0 = v.avar(0);
-            s.a1 = v.avar(1);
-            s.a2 = v.avar(2);
+            if constexpr (PX::REPLAY) {
+                s.rec = v.rec;
+            } else {
+                s.wr = v.w_raw();
+                s.p0 = v.p_raw(0);
+                s.p1 = v.p_raw(1);
+                s.a0 = v.avar(0);
+                s.a1 = v.avar(1);
+                s.a2 = v.avar(2);
+            }
             s.hit = true;
         }
     }
     template <class PX>
-    __device__ static void store(const State &s, const RasterArgs &a, size_t o)
+    __device__ static void store(const State &s0, const RasterArgs &a, size_t o)
     {
-        if (s.hit) {
+        if (s0.hit) {
+            State s = s0;
+            if constexpr (PX::REPLAY) {
+                const int y = (int)(o / (size_t)a.pitch), x = (int)(o - (size_t)y * a.pitch);
+                s.wr = chain_at(s.rec, L::IW, x, y, a.sc_minx, a.sc_maxx);
+                s.a0 = chain_at(s.rec, L::A, x, y, a.sc_minx, a.sc_maxx);
+                s.a1 = chain_at(s.rec, L::A + 1, x, y, a.sc_minx, a.sc_maxx);
+                s.a2 = chain_at(s.rec, L::A + 2, x, y, a.sc_minx, a.sc_maxx);
+                s.p0 = chain_at(s.rec, L::P, x, y, a.sc_minx, a.sc_maxx);
+                s.p1 = chain_at(s.rec, L::P + 1, x, y, a.sc_minx, a.sc_maxx);
+            }
             const float w = PX::w_of(s.wr);  // only the depth-passing pixels read pvar
             const float u = PX::pvar_of(s.p0, w), t = PX::pvar_of(s.p1, w);
             int cu = f2i(floorf(u * a.checker_cells));

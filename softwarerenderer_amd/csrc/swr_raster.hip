@@ -49,6 +49,17 @@ __device__ inline void lds_sync()
     __asm__ volatile("" ::: "memory");
 }
 
+// Planes phase A chains for a program: all of them, unless the program declares CHAINED
+// (it reads only the first CHAINED per fragment and chains the others itself later).
+template <class P, int N, class = void>
+struct chained_planes {
+    static constexpr int value = N;
+};
+template <class P, int N>
+struct chained_planes<P, N, decltype((void)P::CHAINED)> {
+    static constexpr int value = P::CHAINED < N ? P::CHAINED : N;
+};
+
 template <template <class> class ProgT, int NZ, int NW, int NA, int NP, int GSX>
 __global__ void __launch_bounds__(512) k_raster(RasterArgs a0)
 {
@@ -56,6 +67,7 @@ __global__ void __launch_bounds__(512) k_raster(RasterArgs a0)
     using Prog = ProgT<L>;
     constexpr int N = L::N;
     constexpr int NS = N > 0 ? N : 1;
+    constexpr int NC = chained_planes<Prog, N>::value;  // planes phase A chains
     // primitives per group: the program's default, or the tuned value
     constexpr int GS = GSX > 0 ? GSX : (N <= 8 ? 8 : (N <= 16 ? 4 : (N <= 32 ? 2 : 1)));
     static_assert(GS >= 1 && GS <= 8, "a group is at most 8 primitives (one per row slot)");
@@ -165,7 +177,7 @@ __global__ void __launch_bounds__(512) k_raster(RasterArgs a0)
         // chained values: phase A row walk, then transpose to pixel-per-lane
         float val[GS][NS];
 #pragma unroll
-        for (int p = 0; p < N; ++p) {
+        for (int p = 0; p < NC; ++p) {
             float out[8];
             if (act) {
                 const float pa = rec[REC_PARAM + 3 * p], pb = rec[REC_PARAM + 3 * p + 1],
@@ -203,7 +215,7 @@ __global__ void __launch_bounds__(512) k_raster(RasterArgs a0)
                 val[s2][p] = rows[s2 * 64 + pj * 8 + pi];
             lds_sync();
         }
-        if (N == 0)
+        if (NC == 0)
             lds_sync();
 
         // phase B: shade the group's primitives in draw order
