@@ -398,8 +398,8 @@ def batch_bench(args, dev):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
-              f"--gpus {args.gpus}); sharding is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU: the draw path has no multi-GPU launch (asked for "
+              f"--gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
     import torch
 

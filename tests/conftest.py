@@ -9,7 +9,7 @@ if ROOT not in sys.path:
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs a gfx950 GPU (runs on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs a gfx950 GPU")
 
 
 @pytest.fixture(scope="session")
