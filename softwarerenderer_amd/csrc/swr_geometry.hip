@@ -346,7 +346,7 @@ template <int EXPAND_CHUNK>
 __global__ void __launch_bounds__(256) k_expand(GeomArgs a)
 {
     __shared__ uint32_t s_pb[EXPAND_CHUNK + 1];
-    __shared__ uint32_t s_p0, s_np;
+    __shared__ uint32_t s_p0, s_p1;
     if (a.status[ST_OVERFLOW] | a.status[ST_ERROR])
         return;
     const uint64_t M = a.status[ST_PAIRS], P = a.status[ST_PRIMS];
@@ -354,25 +354,31 @@ __global__ void __launch_bounds__(256) k_expand(GeomArgs a)
     if (c0 >= M)
         return;
     const uint64_t c1 = c0 + EXPAND_CHUNK < M ? c0 + EXPAND_CHUNK : M;
-    if (threadIdx.x == 0) {
-        // last primitive whose first pair is <= target
-        auto owner = [&](uint64_t target) {
-            uint64_t lo = 0, hi = P;  // invariant: pb[lo] <= target, answer in [lo, hi)
-            while (hi - lo > 1) {
-                uint64_t mid = (lo + hi) >> 1;
-                if (a.prim_pb[mid] <= target)
-                    lo = mid;
-                else
-                    hi = mid;
+    if (threadIdx.x < 64) {
+        // the last primitive whose first pair is <= target, for the chunk's first slot (lanes
+        // 0-31) and its last (lanes 32-63): a 32-way search, one round of parallel loads per
+        // step (log32 P steps instead of log2 P dependent loads)
+        const uint32_t lane = threadIdx.x, sub = lane & 31u;
+        const uint64_t target = lane < 32 ? c0 : c1 - 1;
+        uint64_t lo = 0, hi = P;  // invariant: pb[lo] <= target (pb[0] = 0), answer in [lo, hi)
+        while (__ballot(hi - lo > 1) != 0ull) {
+            const uint64_t span = hi - lo, step = (span + 31) / 32;
+            const uint64_t idx = lo + (uint64_t)sub * step;
+            const bool le = span > 1 && idx < hi && a.prim_pb[idx] <= target;
+            const unsigned long long m = __ballot(le);
+            const uint32_t half = lane < 32 ? (uint32_t)m : (uint32_t)(m >> 32);
+            if (span > 1) {  // sub 0 (idx = lo) always qualifies, so half != 0
+                lo += (uint64_t)(31 - __builtin_clz(half)) * step;
+                hi = lo + step < hi ? lo + step : hi;
             }
-            return lo;
-        };
-        uint64_t p0 = owner(c0), p1 = owner(c1 - 1);
-        s_p0 = (uint32_t)p0;
-        s_np = (uint32_t)(p1 - p0 + 1);
+        }
+        if (lane == 0)
+            s_p0 = (uint32_t)lo;
+        if (lane == 32)
+            s_p1 = (uint32_t)lo;
     }
     __syncthreads();
-    const uint32_t p0 = s_p0, np = s_np;
+    const uint32_t p0 = s_p0, np = s_p1 - s_p0 + 1;
     for (uint32_t i = threadIdx.x; i <= np; i += blockDim.x)
         s_pb[i] = (p0 + i < P) ? a.prim_pb[p0 + i] : (uint32_t)M;
     __syncthreads();
