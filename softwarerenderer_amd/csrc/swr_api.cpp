@@ -438,7 +438,7 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
     HIP_TRY(c, ensure(c, c->app_ex, T * 4));
     HIP_TRY(c, ensure(c, c->grp, ngrp * 8));
     HIP_TRY(c, ensure(c, c->grp_ex, ngrp * 8));
-    HIP_TRY(c, ensure(c, c->scan_tmp, scan_tmp_elems(std::max<uint64_t>(T, ngrp)) * 8));
+    HIP_TRY(c, ensure(c, c->scan_tmp, (scan_tmp_elems(T) + scan_tmp_elems(ngrp)) * 8));
     HIP_TRY(c, ensure(c, c->ranges, (size_t)tiles * 8));
     HIP_TRY(c, ensure(c, c->status, ST_WORDS * 8));
     c->prim_cap = std::max<uint64_t>(c->prim_cap, T + 256);
@@ -519,10 +519,12 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
         HIP_TRY(c, launch_geom_count(g, c->stream));
         record(c, 1);
         if (!window_input)
-            HIP_TRY(c, scan_u32(g.app, (uint32_t *)c->app_ex.p, T, (uint32_t *)c->scan_tmp.p,
-                                c->stream));
-        HIP_TRY(c, scan_pairs(g.grp, (uint2 *)c->grp_ex.p, ngrp, (uint2 *)c->scan_tmp.p, g.status,
-                              c->prim_cap, c->pair_cap, c->stream));
+            HIP_TRY(c, scan_fans_and_pairs(g.app, (uint32_t *)c->app_ex.p, T, g.grp,
+                                           (uint2 *)c->grp_ex.p, ngrp, c->scan_tmp.p, g.status,
+                                           c->prim_cap, c->pair_cap, c->stream));
+        else
+            HIP_TRY(c, scan_pairs(g.grp, (uint2 *)c->grp_ex.p, ngrp, (uint2 *)c->scan_tmp.p,
+                                  g.status, c->prim_cap, c->pair_cap, c->stream));
         record(c, 2);
         HIP_TRY(c, launch_geom_emit(g, c->stream, c->side, c->fork, c->join));
         HIP_TRY(c, launch_expand(g, c->stream, c->tune.expand_chunk));

@@ -49,6 +49,12 @@ hipError_t scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *tmp
 hipError_t scan_pairs(const uint2 *in, uint2 *out, uint64_t n, uint2 *tmp,
                       unsigned long long *status, uint64_t cap_prims, uint64_t cap_pairs,
                       hipStream_t s);
+// scan_u32 of the appended-fan counts and scan_pairs of the slot counts in shared launches;
+// tmp holds scan_tmp_elems(na) + scan_tmp_elems(ng) 8-byte elements
+hipError_t scan_fans_and_pairs(const uint32_t *app, uint32_t *app_ex, uint64_t na,
+                               const uint2 *grp, uint2 *grp_ex, uint64_t ng, void *tmp,
+                               unsigned long long *status, uint64_t cap_prims,
+                               uint64_t cap_pairs, hipStream_t s);
 int radix_passes(uint32_t tiles);
 hipError_t sort_pairs(SortBuffers &sb, const unsigned long long *status, uint32_t tiles,
                       hipStream_t s);

@@ -100,10 +100,11 @@ __device__ inline void store_items(T *out, uint64_t base, uint64_t n, const T (&
     }
 }
 
+// Sum of scan tile b (SCAN_TILE elements) into sums[b].
 template <typename T>
-__global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce(const T *in, uint64_t n, T *sums)
+__device__ void reduce_tile(const T *in, uint64_t n, T *sums, uint32_t b)
 {
-    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    const uint64_t base = (uint64_t)b * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
     T v[SCAN_ITEMS];
     load_items<T>(in, base, n, v);
     T acc = szero<T>();
@@ -113,7 +114,13 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce(const T *in, uint6
     T total;
     block_exclusive<T>(acc, &total);
     if (threadIdx.x == 0)
-        sums[blockIdx.x] = total;
+        sums[b] = total;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce(const T *in, uint64_t n, T *sums)
+{
+    reduce_tile<T>(in, n, sums, blockIdx.x);
 }
 
 // One block scans the block sums in place (exclusive), SCAN_TILE sums per step.
@@ -179,13 +186,12 @@ __device__ T block_carry(const T *sums, uint32_t b)
 // two-kernel scan); else sums[] holds their exclusive scan.  With `status` set the thread
 // that owns the last element publishes the total (publish_totals).
 template <typename T, bool CARRY>
-__global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(const T *in, uint64_t n,
-                                                          const T *sums, T *out,
-                                                          unsigned long long *status, int word,
-                                                          uint64_t cap_x, uint64_t cap_y)
+__device__ void down_tile(const T *in, uint64_t n, const T *sums, T *out,
+                          unsigned long long *status, int word, uint64_t cap_x, uint64_t cap_y,
+                          uint32_t b)
 {
-    const T carry = CARRY ? block_carry<T>(sums, blockIdx.x) : sums[blockIdx.x];
-    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    const T carry = CARRY ? block_carry<T>(sums, b) : sums[b];
+    const uint64_t base = (uint64_t)b * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
     T vals[SCAN_ITEMS];
     load_items<T>(in, base, n, vals);
     T acc = szero<T>();
@@ -203,6 +209,39 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(const T *in, uint64_
         run = sadd(run, vals[i]);
     }
     store_items<T>(out, base, n, o);
+}
+
+template <typename T, bool CARRY>
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(const T *in, uint64_t n,
+                                                          const T *sums, T *out,
+                                                          unsigned long long *status, int word,
+                                                          uint64_t cap_x, uint64_t cap_y)
+{
+    down_tile<T, CARRY>(in, n, sums, out, status, word, cap_x, cap_y, blockIdx.x);
+}
+
+// The per-triangle appended-fan counts (u32) and the per-slot (primitives, pairs) counts
+// (uint2) scanned together: blocks [0, nb_a) take the first, the rest the second.
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce2(const uint32_t *a, uint64_t na,
+                                                             uint32_t *sa, uint32_t nba,
+                                                             const uint2 *g, uint64_t ng,
+                                                             uint2 *sg)
+{
+    if (blockIdx.x < nba)
+        reduce_tile<uint32_t>(a, na, sa, blockIdx.x);
+    else
+        reduce_tile<uint2>(g, ng, sg, blockIdx.x - nba);
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_down2(
+    const uint32_t *a, uint64_t na, const uint32_t *sa, uint32_t *oa, uint32_t nba, const uint2 *g,
+    uint64_t ng, const uint2 *sg, uint2 *og, unsigned long long *status, uint64_t cap_x,
+    uint64_t cap_y)
+{
+    if (blockIdx.x < nba)
+        down_tile<uint32_t, true>(a, na, sa, oa, nullptr, 0, 0, 0, blockIdx.x);
+    else
+        down_tile<uint2, true>(g, ng, sg, og, status, 0, cap_x, cap_y, blockIdx.x - nba);
 }
 
 // Scans of up to this many blocks skip the pass over the block sums: each block of the
@@ -240,6 +279,29 @@ hipError_t scan_pairs(const uint2 *in, uint2 *out, uint64_t n, uint2 *tmp,
                       hipStream_t s)
 {
     return scan_impl<uint2>(in, out, n, tmp, status, 0, cap_prims, cap_pairs, s);
+}
+
+hipError_t scan_fans_and_pairs(const uint32_t *app, uint32_t *app_ex, uint64_t na,
+                               const uint2 *grp, uint2 *grp_ex, uint64_t ng, void *tmp,
+                               unsigned long long *status, uint64_t cap_prims,
+                               uint64_t cap_pairs, hipStream_t s)
+{
+    const uint32_t nba = (uint32_t)((na + SCAN_TILE - 1) / SCAN_TILE);
+    const uint32_t nbg = (uint32_t)((ng + SCAN_TILE - 1) / SCAN_TILE);
+    // block sums: u32 of the first scan, then (8-byte aligned) uint2 of the second
+    uint32_t *sa = static_cast<uint32_t *>(tmp);
+    uint2 *sg = reinterpret_cast<uint2 *>(sa + ((nba + 1) & ~1u));
+    if (!na || !ng || nba > SCAN_CARRY_BLOCKS || nbg > SCAN_CARRY_BLOCKS) {
+        hipError_t e = scan_u32(app, app_ex, na, sa, s);
+        if (e != hipSuccess)
+            return e;
+        return scan_pairs(grp, grp_ex, ng, sg, status, cap_prims, cap_pairs, s);
+    }
+    hipLaunchKernelGGL(k_scan_reduce2, dim3(nba + nbg), dim3(SCAN_THREADS), 0, s, app, na, sa,
+                       nba, grp, ng, sg);
+    hipLaunchKernelGGL(k_scan_down2, dim3(nba + nbg), dim3(SCAN_THREADS), 0, s, app, na, sa,
+                       app_ex, nba, grp, ng, sg, grp_ex, status, cap_prims, cap_pairs);
+    return hipGetLastError();
 }
 
 // ---- radix sort of (tile, primitive) pairs ---------------------------------------------
