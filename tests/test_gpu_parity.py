@@ -211,3 +211,20 @@ def test_rasterizer_test_triangle(gpu, cpu_oracle):
         b = run_gpu(pipe, vertices=v, indices=np.arange(3, dtype=np.int32))
         assert_same(a, b, f"RasterizerTest mode={mode}")
         assert (a.t0 != 0).sum() > 20000
+
+
+@pytest.mark.parametrize("mode", [O.RASTER_SPAN, O.RASTER_BLOCK])
+@pytest.mark.parametrize("counts", [(0, 0), (2, 1), (3, 2), (4, 4), (8, 8), (16, 16)])
+def test_clip_varying_classes(gpu, cpu_oracle, counts, mode):
+    """Clipped triangles through every varying class of the geometry stage (0, 3, 5, 8, 16
+    and 32 varyings: the register clippers, the clip queue's polygon store in LDS, and its
+    per-thread store for wide varyings), full invocation logs against the oracle."""
+    na, np_ = counts
+    sc = scenes.clip_scene(ntri=3000, seed=17 + na, width=160, height=120, navar=na, npvar=np_,
+                           shrink=(0.01, 0.25))
+    pipe = O.Pipeline(160, 120, raster_mode=mode, cull_mode=O.CULL_NONE, vs_program=O.VS_RAW,
+                      vs_counts=(1, na, np_), ps_program=O.PS_LOG, ps_a=na, ps_p=np_)
+    a = run_cpu(cpu_oracle, pipe, sc, log_capacity=400_000)
+    b = run_gpu(pipe, sc, log_capacity=400_000)
+    assert int(a.log_count[0]) > 10_000
+    assert_same(a, b, f"clip varyings={counts} mode={mode}")
