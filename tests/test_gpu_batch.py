@@ -70,7 +70,7 @@ def run_batch(cases, rasterizer=None, draws=1):
     return res
 
 
-def mvp_cases():
+def mvp_cases(ps=O.PS_HASH):
     sph = S.sphere_scene(64, 128, seed=1, width=640, height=480)
     sph_small = S.sphere_scene(16, 32, seed=2, width=256, height=256)
     sph_big = S.sphere_scene(24, 48, seed=3, width=1024, height=1024)
@@ -79,13 +79,13 @@ def mvp_cases():
     soup_s = S.soup_scene(5_000, seed=6, width=512, height=512)
     empty = S.Scene(sph.vertices, np.zeros(0, np.int32), sph.mvp, 96, 80)
     return [
-        (sph, pipe_for(sph, O.RASTER_SPAN)),
-        (soup, pipe_for(soup, O.RASTER_BLOCK)),
-        (sph_small, pipe_for(sph_small, O.RASTER_BLOCK, cull=O.CULL_NONE)),
-        (empty, pipe_for(empty, O.RASTER_SPAN)),
-        (sph_big, pipe_for(sph_big, O.RASTER_ADAPTIVE, scissor=(100, 50, 700, 900))),
-        (tiny, pipe_for(tiny, O.RASTER_SPAN)),
-        (soup_s, pipe_for(soup_s, O.RASTER_SPAN, cull=O.CULL_CCW)),
+        (sph, pipe_for(sph, O.RASTER_SPAN, ps=ps)),
+        (soup, pipe_for(soup, O.RASTER_BLOCK, ps=ps)),
+        (sph_small, pipe_for(sph_small, O.RASTER_BLOCK, cull=O.CULL_NONE, ps=ps)),
+        (empty, pipe_for(empty, O.RASTER_SPAN, ps=ps)),
+        (sph_big, pipe_for(sph_big, O.RASTER_ADAPTIVE, scissor=(100, 50, 700, 900), ps=ps)),
+        (tiny, pipe_for(tiny, O.RASTER_SPAN, ps=ps)),
+        (soup_s, pipe_for(soup_s, O.RASTER_SPAN, cull=O.CULL_CCW, ps=ps)),
     ]
 
 
@@ -98,6 +98,17 @@ def test_batch_matches_single_draws_and_oracle(gpu, cpu_oracle):
         if sc.triangles <= 40_000:
             assert_same(run_cpu(cpu_oracle, pipe, sc), g, f"scene {i}: batch vs oracle")
         assert int(g.t1.sum()) > 0 or sc.triangles == 0
+
+
+def test_batch_depth_checker(gpu, cpu_oracle):
+    """The depth-checker program in a batch: each pixel chains its winning fragment's planes
+    again when its tile is written, with its own scene's scissor and pitch."""
+    cases = mvp_cases(ps=O.PS_DEPTH_CHECKER)
+    got = run_batch(cases)
+    for i, ((sc, pipe), g) in enumerate(zip(cases, got)):
+        ref = run_cpu(cpu_oracle, pipe, sc) if sc.triangles <= 40_000 else run_gpu(pipe, sc)
+        assert_same(ref, g, f"scene {i}: depth checker in a batch")
+        assert int((g.t0 != 0).sum()) > 0 or sc.triangles == 0
 
 
 def test_batch_redraw_and_reorder(gpu):
