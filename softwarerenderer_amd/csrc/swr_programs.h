@@ -94,27 +94,50 @@ struct ProgCoverage {
     }
 };
 
-// RasterizerTest.cpp:38-48
+// RasterizerTest.cpp:38-48.  Every fragment overwrites the pixel, so only the last one
+// counts: for triangles shade keeps its record (the raster kernel chains nothing, CHAINED)
+// and store chains its colour planes at the pixel (chain_at) and packs them once.
 template <class L>
 struct ProgRGB {
     static constexpr int KEY_WORD = REC_KEY;
-    static constexpr bool PLANES = false;
-    struct State { uint32_t c; bool hit; };
-    __device__ static void load(State &s, const RasterArgs &, size_t) { s.hit = false; s.c = 0; }
+    static constexpr bool PLANES = true;
+    static constexpr int CHAINED = 0;
+    struct State { uint32_t c; const float *rec; bool hit; };
+    __device__ static void load(State &s, const RasterArgs &, size_t)
+    {
+        s.hit = false;
+        s.c = 0;
+        s.rec = nullptr;
+    }
+    __device__ static uint32_t pack(float r0, float g0, float b0)
+    {
+        uint32_t r = (uint32_t)f2i(r0 * 255.0f);
+        uint32_t g = (uint32_t)f2i(g0 * 255.0f);
+        uint32_t b = (uint32_t)f2i(b0 * 255.0f);
+        return (r << 16) | (g << 8) | b;
+    }
     template <class PX>
     __device__ static void shade(State &s, const RasterArgs &, const PX &v, uint32_t, int, int)
     {
-        uint32_t r = (uint32_t)f2i(v.avar(0) * 255.0f);
-        uint32_t g = (uint32_t)f2i(v.avar(1) * 255.0f);
-        uint32_t b = (uint32_t)f2i(v.avar(2) * 255.0f);
-        s.c = (r << 16) | (g << 8) | b;
+        if constexpr (PX::REPLAY)
+            s.rec = v.rec;
+        else
+            s.c = pack(v.avar(0), v.avar(1), v.avar(2));
         s.hit = true;
     }
-    template <class PX = void>
+    template <class PX>
     __device__ static void store(const State &s, const RasterArgs &a, size_t o)
     {
-        if (s.hit)
-            static_cast<uint32_t *>(a.t0)[o] = s.c;
+        if (!s.hit)
+            return;
+        uint32_t c = s.c;
+        if constexpr (PX::REPLAY) {
+            const int y = (int)(o / (size_t)a.pitch), x = (int)(o - (size_t)y * a.pitch);
+            c = pack(chain_at(s.rec, L::A, x, y, a.sc_minx, a.sc_maxx),
+                     chain_at(s.rec, L::A + 1, x, y, a.sc_minx, a.sc_maxx),
+                     chain_at(s.rec, L::A + 2, x, y, a.sc_minx, a.sc_maxx));
+        }
+        static_cast<uint32_t *>(a.t0)[o] = c;
     }
 };
 
@@ -455,8 +478,15 @@ template <class L>
 struct ProgTexture {
     static constexpr int KEY_WORD = REC_KEY;
     static constexpr bool PLANES = true;
-    struct State { uint32_t c; bool hit; };
-    __device__ static void load(State &s, const RasterArgs &, size_t) { s.hit = false; s.c = 0; }
+    // every fragment overwrites the pixel: shade keeps the last one's record and store
+    // samples once, from the planes chained again at the pixel (chain_at)
+    static constexpr int CHAINED = 0;
+    struct State { const float *rec; bool hit; };
+    __device__ static void load(State &s, const RasterArgs &, size_t)
+    {
+        s.hit = false;
+        s.rec = nullptr;
+    }
     __device__ static void derivs(const float *pl, const float *iw, float xf, float yf,
                                   float &ddx, float &ddy)
     {
@@ -466,23 +496,26 @@ struct ProgTexture {
         ddy = (cur * pl[1] - var * iw[1]) / (cur * cur);
     }
     template <class PX>
-    __device__ static void shade(State &s, const RasterArgs &a, const PX &v, uint32_t, int x, int y)
+    __device__ static void shade(State &s, const RasterArgs &, const PX &v, uint32_t, int, int)
     {
-        const float w = v.w();
-        const float u = v.pvar(0, w), t = v.pvar(1, w);
-        const float *P = v.rec + REC_PARAM;
+        s.rec = v.rec;
+        s.hit = true;
+    }
+    template <class PX>
+    __device__ static void store(const State &s, const RasterArgs &a, size_t o)
+    {
+        if (!s.hit)
+            return;
+        const int y = (int)(o / (size_t)a.pitch), x = (int)(o - (size_t)y * a.pitch);
+        const float w = PX::w_of(chain_at(s.rec, L::IW, x, y, a.sc_minx, a.sc_maxx));
+        const float u = PX::pvar_of(chain_at(s.rec, L::P, x, y, a.sc_minx, a.sc_maxx), w);
+        const float t = PX::pvar_of(chain_at(s.rec, L::P + 1, x, y, a.sc_minx, a.sc_maxx), w);
+        const float *P = s.rec + REC_PARAM;
         const float xf = (float)x + 0.5f, yf = (float)y + 0.5f;
         float dudx, dudy, dvdx, dvdy;
         derivs(P + 3 * (L::P + 0), P + 3 * L::IW, xf, yf, dudx, dudy);
         derivs(P + 3 * (L::P + 1), P + 3 * L::IW, xf, yf, dvdx, dvdy);
-        s.c = tex_sample(*a.tex, u, t, dudx, dvdx, dudy, dvdy);
-        s.hit = true;
-    }
-    template <class PX = void>
-    __device__ static void store(const State &s, const RasterArgs &a, size_t o)
-    {
-        if (s.hit)
-            static_cast<uint32_t *>(a.t0)[o] = s.c;
+        static_cast<uint32_t *>(a.t0)[o] = tex_sample(*a.tex, u, t, dudx, dvdx, dudy, dvdy);
     }
 };
 
