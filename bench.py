@@ -61,18 +61,34 @@ def parse():
     return p.parse_args()
 
 
-def measured_traffic(workload: str):
-    """HBM bytes per raster launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded
-    in profiles/traffic.json (scaled by the copy-probe factors recorded with them)."""
+def _counters(workload: str) -> dict:
+    """The raster kernel's rocprofv3 counter figures recorded in profiles/traffic.json."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
+        return {}
+    return t if t.get("workload") == workload else {}
+
+
+def measured_traffic(workload: str):
+    """HBM bytes per raster launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled
+    by the copy-probe factors recorded with them)."""
+    return _counters(workload).get("raster_bytes")
+
+
+def valu_issue(workload: str, raster_ms: float):
+    """The raster kernel's VALU issue rate: SQ_INSTS_VALU per launch (rocprofv3) over the
+    launch time measured here, against the wave64 fp32 issue peak."""
+    t = _counters(workload)
+    insts, peak = t.get("valu_insts_per_launch"), t.get("valu_peak_winst_per_s")
+    if not insts or not peak or raster_ms <= 0:
         return None
-    if t.get("workload") != workload:
-        return None
-    return t.get("raster_bytes")
+    rate = insts / (raster_ms * 1e-3)
+    return {"achieved": round(rate / 1e9, 1), "peak": round(peak / 1e9, 1),
+            "unit": "G wave-instructions/s", "frac": round(rate / peak, 4),
+            "insts_per_launch": int(insts)}
 
 
 def diff_bench(args, dev):
@@ -522,7 +538,8 @@ def main():
                      "achieved": round(raster_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(raster_gbs / HBM_PEAK_GBS, 5),
                      "traffic": measured_traffic("soup-1M-1080p-block"),
-                     "algorithmic_bytes": raster_bytes},
+                     "algorithmic_bytes": raster_bytes,
+                     "valu_issue": valu_issue("soup-1M-1080p-block", raster_ms)},
         "tuning": tuning,
         "frame": {"algorithmic_bytes": frame_bytes, "kernel_ms": round(kern_ms, 4),
                   "achieved_GBs": round(frame_bytes / (kern_ms * 1e-3) / 1e9, 2)
