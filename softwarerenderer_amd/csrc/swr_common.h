@@ -80,7 +80,13 @@ __host__ __device__ inline float ref_max(float a, float b) { return (a < b) ? b 
 __host__ __device__ inline int imin(int a, int b) { return (b < a) ? b : a; }
 __host__ __device__ inline int imax(int a, int b) { return (a < b) ? b : a; }
 
-__device__ inline bool edge_pass(float v, bool tie) { return v > 0.0f || (v == 0.0f && tie); }
+// EdgeEquation::test (EdgeEquation.h:53-61): v > 0 || (v == 0 && tie), as one compare:
+// v >= 0 with the tie, v >= the smallest denormal without it (the kernels keep fp32
+// denormals, so nothing lies strictly between 0 and it; NaN fails both forms).
+__device__ inline bool edge_pass(float v, bool tie)
+{
+    return v >= (tie ? 0.0f : __uint_as_float(1u));
+}
 
 // Block classification from the four corners, Rasterizer.h:270-304: 0 skip,
 // 1 drawBlock<true> (per-pixel edge tests), 2 drawBlock<false> (all 64 pixels).
