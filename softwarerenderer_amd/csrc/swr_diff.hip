@@ -465,9 +465,13 @@ __device__ inline void bary_color(const DiffArgs &d, const Bary &B, double *col)
 // table in LDS keyed by vertex, flushed to global memory once per distinct vertex.  The
 // pixels of a tile touch a few dozen vertices, so this replaces most global float atomics
 // (which serialise on shared vertices) by LDS atomics.  A vertex that finds no slot after
-// ACC_PROBES probes goes straight to global memory.
+// `probes` probes goes straight to global memory.  Slots are never freed, so a vertex that
+// got a slot is found again within the same probes.
 constexpr int ACC_SLOTS = 256;   // power of two
-constexpr int ACC_PROBES = 32;
+constexpr int ACC_PROBES = 32;   // one 16x16 tile: a few dozen vertices, the table never fills
+// the edge backward's table serves a group of 64 tiles, whose silhouette pixels can touch
+// more vertices than it holds: a full table then costs every new vertex all its probes
+constexpr int ACC_PROBES_SPARSE = 8;
 constexpr int ACC_VALS = 8;      // at most: clip x, y, w, attributes 0..4
 // values per slot with C attribute channels: clip x, y, w and the C attributes
 template <int C> constexpr int acc_vals() { return 3 + C; }
@@ -481,6 +485,7 @@ __device__ inline int acc_copy() { return (int)((threadIdx.x & 1u) | ((threadIdx
 struct VertexAcc {
     int *key;
     float *val;
+    int probes;
 };
 
 __device__ inline int acc_slot(const VertexAcc &A, int v)
@@ -489,7 +494,7 @@ __device__ inline int acc_slot(const VertexAcc &A, int v)
         return -1;  // no table: straight to global memory
     const unsigned h = ((unsigned)v * 2654435761u) >> 24;  // 8 bits: ACC_SLOTS
 #pragma unroll 1
-    for (int probe = 0; probe < ACC_PROBES; ++probe) {
+    for (int probe = 0; probe < A.probes; ++probe) {
         const int sl = (int)((h + (unsigned)probe) & (ACC_SLOTS - 1));
         const int k = __atomic_load_n(&A.key[sl], __ATOMIC_RELAXED);
         if (k == v)
@@ -1154,7 +1159,7 @@ __global__ void __launch_bounds__(256) k_diff_backward(DiffArgs d)
     const bool valid = pixel_of_thread(d, pi);
     tile_pairs<false>(d, pi, valid, s_h, s_v);
     __syncthreads();
-    const VertexAcc A{s_key, s_val};
+    const VertexAcc A{s_key, s_val, ACC_PROBES};
     if (valid)  // every thread reaches the barrier below
         backward_pixel<C, BWD_ALL>(d, A, pi, TilePairs{s_h, s_v});
     __syncthreads();
@@ -1243,7 +1248,7 @@ __global__ void __launch_bounds__(256) k_diff_bwd_interior(DiffArgs d)
     SWR_MERGE(4)
 #undef SWR_MERGE
     __syncthreads();
-    const VertexAcc A{s_key, s_val};
+    const VertexAcc A{s_key, s_val, ACC_PROBES};
     if (keep)
         acc_triangle<C>(d, A, pi.view, vid, g);
     __syncthreads();
@@ -1276,7 +1281,8 @@ __global__ void __launch_bounds__(256) k_diff_bwd_edges(DiffArgs d)
         __syncthreads();
     }
     const size_t HW = (size_t)d.W * d.H;
-    const VertexAcc A = one_view ? VertexAcc{s_key, s_val} : VertexAcc{nullptr, nullptr};
+    const VertexAcc A = one_view ? VertexAcc{s_key, s_val, ACC_PROBES_SPARSE}
+                                   : VertexAcc{nullptr, nullptr, 0};
     for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
         const uint32_t t = group_tile(s_off, j);
         PixelIdx pi;
