@@ -397,7 +397,7 @@ struct Bary {
     int v[3];
     H3 h[3];
     H3 r;
-    double b[3], S;
+    double b[3], S, inv;  // inv = 1 / S: one fp64 division per pixel, products after it
 };
 
 __device__ inline bool bary_at_ray(const DiffArgs &d, int view, int t, const H3 &r, Bary &B)
@@ -413,9 +413,10 @@ __device__ inline bool bary_at_ray(const DiffArgs &d, int view, int t, const H3 
     B.S = D0 + D1 + D2;
     if (B.S == 0.0)
         return false;
-    B.b[0] = D0 / B.S;
-    B.b[1] = D1 / B.S;
-    B.b[2] = D2 / B.S;
+    B.inv = 1.0 / B.S;
+    B.b[0] = D0 * B.inv;
+    B.b[1] = D1 * B.inv;
+    B.b[2] = D2 * B.inv;
     return true;
 }
 
@@ -444,8 +445,8 @@ __device__ inline void bary_color_dr(const DiffArgs &d, const Bary &B, const H3 
     const H3 c0 = cross3(B.h[1], B.h[2]), c1 = cross3(B.h[2], B.h[0]), c2 = cross3(B.h[0], B.h[1]);
     const double e0 = dot3(c0, dr), e1 = dot3(c1, dr), e2 = dot3(c2, dr);
     const double es = e0 + e1 + e2;
-    const double db[3] = {(e0 - B.b[0] * es) / B.S, (e1 - B.b[1] * es) / B.S,
-                          (e2 - B.b[2] * es) / B.S};
+    const double db[3] = {(e0 - B.b[0] * es) * B.inv, (e1 - B.b[1] * es) * B.inv,
+                          (e2 - B.b[2] * es) * B.inv};
 #pragma unroll
     for (int c = 0; c < C; ++c)
         out[c] = db[0] * d.attr[(size_t)B.v[0] * C + c] + db[1] * d.attr[(size_t)B.v[1] * C + c] +
@@ -630,7 +631,7 @@ __device__ inline void bary_grads(const DiffArgs &d, const Bary &B, const double
     H3 g[3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
-        const double gD = (u[m] - ub) / B.S;
+        const double gD = (u[m] - ub) * B.inv;
         const int j = (m + 1) % 3, k = (m + 2) % 3;
         // dD_m/dh_j = h_k x r, dD_m/dh_k = r x h_j
         const H3 dj = cross3(B.h[k], B.r), dk = cross3(B.r, B.h[j]);
@@ -852,8 +853,12 @@ __device__ inline uint32_t group_tile(const uint32_t *s_off, uint32_t j)
 }
 
 // The queued pairs, one lane each: the silhouette walk, the stored decision, and the tile
-// flags of active pairs.
-__global__ void __launch_bounds__(256) k_diff_march(DiffArgs d)
+// flags of active pairs.  A walk is a chain of dependent loads (adjacency, then the next
+// triangle's vertices), so the kernel is latency-bound: 1024 lanes per group of 64 tiles
+// take a group's queue in one pass where 256 lanes needed several in turn.
+constexpr int MARCH_THREADS = 1024;
+
+__global__ void __launch_bounds__(MARCH_THREADS) k_diff_march(DiffArgs d)
 {
     __shared__ uint32_t s_off[TILES_PER_GROUP + 1];
     const uint32_t tiles = (uint32_t)(d.tiles_per_view * d.V), t0 = blockIdx.x * TILES_PER_GROUP;
@@ -1342,7 +1347,7 @@ static hipError_t launch_diff(const DiffArgs &d, int which, hipStream_t s)
     const uint32_t groups = (blocks + TILES_PER_GROUP - 1) / TILES_PER_GROUP;
     if (which == 0) {
         hipLaunchKernelGGL(k_diff_interp<C>, dim3(blocks), dim3(256), 0, s, d);
-        hipLaunchKernelGGL(k_diff_march, dim3(groups), dim3(256), 0, s, d);
+        hipLaunchKernelGGL(k_diff_march, dim3(groups), dim3(MARCH_THREADS), 0, s, d);
         hipLaunchKernelGGL(k_diff_blend<C>, dim3(blocks), dim3(256), 0, s, d);
     } else {
         if (d.pairs == 2) {
