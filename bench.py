@@ -19,6 +19,7 @@ against one view at a time, and the HIP-graph Adam step.
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import statistics
@@ -56,6 +57,8 @@ def parse():
     p.add_argument("--terrain-size", type=int, default=1024)
     p.add_argument("--terrain-reserve", type=float, default=8.0,
                    help="tile-pair arena of the terrain workload, in multiples of the target")
+    p.add_argument("--no-span", action="store_true",
+                   help="skip the Span-mode workloads (sphere, the reference's Benchmark scene)")
     p.add_argument("--no-tune", action="store_true",
                    help="skip the launch-shape autotuner (built-in defaults)")
     return p.parse_args()
@@ -89,6 +92,85 @@ def valu_issue(workload: str, raster_ms: float):
     return {"achieved": round(rate / 1e9, 1), "peak": round(peak / 1e9, 1),
             "unit": "G wave-instructions/s", "frac": round(rate / peak, 4),
             "insts_per_launch": int(insts)}
+
+
+def span_bench(args, dev):
+    """Span-mode frames: the 640x480 UV sphere (16,128 triangles, CW cull, depth-checker
+    program) and the reference's Benchmark scene (Benchmark.cpp:66-105: 40,960 large
+    overlapping triangles, no cull, the RGB program, 240 M fragments).  Each is drawn K times
+    back to back (targets cleared each time), inputs resident in HBM."""
+    import torch
+
+    from softwarerenderer_amd import api, scenes
+
+    out = {}
+    stream = torch.cuda.current_stream(dev)
+    for name, sc, prog, cull in (
+            ("sphere-640x480-span", scenes.sphere_scene(), "depth", api.CullMode.CW),
+            ("benchmark-640x480-span", scenes.benchmark_scene(), "rgb", api.CullMode.None_)):
+        H, W = sc.height, sc.width
+        verts = torch.from_numpy(sc.vertices).to(dev)
+        idx = torch.from_numpy(sc.indices).to(dev)
+        color = torch.zeros((H, W), dtype=torch.int32, device=dev)
+        depth = torch.ones((H, W), dtype=torch.float32, device=dev)
+        r = api.Rasterizer(0)
+        r.setStream(stream.cuda_stream)
+        v = api.VertexProcessor(r)
+        r.setRasterMode(api.RasterMode.Span)
+        r.setScissorRect(0, 0, W, H)
+        v.setViewport(0, 0, W, H)
+        v.setCullMode(cull)
+        if prog == "depth":
+            v.setVertexShader(api.MvpVertexShader(sc.mvp))
+        else:
+            v.setVertexShader(api.PosRGBVertexShader())
+        v.setVertexAttribPointer(0, sc.stride, verts)
+        # fragments shaded: one hash-program draw with the same varyings
+        hsh = torch.zeros((H, W), dtype=torch.int32, device=dev)
+        cnt = torch.zeros((H, W), dtype=torch.int32, device=dev)
+        if prog == "depth":
+            r.setPixelShader(api.HashShader(hsh, cnt))
+        else:
+            r.setPixelShader(api.HashShader(hsh, cnt, z=False, w=False, avars=3, pvars=0))
+        v.drawElements(api.DrawMode.Triangle, idx)
+        frags = int(cnt.sum().item())
+        r.setPixelShader(api.DepthCheckerShader(color, depth) if prog == "depth"
+                         else api.RGBShader(color))
+
+        def step(sync):
+            color.zero_()
+            depth.fill_(1.0)
+            (v.drawElements if sync else v.drawElementsAsync)(api.DrawMode.Triangle, idx)
+
+        step(True)
+        for _ in range(args.warmup):
+            step(False)
+        r.synchronize()
+        steps = max(args.steps, 20)
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0.record(stream)
+        for _ in range(steps):
+            step(False)
+        t1.record(stream)
+        r.synchronize()
+        torch.cuda.synchronize()
+        ms = t0.elapsed_time(t1) / steps
+        r.setProfiling(True)
+        stage = np.zeros(8)
+        for _ in range(10):
+            step(True)
+            stage += np.array(r.stats().stage_ms[:8], dtype=np.float64)
+        r.setProfiling(False)
+        out[name] = {"triangles": sc.triangles, "fragments": frags,
+                     "pixel_program": "depth<, rgb*checker(uv,8)" if prog == "depth" else "rgb",
+                     "ms_per_frame": round(ms, 4),
+                     "mtri_per_s": round(sc.triangles / (ms * 1e-3) / 1e6, 2),
+                     "gpix_per_s": round(frags / (ms * 1e-3) / 1e9, 3),
+                     "raster_ms": round(stage[5] / 10, 4)}
+        r.close()
+    return out
 
 
 def diff_bench(args, dev):
@@ -487,6 +569,8 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
     ms = t0.elapsed_time(t1) / args.steps
+    # outside the timed region: the last timed frame against the reference's digest of it
+    parity = frame_parity(sc, args, color, depth)
 
     # per-stage kernel time (HIP events on the draw's stream), synchronous draws
     r.setProfiling(True)
@@ -540,6 +624,7 @@ def main():
                      "traffic": measured_traffic("soup-1M-1080p-block"),
                      "algorithmic_bytes": raster_bytes,
                      "valu_issue": valu_issue("soup-1M-1080p-block", raster_ms)},
+        "parity": parity,
         "tuning": tuning,
         "frame": {"algorithmic_bytes": frame_bytes, "kernel_ms": round(kern_ms, 4),
                   "achieved_GBs": round(frame_bytes / (kern_ms * 1e-3) / 1e9, 2)
@@ -547,6 +632,8 @@ def main():
     }
     r.close()
 
+    if not args.no_span:
+        out["span"] = span_bench(args, dev)
     if not args.no_diff:
         out["diff"] = diff_bench(args, dev)
     if not args.no_batch:
@@ -556,6 +643,26 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_runs)
     print(json.dumps(out))
+
+
+def frame_parity(sc, args, color, depth) -> dict:
+    """The drawn frame (RGBA8 words, then depth words) against the digest the reference
+    wrote for the same scene (tests/golden/frame1m.npz, Block, depth-checker program)."""
+    path = os.path.join(ROOT, "tests", "golden", "frame1m.npz")
+    if not os.path.exists(path):
+        return {"reference_digest": "absent"}
+    gold = np.load(path, allow_pickle=False)
+    inp = hashlib.sha256()
+    for a in (sc.vertices, sc.indices, sc.mvp):
+        inp.update(np.ascontiguousarray(a).view(np.uint8).tobytes())
+    if inp.hexdigest() != str(gold["input"]):
+        return {"reference_digest": "not this scene"}
+    h = hashlib.sha256()
+    for t in (color, depth):
+        h.update(t.cpu().numpy().view(np.uint8).tobytes())
+    ok = h.hexdigest() == str(gold["m1_p1"])
+    return {"reference_digest": "match" if ok else "MISMATCH",
+            "checked": "last timed frame, RGBA8 + depth words, sha256"}
 
 
 def cpu_baseline(sc, runs: int):

@@ -180,6 +180,9 @@ const char *swr_last_error(const swr_context *ctx);
 /* Use `stream` (a hipStream_t; NULL is the null stream) for all later work.  A context
  * starts on a non-blocking stream of its own. */
 int  swr_set_stream(swr_context *ctx, void *stream);
+/* Waits for the context stream.  Reports, once, the first failure of any asynchronous draw
+ * since the last report (SWR_ERR_CAPACITY for an arena overflow: that draw drew nothing;
+ * SWR_ERR_INDEX_RANGE, SWR_ERR_CLIP_OVERFLOW); a synchronous draw reports them too. */
 int  swr_synchronize(swr_context *ctx);
 
 /* ---- Rasterizer (Rasterizer.h:52-141) ------------------------------------------------ */
@@ -212,12 +215,16 @@ int  swr_set_vertex_attrib_pointer(swr_context *ctx, int index, int stride,
  * count/2 pairs, points every index.  Lines and points: clipLines / clipPoints
  * (VertexProcessor.cpp:152-215), then the line and point lists above. */
 int  swr_draw_elements(swr_context *ctx, int mode, size_t count, const int32_t *indices);
-/* Same, enqueued on the context stream without waiting.  Device buffers only.  The context's
- * arenas must already be large enough (swr_reserve, or a previous synchronous draw of the
- * same scene); otherwise the draw leaves the targets untouched and the next swr_synchronize
- * returns SWR_ERR_CAPACITY. */
+/* Same, enqueued on the context stream without waiting.  Device buffers only: indices,
+ * vertex attribute 0, render targets and log buffers in host memory fail with
+ * SWR_ERR_INVALID_ARG.  The context's arenas must already be large enough (swr_reserve, or
+ * a previous synchronous draw of the same scene); otherwise the draw leaves the targets
+ * untouched and the next swr_synchronize returns SWR_ERR_CAPACITY, however many draws were
+ * queued after it.  On a stream being captured, a draw that would have to grow an arena
+ * returns SWR_ERR_CAPACITY at once and enqueues nothing. */
 int  swr_draw_elements_async(swr_context *ctx, int mode, size_t count, const int32_t *indices);
 int  swr_reserve(swr_context *ctx, size_t primitives, size_t tile_pairs);
+
 
 /* ---- batched draws --------------------------------------------------------------------
  * No counterpart in the reference: a batch stands for a sequence of independent

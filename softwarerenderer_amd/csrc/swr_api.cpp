@@ -44,6 +44,7 @@ struct swr_context {
     hipStream_t side = nullptr;               // the clip queue's emit (fork / join)
     hipEvent_t fork = nullptr, join = nullptr;
     std::string err;
+    int err_code = SWR_OK;                    // code of the last failure (with err)
 
     // Rasterizer (Rasterizer.h:67-72 defaults)
     int raster_mode = SWR_RASTER_SPAN;
@@ -62,11 +63,12 @@ struct swr_context {
     } attr[SWR_MAX_VERTEX_ATTRIBS];
 
     // arenas
-    DevBuf idx, attr0, wverts, app, app_ex, grp, grp_ex, scan_tmp, prims, pk_a, pv_a, pk_b, pv_b,
+    DevBuf idx, attr0, wverts, app, app_ex, live, grp, grp_ex, scan_tmp, prims, pk_a, pv_a, pk_b, pv_b,
         ranges, hist, hist_ex, hist_tmp, status, tgt[3], log, logcount,
         bigq,       // per-primitive first pair index
         clipq,      // triangles queued for the clipper
         clipq_cnt,  // its per-region counters
+        vwin, vmask,  // post-transform vertices of the vertex pass
         zscratch[2];  // scratch of the differentiation kernels
     uint64_t prim_cap = 0, pair_cap = 0;
     unsigned long long *status_host = nullptr;
@@ -113,6 +115,7 @@ int fail(swr_context *c, int code, const char *fmt, ...)
         vsnprintf(buf, sizeof buf, fmt, ap);
         va_end(ap);
         c->err = buf;
+        c->err_code = code;
     }
     return code;
 }
@@ -120,9 +123,24 @@ int fail(swr_context *c, int code, const char *fmt, ...)
 #define HIP_TRY(c, expr)                                                                      \
     do {                                                                                      \
         hipError_t e_ = (expr);                                                               \
+        if (e_ == hipErrorStreamCaptureUnsupported)                                           \
+            return fail((c), SWR_ERR_CAPACITY,                                                \
+                        "%s: a device arena must grow while the stream is being captured "    \
+                        "(reserve it before capturing: swr_reserve or one uncaptured call)",  \
+                        #expr);                                                               \
         if (e_ != hipSuccess)                                                                 \
             return fail((c), SWR_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));      \
     } while (0)
+
+bool stream_capturing(hipStream_t s)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return cs != hipStreamCaptureStatusNone;
+}
 
 hipError_t ensure(swr_context *c, DevBuf &b, size_t bytes)
 {
@@ -130,6 +148,9 @@ hipError_t ensure(swr_context *c, DevBuf &b, size_t bytes)
         bytes = 256;
     if (b.bytes >= bytes)
         return hipSuccess;
+    // growing synchronizes, frees and allocates: none of that may enter a stream capture
+    if (stream_capturing(c->stream))
+        return hipErrorStreamCaptureUnsupported;
     if (b.p) {
         hipError_t e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess)
@@ -338,6 +359,30 @@ int check_status(swr_context *c)
     return SWR_OK;
 }
 
+// Failures of earlier asynchronous draws (the device's ST_STICKY word, in the last status
+// copy), reported once: the device word is cleared when they are reported.
+int take_sticky(swr_context *c)
+{
+    const unsigned long long f = c->status_host[ST_STICKY];
+    if (!f)
+        return SWR_OK;
+    c->status_host[ST_STICKY] = 0;
+    HIP_TRY(c, hipMemsetAsync(static_cast<unsigned long long *>(c->status.p) + ST_STICKY, 0, 8,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const unsigned long long err = f >> 1;
+    if (err & 1ull)
+        return fail(c, SWR_ERR_INDEX_RANGE,
+                    "an asynchronous draw: an index addresses outside the vertex data");
+    if (err & 2ull)
+        return fail(c, SWR_ERR_CLIP_OVERFLOW,
+                    "an asynchronous draw: a clipped polygon exceeded %d vertices",
+                    SWR_MAX_CLIP_VERTS);
+    return fail(c, SWR_ERR_CAPACITY,
+                "an asynchronous draw overflowed the context's arenas and drew nothing "
+                "(reserve them first: swr_reserve or a synchronous draw of the same scene)");
+}
+
 void collect_stats(swr_context *c)
 {
     c->stats.raster_primitives = c->status_host[ST_PRIMS];
@@ -353,6 +398,47 @@ void collect_stats(swr_context *c)
         }
     }
 }
+
+// Device memory one triangle draw works in, beyond its inputs and render targets.
+struct Arenas {
+    uint2 *app = nullptr, *app_ex = nullptr;
+    uint32_t *live = nullptr, *clipq = nullptr, *clipq_cnt = nullptr;
+    uint2 *grp = nullptr, *grp_ex = nullptr;
+    void *scan_tmp = nullptr;
+    uint32_t *ranges = nullptr;
+    float *prims = nullptr;
+    uint32_t *bigq = nullptr, *pk_a = nullptr, *pv_a = nullptr, *pk_b = nullptr, *pv_b = nullptr;
+    uint32_t *hist = nullptr, *hist_ex = nullptr, *hist_tmp = nullptr;
+    uint64_t prim_cap = 0, pair_cap = 0;
+    uint32_t G = 1;  // radix sort workgroups
+};
+
+// Sizes of the arenas that depend only on the input (triangles, batches, tiles).
+struct FixedSizes {
+    size_t app, live, clipq, clipq_cnt, grp, scan_tmp, ranges;
+};
+
+FixedSizes fixed_sizes(size_t T, uint64_t ngrp, uint32_t tiles)
+{
+    FixedSizes f;
+    f.app = T * 8;
+    f.live = T * 4;
+    const size_t blocks = (T + 255) / 256, per_region = ((blocks + 63) / 64) * 256;
+    f.clipq = 64 * per_region * 4;
+    f.clipq_cnt = 64 * 16 * 4;
+    f.grp = ngrp * 8;
+    f.scan_tmp = (scan_tmp_elems(T) + scan_tmp_elems(ngrp)) * 8;
+    f.ranges = (size_t)tiles * 8;
+    return f;
+}
+
+uint32_t sort_blocks(uint64_t pair_cap)
+{
+    return (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(1, pair_cap / 4096));
+}
+
+size_t hist_bytes(uint32_t G) { return (size_t)256 * G * 4; }
+size_t hist_tmp_bytes(uint32_t G) { return scan_tmp_elems((uint64_t)256 * G) * 4; }
 
 // The triangle draw: drawElements (window_input = false), the IRasterizer seam, or a batch
 // of scenes (bt != nullptr: inputs, state and targets come from the scene table).
@@ -383,8 +469,23 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
     if (tiles == 0)
         return SWR_OK;
     if (!sync && !bt) {
+        // nothing is staged or copied back for an asynchronous draw
         if (!is_device_ptr(indices))
-            return fail(c, SWR_ERR_INVALID_ARG, "async draws take device buffers only");
+            return fail(c, SWR_ERR_INVALID_ARG, "async draws take device buffers only: indices");
+        bool need_t[3];
+        targets_needed(ps.program, need_t);
+        void *const t[3] = {ps.target0, ps.target1, ps.target2};
+        for (int i = 0; i < 3; ++i)
+            if (need_t[i] && !is_device_ptr(t[i]))
+                return fail(c, SWR_ERR_INVALID_ARG,
+                            "async draws take device buffers only: target%d", i);
+        if (ps.program == SWR_PS_LOG &&
+            ((ps.log_capacity && !is_device_ptr(ps.log)) || !is_device_ptr(ps.log_count)))
+            return fail(c, SWR_ERR_INVALID_ARG,
+                        "async draws take device buffers only: log and log_count");
+        if (c->vs.program != SWR_VS_DUMMY && !is_device_ptr(c->attr[0].ptr))
+            return fail(c, SWR_ERR_INVALID_ARG,
+                        "async draws take device buffers only: vertex attribute 0");
     }
 
     TargetMap tm;
@@ -429,32 +530,71 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
 
     const uint32_t batches = (uint32_t)((T + BATCH - 1) / BATCH);
     const uint64_t ngrp = (uint64_t)batches * 2048;
-    HIP_TRY(c, ensure(c, c->app, T * 4));
+    Arenas A;
     {
-        const size_t blocks = (T + 255) / 256, per_region = ((blocks + 63) / 64) * 256;
-        HIP_TRY(c, ensure(c, c->clipq, 64 * per_region * 4));
-        HIP_TRY(c, ensure(c, c->clipq_cnt, 64 * 16 * 4));
+        const FixedSizes f = fixed_sizes(T, ngrp, tiles);
+        HIP_TRY(c, ensure(c, c->app, f.app));
+        HIP_TRY(c, ensure(c, c->clipq, f.clipq));
+        HIP_TRY(c, ensure(c, c->clipq_cnt, f.clipq_cnt));
+        HIP_TRY(c, ensure(c, c->app_ex, f.app));
+        HIP_TRY(c, ensure(c, c->live, f.live));
+        HIP_TRY(c, ensure(c, c->grp, f.grp));
+        HIP_TRY(c, ensure(c, c->grp_ex, f.grp));
+        HIP_TRY(c, ensure(c, c->scan_tmp, f.scan_tmp));
+        HIP_TRY(c, ensure(c, c->ranges, f.ranges));
+        A.app = (uint2 *)c->app.p;
+        A.app_ex = (uint2 *)c->app_ex.p;
+        A.live = (uint32_t *)c->live.p;
+        A.clipq = (uint32_t *)c->clipq.p;
+        A.clipq_cnt = (uint32_t *)c->clipq_cnt.p;
+        A.grp = (uint2 *)c->grp.p;
+        A.grp_ex = (uint2 *)c->grp_ex.p;
+        A.scan_tmp = c->scan_tmp.p;
+        A.ranges = (uint32_t *)c->ranges.p;
+        c->prim_cap = std::max<uint64_t>(c->prim_cap, T + 256);
+        c->pair_cap = std::max<uint64_t>(c->pair_cap, 2 * T + 4096);
     }
-    HIP_TRY(c, ensure(c, c->app_ex, T * 4));
-    HIP_TRY(c, ensure(c, c->grp, ngrp * 8));
-    HIP_TRY(c, ensure(c, c->grp_ex, ngrp * 8));
-    HIP_TRY(c, ensure(c, c->scan_tmp, (scan_tmp_elems(T) + scan_tmp_elems(ngrp)) * 8));
-    HIP_TRY(c, ensure(c, c->ranges, (size_t)tiles * 8));
-    HIP_TRY(c, ensure(c, c->status, ST_WORDS * 8));
-    c->prim_cap = std::max<uint64_t>(c->prim_cap, T + 256);
-    c->pair_cap = std::max<uint64_t>(c->pair_cap, 2 * T + 4096);
+
+    // the vertex pass: every vertex of attribute 0 shaded once, when the buffer's vertex
+    // count is known (tuning: 0 automatic = when vertices are referenced at least once on
+    // average, 1 on, -1 off)
+    uint32_t vcount = 0;
+    if (!bt && !window_input && c->vs.program != SWR_VS_DUMMY && attr_bytes &&
+        attr_bytes >= need && c->attr[0].stride > 0 && c->tune.reserved[0] >= 0) {
+        const uint64_t nv = (attr_bytes - need) / (uint64_t)c->attr[0].stride + 1;
+        const bool want = c->tune.reserved[0] > 0 || nv <= 3 * (uint64_t)T;
+        if (want && nv < (1ull << 31)) {
+            vcount = (uint32_t)nv;
+            HIP_TRY(c, ensure(c, c->vwin, (size_t)vcount * 16));
+            HIP_TRY(c, ensure(c, c->vmask, (size_t)vcount));
+        }
+    }
 
     for (int attempt = 0; attempt < 4; ++attempt) {
-        HIP_TRY(c, ensure(c, c->prims, c->prim_cap * (size_t)L.stride * 4));
-        HIP_TRY(c, ensure(c, c->bigq, c->prim_cap * 4));
-        HIP_TRY(c, ensure(c, c->pk_a, c->pair_cap * 4));
-        HIP_TRY(c, ensure(c, c->pv_a, c->pair_cap * 4));
-        HIP_TRY(c, ensure(c, c->pk_b, c->pair_cap * 4));
-        HIP_TRY(c, ensure(c, c->pv_b, c->pair_cap * 4));
-        uint32_t G = (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(1, c->pair_cap / 4096));
-        HIP_TRY(c, ensure(c, c->hist, (size_t)256 * G * 4));
-        HIP_TRY(c, ensure(c, c->hist_ex, (size_t)256 * G * 4));
-        HIP_TRY(c, ensure(c, c->hist_tmp, scan_tmp_elems((uint64_t)256 * G) * 4));
+        {
+            HIP_TRY(c, ensure(c, c->prims, c->prim_cap * (size_t)L.stride * 4));
+            HIP_TRY(c, ensure(c, c->bigq, c->prim_cap * 4));
+            HIP_TRY(c, ensure(c, c->pk_a, c->pair_cap * 4));
+            HIP_TRY(c, ensure(c, c->pv_a, c->pair_cap * 4));
+            HIP_TRY(c, ensure(c, c->pk_b, c->pair_cap * 4));
+            HIP_TRY(c, ensure(c, c->pv_b, c->pair_cap * 4));
+            A.G = sort_blocks(c->pair_cap);
+            HIP_TRY(c, ensure(c, c->hist, hist_bytes(A.G)));
+            HIP_TRY(c, ensure(c, c->hist_ex, hist_bytes(A.G)));
+            HIP_TRY(c, ensure(c, c->hist_tmp, hist_tmp_bytes(A.G)));
+            A.prims = (float *)c->prims.p;
+            A.bigq = (uint32_t *)c->bigq.p;
+            A.pk_a = (uint32_t *)c->pk_a.p;
+            A.pv_a = (uint32_t *)c->pv_a.p;
+            A.pk_b = (uint32_t *)c->pk_b.p;
+            A.pv_b = (uint32_t *)c->pv_b.p;
+            A.hist = (uint32_t *)c->hist.p;
+            A.hist_ex = (uint32_t *)c->hist_ex.p;
+            A.hist_tmp = (uint32_t *)c->hist_tmp.p;
+            A.prim_cap = c->prim_cap;
+            A.pair_cap = c->pair_cap;
+        }
+        const uint32_t G = A.G;
 
         GeomArgs g;
         std::memset(&g, 0, sizeof g);
@@ -492,26 +632,32 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
         g.window_input = window_input ? 1 : 0;
         g.wverts = (const swr_rasterizer_vertex *)dwv;
         g.wvert_count = nverts;
-        g.app = (uint32_t *)c->app.p;
-        g.app_ex = (const uint32_t *)c->app_ex.p;
-        g.grp = (uint2 *)c->grp.p;
-        g.grp_ex = (const uint2 *)c->grp_ex.p;
-        g.prims = (float *)c->prims.p;
-        g.prim_cap = c->prim_cap;
-        g.pair_keys = (uint32_t *)c->pk_a.p;
-        g.pair_vals = (uint32_t *)c->pv_a.p;
-        g.pair_cap = c->pair_cap;
+        g.app = A.app;
+        g.app_ex = A.app_ex;
+        g.live = window_input ? nullptr : A.live;
+        g.grp = A.grp;
+        g.grp_ex = A.grp_ex;
+        g.prims = A.prims;
+        g.prim_cap = A.prim_cap;
+        g.pair_keys = A.pk_a;
+        g.pair_vals = A.pv_a;
+        g.pair_cap = A.pair_cap;
         g.status = (unsigned long long *)c->status.p;
-        g.ranges = (uint32_t *)c->ranges.p;
+        g.ranges = A.ranges;
         g.tiles = tiles;
         g.sentinel = pair_sentinel(tiles);
-        g.prim_pb = (uint32_t *)c->bigq.p;
-        g.clipq = (uint32_t *)c->clipq.p;
-        g.clipq_cnt = (uint32_t *)c->clipq_cnt.p;
+        g.prim_pb = A.bigq;
+        g.clipq = A.clipq;
+        g.clipq_cnt = A.clipq_cnt;
         if (bt) {
             g.scenes = (const SceneDev *)bt->scenes.p;
             g.batch_scene = (const uint32_t *)bt->bmap.p;
             g.nscenes = bt->nscenes;
+        }
+        if (vcount) {
+            g.vwin = (float4 *)c->vwin.p;
+            g.vmask = (uint8_t *)c->vmask.p;
+            g.nverts = vcount;
         }
 
         record(c, 0);
@@ -519,25 +665,25 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
         HIP_TRY(c, launch_geom_count(g, c->stream));
         record(c, 1);
         if (!window_input)
-            HIP_TRY(c, scan_fans_and_pairs(g.app, (uint32_t *)c->app_ex.p, T, g.grp,
-                                           (uint2 *)c->grp_ex.p, ngrp, c->scan_tmp.p, g.status,
-                                           c->prim_cap, c->pair_cap, c->stream));
+            HIP_TRY(c, scan_fans_and_pairs(g.app, A.app_ex, T, A.live, g.grp, A.grp_ex, ngrp,
+                                           A.scan_tmp, g.status, A.prim_cap, A.pair_cap,
+                                           c->stream));
         else
-            HIP_TRY(c, scan_pairs(g.grp, (uint2 *)c->grp_ex.p, ngrp, (uint2 *)c->scan_tmp.p,
-                                  g.status, c->prim_cap, c->pair_cap, c->stream));
+            HIP_TRY(c, scan_pairs(g.grp, A.grp_ex, ngrp, (uint2 *)A.scan_tmp, g.status,
+                                  A.prim_cap, A.pair_cap, c->stream));
         record(c, 2);
         HIP_TRY(c, launch_geom_emit(g, c->stream, c->side, c->fork, c->join));
         HIP_TRY(c, launch_expand(g, c->stream, c->tune.expand_chunk));
         record(c, 3);
         SortBuffers sb;
-        sb.keys_a = (uint32_t *)c->pk_a.p;
-        sb.vals_a = (uint32_t *)c->pv_a.p;
-        sb.keys_b = (uint32_t *)c->pk_b.p;
-        sb.vals_b = (uint32_t *)c->pv_b.p;
-        sb.hist = (uint32_t *)c->hist.p;
-        sb.hist_ex = (uint32_t *)c->hist_ex.p;
-        sb.scan_tmp = (uint32_t *)c->hist_tmp.p;
-        sb.ranges = (uint32_t *)c->ranges.p;
+        sb.keys_a = A.pk_a;
+        sb.vals_a = A.pv_a;
+        sb.keys_b = A.pk_b;
+        sb.vals_b = A.pv_b;
+        sb.hist = A.hist;
+        sb.hist_ex = A.hist_ex;
+        sb.scan_tmp = A.hist_tmp;
+        sb.ranges = A.ranges;
         sb.blocks = G;
         HIP_TRY(c, sort_pairs(sb, g.status, tiles, c->stream));
         record(c, 4);
@@ -546,9 +692,9 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
 
         RasterArgs r;
         std::memset(&r, 0, sizeof r);
-        r.prims = (const float *)c->prims.p;
+        r.prims = A.prims;
         r.stride = L.stride;
-        r.ranges = (const uint32_t *)c->ranges.p;
+        r.ranges = A.ranges;
         r.vals = sb.sorted_vals;
         r.status = g.status;
         r.tiles = (int)tiles;
@@ -574,6 +720,7 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
             r.nscenes = g.nscenes;
         }
         r.xcd = c->tune.xcd_order >= 0 ? 1 : 0;
+        r.fold = sync ? 0 : 1;
         if (ps.program == SWR_PS_TEXTURE)
             r.tex = (const TexDev *)static_cast<const swr_texture *>(ps.texture)->desc.p;
         HIP_TRY(c, launch_raster(ps, L, r, c->stream, c->tune.raster_waves ? c->tune.raster_waves : 4,
@@ -595,6 +742,9 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
         }
         break;
     }
+    rc = take_sticky(c);
+    if (rc)
+        return rc;
     rc = check_status(c);
     if (rc)
         return rc;
@@ -713,6 +863,9 @@ int draw_lines_points(swr_context *c, int mode, const int32_t *indices, size_t c
     HIP_TRY(c, hipMemcpyAsync(c->status_host, c->status.p, ST_WORDS * 8, hipMemcpyDeviceToHost,
                               c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    rc = take_sticky(c);
+    if (rc)
+        return rc;
     if (c->status_host[ST_ERROR] & 4ull)
         return fail(c, SWR_ERR_CAPACITY, "a line spans more than %d steps", SWR_MAX_LINE_STEPS);
     rc = check_status(c);
@@ -811,6 +964,8 @@ hipStream_t context_stream(swr_context *c)
 
 int context_fail(swr_context *c, int code, const char *msg) { return fail(c, code, "%s", msg); }
 
+int context_last_code(swr_context *c) { return c->err_code != SWR_OK ? c->err_code : SWR_ERR_HIP; }
+
 bool context_is_device_ptr(const void *p) { return is_device_ptr(p); }
 
 uint64_t *context_diff_tag(swr_context *c) { return c->diff_tag; }
@@ -850,6 +1005,11 @@ void *context_zeroed_scratch(swr_context *c, int slot, size_t bytes)
 {
     DevBuf &b = c->zscratch[slot & 1];
     if (b.bytes < bytes) {
+        if (stream_capturing(c->stream)) {
+            fail(c, SWR_ERR_CAPACITY, "scratch of %zu bytes must grow while the stream is being "
+                                      "captured (run the call once before capturing)", bytes);
+            return nullptr;
+        }
         hipError_t e = ensure(c, b, bytes);
         if (e == hipSuccess)  // ordered before the kernels that use it on this stream
             e = hipMemsetAsync(b.p, 0, b.bytes, c->stream);
@@ -893,6 +1053,17 @@ int swr_create(swr_context **out, int device)
     }
     std::memset(c->status_host, 0, ST_WORDS * 8);
     c->stream = c->own;
+    // the status words start at zero (the sticky word is only ever cleared by the host)
+    if (ensure(c, c->status, ST_WORDS * 8) != hipSuccess ||
+        hipMemsetAsync(c->status.p, 0, c->status.bytes, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+        if (c->status.p)
+            (void)hipFree(c->status.p);
+        (void)hipHostFree(c->status_host);
+        (void)hipStreamDestroy(c->own);
+        delete c;
+        return SWR_ERR_HIP;
+    }
     for (int i = 0; i <= S_COUNT; ++i)
         (void)hipEventCreate(&c->ev[i]);
     // the side stream is optional: without it the clip queue's emit stays on the draw stream
@@ -912,11 +1083,13 @@ int swr_destroy(swr_context *c)
         return SWR_ERR_INVALID_ARG;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    DevBuf *bufs[] = {&c->idx, &c->attr0, &c->wverts, &c->app, &c->app_ex, &c->grp, &c->grp_ex,
+    DevBuf *bufs[] = {&c->idx, &c->attr0, &c->wverts, &c->app, &c->app_ex, &c->live, &c->grp,
+                      &c->grp_ex,
                       &c->scan_tmp, &c->prims, &c->pk_a, &c->pv_a, &c->pk_b, &c->pv_b,
                       &c->ranges, &c->hist, &c->hist_ex, &c->hist_tmp, &c->status, &c->tgt[0],
                       &c->tgt[1], &c->tgt[2], &c->log, &c->logcount, &c->bigq, &c->clipq,
-                      &c->clipq_cnt, &c->zscratch[0], &c->zscratch[1], &c->diff_table};
+                      &c->clipq_cnt, &c->vwin, &c->vmask, &c->zscratch[0], &c->zscratch[1],
+                      &c->diff_table};
     for (DevBuf *b : bufs)
         if (b->p)
             (void)hipFree(b->p);
@@ -958,7 +1131,8 @@ int swr_synchronize(swr_context *c)
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (c->async_pending) {
         c->async_pending = false;
-        int rc = check_status(c);
+        // each asynchronous draw folded its own failures into the sticky word
+        int rc = take_sticky(c);
         if (rc)
             return rc;
         collect_stats(c);

@@ -49,7 +49,11 @@ constexpr int ST_ERROR = 3;        // bit 0: index range, bit 1: clip overflow
 constexpr int ST_BIG = 4;          // primitives queued for cooperative pair expansion
 constexpr int ST_KEPT = 5;         // pairs left after dropping rejected blocks / tiles
 constexpr int ST_CLIPQ = 6;        // triangles queued for the clipper
-constexpr int ST_WORDS = 8;
+constexpr int ST_LIVE = 8;         // triangles the emit pass walks (the live list)
+constexpr int ST_STICKY = 7;       // failures of asynchronous draws not yet reported: bit 0
+                                   // overflow, bits 1.. the ST_ERROR bits; never cleared by
+                                   // a draw, only by the host once it has reported them
+constexpr int ST_WORDS = 16;       // a draw clears every word but ST_STICKY
 
 // record word 6: index of the primitive's first (tile, primitive) pair
 constexpr int REC_PAIRBASE = 6;
@@ -210,8 +214,9 @@ struct GeomArgs {
     const swr_rasterizer_vertex *wverts;
     uint64_t wvert_count;
     // scan buffers
-    uint32_t *app;              // [tris] appended fan triangles per input triangle
-    const uint32_t *app_ex;     // exclusive scan of app
+    uint2 *app;                 // [tris] (fan triangles appended, live) per input triangle
+    const uint2 *app_ex;        // exclusive scan of app
+    uint32_t *live;             // [tris] the live triangles in order (nullptr: emit walks all)
     uint2 *grp;                 // [batches*2048] (prims, pairs) per (batch, phase, j)
     const uint2 *grp_ex;        // exclusive scan of grp
     // outputs
@@ -227,6 +232,11 @@ struct GeomArgs {
     uint32_t *prim_pb;          // [prim_cap] index of each primitive's first pair
     uint32_t *clipq;            // [tris + 64*256] triangles that straddle a clip plane
     uint32_t *clipq_cnt;        // [64 * 16] per-region queue counters
+    // post-transform vertices (the vertex pass; vwin == nullptr: off): window-space
+    // (x, y, z, clip w) and the clip mask of each vertex [0, nverts) of attribute 0
+    float4 *vwin;
+    uint8_t *vmask;
+    uint32_t nverts;
     // batched draw: per-scene state replaces the fields above (nscenes == 0: single draw)
     const SceneDev *scenes;
     const uint32_t *batch_scene;  // [batches] scene of each virtual batch
@@ -263,7 +273,17 @@ struct RasterArgs {
     uint32_t nscenes;
     int xcd;                    // 1: tile blocks grouped per XCD
     const TexDev *tex;          // SWR_PS_TEXTURE
+    int fold;                   // 1: an asynchronous draw; its failures go to ST_STICKY
 };
+
+// An asynchronous draw's failures, kept for the next swr_synchronize: once every earlier
+// kernel of the draw has finished (stream order), its overflow and error words are final.
+__device__ inline void fold_status(unsigned long long *st)
+{
+    const unsigned long long f = (st[ST_OVERFLOW] ? 1ull : 0ull) | (st[ST_ERROR] << 1);
+    if (f)
+        atomicOr(&st[ST_STICKY], f);
+}
 
 // Scene-local state of a batched draw over the single-draw arguments.
 __device__ inline void apply_scene(GeomArgs &a, const SceneDev &s)

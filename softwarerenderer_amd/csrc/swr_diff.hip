@@ -1453,7 +1453,7 @@ static int diff_run(swr_context *c, const swr_diff_desc *dd, const float *gimage
         const unsigned char *dev =
             static_cast<const unsigned char *>(context_diff_table(c, tab.data(), tab.size(), &table_gen));
         if (!dev)
-            return SWR_ERR_HIP;
+            return context_last_code(c);
         d.vrow = reinterpret_cast<const long long *>(dev);
         d.vtri = reinterpret_cast<const int2 *>(dev + (size_t)8 * d.V);
         d.svert = reinterpret_cast<const int *>(dev + (size_t)16 * d.V);
@@ -1464,7 +1464,7 @@ static int diff_run(swr_context *c, const swr_diff_desc *dd, const float *gimage
         // per-view partials: views do not contend on one vertex's attribute gradient
         d.gattr = (float *)context_zeroed_scratch(c, 0, rows * q.channels * 4);
         if (!d.gattr)
-            return SWR_ERR_HIP;
+            return context_last_code(c);
     }
     // The forward stores its pair decisions; a backward over the same buffers and sizes
     // reuses them instead of walking the silhouettes again.
@@ -1477,7 +1477,7 @@ static int diff_run(swr_context *c, const swr_diff_desc *dd, const float *gimage
         return context_fail(c, SWR_ERR_CAPACITY, "too many pixels for one swr_diff call");
     uint8_t *pbuf = (uint8_t *)context_zeroed_scratch(c, 1, off_e + tiles * 256 * 4);
     if (!pbuf)
-        return SWR_ERR_HIP;
+        return context_last_code(c);
     d.qcnt = reinterpret_cast<uint32_t *>(pbuf + off_t) + tiles;
     d.ecnt = reinterpret_cast<uint32_t *>(pbuf + off_t) + 2 * tiles;
     d.pflag = pbuf + off_f;

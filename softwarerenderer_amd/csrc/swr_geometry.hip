@@ -11,9 +11,15 @@
 //   triangle 0 (written in place, VertexProcessor.cpp:254-256), then the fan triangles
 //   k >= 1 of every triangle in j order (appended, VertexProcessor.cpp:257-261).
 //
-// The vertex cache of the reference (VertexCache.h) maps an index to an already shaded
-// output vertex within a batch.  Vertex programs are pure, so re-running the program for
-// every reference to a vertex yields the same bits; the cache is dropped.
+// The vertex cache of the reference (VertexCache.h:29-63, used at VertexProcessor.cpp:86-117)
+// maps an index to an already shaded output vertex within a batch.  Vertex programs are
+// pure, so the results do not depend on which reference to a vertex shades it.  Here the
+// vertex pass (k_vertex) shades every vertex of the attribute buffer once, in index order,
+// into post-transform arrays (window-space position and clip mask), which both geometry
+// passes read by index; the emit pass fetches only the varyings the vertex program copies.
+// Triangles with a vertex outside a clip plane (clip mask != 0) re-run the program for the
+// clip-space position the clipper needs.  Without the vertex pass (a batch of scenes, an
+// attribute buffer of unknown size, or the tuning's choice) every corner runs the program.
 #include "swr_common.h"
 #include "swr_internal.h"
 #include "swr_vertex.h"
@@ -417,6 +423,7 @@ struct FanCtx {
     uint32_t slot1;                 // batch-local slot of fan triangle 1
     uint64_t pos0, pairs0, pos1, pairs1;
     uint32_t cnt0, cntp0, cnt1, cntp1;
+    uint32_t app_n;                 // count pass: fan triangles appended (k >= 1)
 };
 
 // One fan triangle (p0, p[k+1], p[k+2]): transform, cull, setup, count or emit.
@@ -586,7 +593,7 @@ __device__ inline void clip_path(const GeomArgs &a, FanCtx &c, int cm, const Pos
         n = m;
     }
     if (!EMIT)
-        a.app[c.t] = n > 3 ? (uint32_t)(n - 3) : 0u;
+        c.app_n = n > 3 ? (uint32_t)(n - 3) : 0u;
     for (int k = 0; k + 2 < n; ++k) {
         const int ia = N.g(cur, 0), ib = N.g(cur, k + 1), ic = N.g(cur, k + 2);
         float va[NV > 0 ? NV : 1], vb[NV > 0 ? NV : 1], vc[NV > 0 ? NV : 1];
@@ -888,7 +895,13 @@ __device__ __forceinline__ int clip_two_planes(const GeomArgs &a, int cm, const 
 // CLIPQ = false: the main pass over all input triangles; triangles that need the clipper
 // go to a queue (count pass) or are left to the queue kernel (emit pass).
 // CLIPQ = true: one queued triangle, clipped.
-template <int NV, bool EMIT, bool CLIPQ = false>
+// LISTED: the emit pass over the compacted list of triangles the count pass found live (a
+// workgroup's triangles may then come from several batches and scenes).
+//
+// The count pass writes app[t] = (fan triangles appended, live): live when the main pass
+// itself emits a primitive for t (a triangle the clip queue takes is not live).  Their scan
+// gives the appended fans' draw-order slots and, with the live flags, the compacted list.
+template <int NV, bool EMIT, bool CLIPQ = false, bool LISTED = false>
 __device__ void process_triangle(const GeomArgs &a0, uint32_t t)
 {
     GeomArgs a = a0;
@@ -902,11 +915,12 @@ __device__ void process_triangle(const GeomArgs &a0, uint32_t t)
     c.slot1 = 0;
     c.pos0 = c.pairs0 = c.pos1 = c.pairs1 = 0;
     c.cnt0 = c.cntp0 = c.cnt1 = c.cntp1 = 0;
+    c.app_n = 0;
     uint32_t ntris = a.tris, nbatches = a.batches;  // extent of this triangle's draw
     bool live = true;
     if (a0.nscenes) {
         uint32_t sid = a0.batch_scene[c.b];
-        if (!CLIPQ)  // a 256-thread block lies inside one 1024-triangle batch
+        if (!CLIPQ && !LISTED)  // a 256-thread block lies inside one 1024-triangle batch
             sid = __builtin_amdgcn_readfirstlane(sid);
         const SceneDev &S = a0.scenes[sid];
         apply_scene(a, S);
@@ -920,7 +934,7 @@ __device__ void process_triangle(const GeomArgs &a0, uint32_t t)
     const uint32_t nbt = (c.kb + 1 < nbatches) ? (uint32_t)BATCH : ntris - c.kb * (uint32_t)BATCH;
     if (!live) {
         if (!EMIT) {
-            a.app[t] = 0;
+            a.app[t] = make_uint2(0, 0);
             a.grp[(size_t)c.b * 2048 + c.j] = make_uint2(0, 0);
             a.grp[(size_t)c.b * 2048 + 1024 + c.j] = make_uint2(0, 0);
         }
@@ -928,7 +942,7 @@ __device__ void process_triangle(const GeomArgs &a0, uint32_t t)
     }
     if (EMIT) {
         if (!a.window_input)
-            c.slot1 = nbt + (a.app_ex[t] - a.app_ex[c.b * BATCH]);
+            c.slot1 = nbt + (a.app_ex[t].x - a.app_ex[c.b * BATCH].x);
         uint2 g0 = a.grp_ex[(size_t)c.b * 2048 + c.j];
         uint2 g1 = a.grp_ex[(size_t)c.b * 2048 + 1024 + c.j];
         c.pos0 = g0.x;
@@ -984,10 +998,30 @@ __device__ void process_triangle(const GeomArgs &a0, uint32_t t)
                 bad = true;
         if (bad) {
             atomicOr(&a.status[ST_ERROR], 1ull);
-            if (!EMIT)
-                a.app[t] = 0;
         } else {
+            int cmv = -1;  // clip mask from the post-transform vertices
+            if (!CLIPQ && a.vwin)
+                cmv = a.vmask[ids[0]] | a.vmask[ids[1]] | a.vmask[ids[2]];
             Pos4 P[3];
+            if (cmv == 0) {
+                // inside every plane: the shaded window-space vertices as they are
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const float4 w = a.vwin[ids[q]];
+                    W[q].x = w.x;
+                    W[q].y = w.y;
+                    W[q].z = w.z;
+                    W[q].w = w.w;
+                    if (EMIT) {
+                        fetch_vertex<NV, false>(a, ids[q], P[q], var3[q]);
+#pragma unroll
+                        for (int k = 0; k < NVS; ++k)
+                            V[q][k] = var3[q][k];
+                    }
+                }
+                nfan = 1;
+                goto fans;
+            }
 #pragma unroll
             for (int q = 0; q < 3; ++q)
                 fetch_vertex<EMIT ? NV : 0>(a, ids[q], P[q], var3[q]);
@@ -1001,8 +1035,7 @@ __device__ void process_triangle(const GeomArgs &a0, uint32_t t)
                     r2 = clip_two_planes<NVS, EMIT>(a, cm, P, var3, W, V);
                 if (r2 >= 0) {
                     nfan = r2;
-                    if (!EMIT)
-                        a.app[t] = nfan > 1 ? (uint32_t)(nfan - 1) : 0u;
+                    c.app_n = nfan > 1 ? (uint32_t)(nfan - 1) : 0u;
                 } else {
                     clip_path<NVS, EMIT>(a, c, cm, P, var3);
                 }
@@ -1027,10 +1060,11 @@ __device__ void process_triangle(const GeomArgs &a0, uint32_t t)
                 }
                 return;
             }
-            if (!EMIT && !CLIPQ)
-                a.app[t] = nfan == 2 ? 1u : 0u;
+            if (!CLIPQ)
+                c.app_n = nfan == 2 ? 1u : 0u;
         }
     }
+fans:
     // fan triangles (p0, p[k+1], p[k+2]), VertexProcessor.cpp:254-261; one call site so the
     // vertex arrays stay in registers
 #pragma unroll 1
@@ -1048,13 +1082,15 @@ __device__ void process_triangle(const GeomArgs &a0, uint32_t t)
     if (!EMIT) {
         a.grp[(size_t)c.b * 2048 + c.j] = make_uint2(c.cnt0, c.cntp0);
         a.grp[(size_t)c.b * 2048 + 1024 + c.j] = make_uint2(c.cnt1, c.cntp1);
+        if (!a.window_input)
+            a.app[t] = make_uint2(c.app_n, (!CLIPQ && c.cnt0 + c.cnt1 > 0) ? 1u : 0u);
     }
 }
 
 __global__ void __launch_bounds__(256) k_begin(GeomArgs a)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < ST_WORDS)
+    if (i < ST_WORDS && i != ST_STICKY)
         a.status[i] = 0;
     if (i < CLIPQ_REGIONS)
         a.clipq_cnt[i * CLIPQ_STRIDE] = 0;
@@ -1068,6 +1104,23 @@ __global__ void __launch_bounds__(256) k_begin(GeomArgs a)
         a.grp[(size_t)bb * 2048 + jj] = make_uint2(0, 0);
         a.grp[(size_t)bb * 2048 + 1024 + jj] = make_uint2(0, 0);
     }
+}
+
+// The vertex pass: every vertex of attribute 0 shaded once (VertexProcessor.cpp:134-150 with
+// the built-in program), its clip mask (VertexProcessor.cpp:122-132) and its window-space
+// position (VertexProcessor.cpp:364-373; meaningful where the mask is 0, which is the only
+// case that reads it).
+__global__ void __launch_bounds__(256) k_vertex(GeomArgs a)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= a.nverts)
+        return;
+    Pos4 P;
+    float none[1];
+    fetch_vertex<0>(a, (int32_t)v, P, none);
+    const WinV w = to_window(a, P);
+    a.vwin[v] = make_float4(w.x, w.y, w.z, w.w);
+    a.vmask[v] = (uint8_t)clip_mask(P);
 }
 
 __global__ void __launch_bounds__(256) k_geom_count(GeomArgs a)
@@ -1085,6 +1138,17 @@ __global__ void __launch_bounds__(256) k_geom_emit(GeomArgs a)
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < a.tris)
         process_triangle<NV, true>(a, t);
+}
+
+// The emit pass over the live triangles only, in draw order (the list the scan compacted).
+template <int NV>
+__global__ void __launch_bounds__(256) k_geom_emit_listed(GeomArgs a)
+{
+    if (a.status[ST_OVERFLOW] | a.status[ST_ERROR])
+        return;
+    const uint32_t n = (uint32_t)a.status[ST_LIVE];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        process_triangle<NV, true, false, true>(a, a.live[i]);
 }
 
 // The queued triangles that straddle a clip plane, densely packed (order does not matter:
@@ -1159,6 +1223,8 @@ hipError_t launch_geom_count(const GeomArgs &a, hipStream_t s)
 {
     if (!a.tris)
         return hipSuccess;
+    if (a.vwin && a.nverts)
+        hipLaunchKernelGGL(k_vertex, dim3((a.nverts + 255) / 256), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_geom_count, dim3((a.tris + 255) / 256), dim3(256), 0, s, a);
     if (!a.window_input)
         hipLaunchKernelGGL(k_clip_count, dim3(clip_blocks(a.tris)), dim3(CLIPQ_THREADS), 0, s, a);
@@ -1204,7 +1270,12 @@ static void emit_kernels(const GeomArgs &a, hipStream_t s, hipStream_t side, boo
     dim3 g((a.tris + 255) / 256), blk(256), gq(clip_blocks(a.tris)), bq(CLIPQ_THREADS);
     if (q)
         hipLaunchKernelGGL(k_clip_emit<NV == 0 ? 1 : NV>, gq, bq, 0, side ? side : s, a);
-    hipLaunchKernelGGL(k_geom_emit<NV>, g, blk, 0, s, a);
+    if (a.live) {  // at most 2048 workgroups walk the live list
+        const dim3 gl(g.x < 2048 ? g.x : 2048);
+        hipLaunchKernelGGL(k_geom_emit_listed<NV>, gl, blk, 0, s, a);
+    } else {
+        hipLaunchKernelGGL(k_geom_emit<NV>, g, blk, 0, s, a);
+    }
 }
 
 hipError_t launch_geom_emit(const GeomArgs &a, hipStream_t s, hipStream_t side, hipEvent_t fork,

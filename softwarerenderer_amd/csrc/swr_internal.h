@@ -49,9 +49,10 @@ hipError_t scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *tmp
 hipError_t scan_pairs(const uint2 *in, uint2 *out, uint64_t n, uint2 *tmp,
                       unsigned long long *status, uint64_t cap_prims, uint64_t cap_pairs,
                       hipStream_t s);
-// scan_u32 of the appended-fan counts and scan_pairs of the slot counts in shared launches;
-// tmp holds scan_tmp_elems(na) + scan_tmp_elems(ng) 8-byte elements
-hipError_t scan_fans_and_pairs(const uint32_t *app, uint32_t *app_ex, uint64_t na,
+// the (appended fans, live) scan of the input triangles, compacting the live ones into
+// `live`, and scan_pairs of the slot counts, in shared launches; tmp holds
+// scan_tmp_elems(na) + scan_tmp_elems(ng) 8-byte elements
+hipError_t scan_fans_and_pairs(const uint2 *app, uint2 *app_ex, uint64_t na, uint32_t *live,
                                const uint2 *grp, uint2 *grp_ex, uint64_t ng, void *tmp,
                                unsigned long long *status, uint64_t cap_prims,
                                uint64_t cap_pairs, hipStream_t s);
@@ -65,6 +66,8 @@ uint32_t pair_sentinel(uint32_t tiles);
 // context access for the kernel files' own entry points (swr_api.cpp)
 hipStream_t context_stream(swr_context *c);
 int context_fail(swr_context *c, int code, const char *msg);
+// The status code of the context's last failure (SWR_ERR_HIP if none was recorded).
+int context_last_code(swr_context *c);
 bool context_is_device_ptr(const void *p);
 // Device buffer `slot` (0 or 1) of at least `bytes` owned by the context, zero when first
 // handed out at this size.  nullptr (and the context's error set) on failure.

@@ -267,7 +267,7 @@ __device__ inline unsigned long long wave_sum(unsigned long long v)
 __global__ void __launch_bounds__(256) k_lines_begin(LineArgs A)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < ST_WORDS)
+    if (i < ST_WORDS && i != ST_STICKY)
         A.g.status[i] = 0ull;
     for (uint64_t k = i; k < 2ull * A.g.tiles; k += (uint64_t)gridDim.x * blockDim.x)
         A.g.ranges[k] = 0u;

@@ -5,6 +5,7 @@
 // runs the reference's drawPixel for one invocation, store() writes back what changed.
 #pragma once
 
+#include "swr_chain.h"
 #include "swr_common.h"
 
 #pragma clang fp contract(off)
@@ -68,10 +69,7 @@ __device__ inline float chain_at(const float *rec, int p, int x, int y, int sc_m
     }
     int xl = 0, xr = 0;
     span_row_extent(rec, flags, y, sc_minx, sc_maxx, xl, xr);
-    float v = pa * ((float)xl + 0.5f) + pb * ((float)y + 0.5f) + pc;
-    for (int i = xl; i < x; ++i)
-        v = v + pa;
-    return v;
+    return chain_add(pa * ((float)xl + 0.5f) + pb * ((float)y + 0.5f) + pc, pa, x - xl);
 }
 
 // ---- pixel programs (include/swr_hip.h) ---------------------------------------------------

@@ -11,7 +11,9 @@
 //     values alike.  The corner classification (Rasterizer.h:270-304) was done at binning
 //     and rides in the pair value.
 //   Span (PixelShaderBase.h:96-112): value(x) = init(xl+.5, y+.5) then +a per pixel from
-//     xl, with [xl, xr) from the flat-part edge walk (Rasterizer.h:370-405).
+//     xl, with [xl, xr) from the flat-part edge walk (Rasterizer.h:370-405); the additions
+//     before the tile's first column are fast-forwarded (chain_add, swr_chain.h), so a
+//     tile's cost does not grow with its distance from the span's start.
 //   w = 1/invw and pvar = pvarTemp * w at the pixel (PixelData.h:84-103).
 //
 // Work split.  The chains are independent of draw order, only shading is not.  So the
@@ -75,6 +77,8 @@ __global__ void __launch_bounds__(512) k_raster(RasterArgs a0)
     __shared__ float s_rows[MAXW][GS * 64];
     __shared__ unsigned long long s_cov[MAXW][GS];
 
+    if (a0.fold && blockIdx.x == 0 && threadIdx.x == 0)
+        fold_status(const_cast<unsigned long long *>(a0.status));
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wpb = (int)(blockDim.x >> 6);
     const unsigned blk = a0.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
@@ -195,9 +199,8 @@ __global__ void __launch_bounds__(512) k_raster(RasterArgs a0)
                     for (int i = 1; i < 8; ++i)
                         out[i] = out[i - 1] + pa;
                 } else {
-                    float v = pa * xs + pb * ys + pc;
-                    for (int k = 0; k < pre; ++k)
-                        v = v + pa;
+                    // init at xl, then the pre additions up to the tile's first column
+                    float v = chain_add(pa * xs + pb * ys + pc, pa, pre);
 #pragma unroll
                     for (int i = 0; i < 8; ++i) {
                         out[i] = v;

@@ -184,11 +184,14 @@ __device__ T block_carry(const T *sums, uint32_t b)
 
 // Writes the exclusive scan.  CARRY: the block adds up the block sums before it (the
 // two-kernel scan); else sums[] holds their exclusive scan.  With `status` set the thread
-// that owns the last element publishes the total (publish_totals).
+// that owns the last element publishes the total (publish_totals).  With `list` (the
+// (appended fans, live) scan of the geometry pass): element i with a live flag writes i at
+// position (its exclusive live count), compacting the live triangles in order, and the
+// live total goes to status[ST_LIVE].
 template <typename T, bool CARRY>
 __device__ void down_tile(const T *in, uint64_t n, const T *sums, T *out,
                           unsigned long long *status, int word, uint64_t cap_x, uint64_t cap_y,
-                          uint32_t b)
+                          uint32_t b, uint32_t *list = nullptr)
 {
     const T carry = CARRY ? block_carry<T>(sums, b) : sums[b];
     const uint64_t base = (uint64_t)b * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
@@ -204,7 +207,15 @@ __device__ void down_tile(const T *in, uint64_t n, const T *sums, T *out,
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i) {
         o[i] = run;
-        if (status && base + i == n - 1)
+        if constexpr (sizeof(T) == 8) {
+            if (list) {
+                if (base + i < n && vals[i].y)
+                    list[run.y] = (uint32_t)(base + i);
+                if (base + i == n - 1)
+                    status[ST_LIVE] = run.y + vals[i].y;
+            }
+        }
+        if (status && !list && base + i == n - 1)
             publish_totals(sadd(run, vals[i]), status, word, cap_x, cap_y);
         run = sadd(run, vals[i]);
     }
@@ -215,31 +226,32 @@ template <typename T, bool CARRY>
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(const T *in, uint64_t n,
                                                           const T *sums, T *out,
                                                           unsigned long long *status, int word,
-                                                          uint64_t cap_x, uint64_t cap_y)
+                                                          uint64_t cap_x, uint64_t cap_y,
+                                                          uint32_t *list)
 {
-    down_tile<T, CARRY>(in, n, sums, out, status, word, cap_x, cap_y, blockIdx.x);
+    down_tile<T, CARRY>(in, n, sums, out, status, word, cap_x, cap_y, blockIdx.x, list);
 }
 
-// The per-triangle appended-fan counts (u32) and the per-slot (primitives, pairs) counts
-// (uint2) scanned together: blocks [0, nb_a) take the first, the rest the second.
-__global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce2(const uint32_t *a, uint64_t na,
-                                                             uint32_t *sa, uint32_t nba,
+// The per-triangle (appended fans, live) counts and the per-slot (primitives, pairs) counts
+// scanned together: blocks [0, nb_a) take the first, the rest the second.
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce2(const uint2 *a, uint64_t na,
+                                                             uint2 *sa, uint32_t nba,
                                                              const uint2 *g, uint64_t ng,
                                                              uint2 *sg)
 {
     if (blockIdx.x < nba)
-        reduce_tile<uint32_t>(a, na, sa, blockIdx.x);
+        reduce_tile<uint2>(a, na, sa, blockIdx.x);
     else
         reduce_tile<uint2>(g, ng, sg, blockIdx.x - nba);
 }
 
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_down2(
-    const uint32_t *a, uint64_t na, const uint32_t *sa, uint32_t *oa, uint32_t nba, const uint2 *g,
-    uint64_t ng, const uint2 *sg, uint2 *og, unsigned long long *status, uint64_t cap_x,
-    uint64_t cap_y)
+    const uint2 *a, uint64_t na, const uint2 *sa, uint2 *oa, uint32_t nba, uint32_t *list,
+    const uint2 *g, uint64_t ng, const uint2 *sg, uint2 *og, unsigned long long *status,
+    uint64_t cap_x, uint64_t cap_y)
 {
     if (blockIdx.x < nba)
-        down_tile<uint32_t, true>(a, na, sa, oa, nullptr, 0, 0, 0, blockIdx.x);
+        down_tile<uint2, true>(a, na, sa, oa, status, 0, 0, 0, blockIdx.x, list);
     else
         down_tile<uint2, true>(g, ng, sg, og, status, 0, cap_x, cap_y, blockIdx.x - nba);
 }
@@ -250,7 +262,8 @@ constexpr uint32_t SCAN_CARRY_BLOCKS = 1024;
 
 template <typename T>
 hipError_t scan_impl(const T *in, T *out, uint64_t n, T *tmp, unsigned long long *status,
-                     int word, uint64_t cap_x, uint64_t cap_y, hipStream_t s)
+                     int word, uint64_t cap_x, uint64_t cap_y, hipStream_t s,
+                     uint32_t *list = nullptr)
 {
     if (!n)
         return hipSuccess;
@@ -258,11 +271,11 @@ hipError_t scan_impl(const T *in, T *out, uint64_t n, T *tmp, unsigned long long
     hipLaunchKernelGGL(k_scan_reduce<T>, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, tmp);
     if (nb <= SCAN_CARRY_BLOCKS) {
         hipLaunchKernelGGL((k_scan_down<T, true>), dim3(nb), dim3(SCAN_THREADS), 0, s, in, n,
-                           tmp, out, status, word, cap_x, cap_y);
+                           tmp, out, status, word, cap_x, cap_y, list);
     } else {
         hipLaunchKernelGGL(k_scan_sums<T>, dim3(1), dim3(SCAN_THREADS), 0, s, tmp, nb);
         hipLaunchKernelGGL((k_scan_down<T, false>), dim3(nb), dim3(SCAN_THREADS), 0, s, in, n,
-                           tmp, out, status, word, cap_x, cap_y);
+                           tmp, out, status, word, cap_x, cap_y, list);
     }
     return hipGetLastError();
 }
@@ -281,18 +294,18 @@ hipError_t scan_pairs(const uint2 *in, uint2 *out, uint64_t n, uint2 *tmp,
     return scan_impl<uint2>(in, out, n, tmp, status, 0, cap_prims, cap_pairs, s);
 }
 
-hipError_t scan_fans_and_pairs(const uint32_t *app, uint32_t *app_ex, uint64_t na,
+hipError_t scan_fans_and_pairs(const uint2 *app, uint2 *app_ex, uint64_t na, uint32_t *live,
                                const uint2 *grp, uint2 *grp_ex, uint64_t ng, void *tmp,
                                unsigned long long *status, uint64_t cap_prims,
                                uint64_t cap_pairs, hipStream_t s)
 {
     const uint32_t nba = (uint32_t)((na + SCAN_TILE - 1) / SCAN_TILE);
     const uint32_t nbg = (uint32_t)((ng + SCAN_TILE - 1) / SCAN_TILE);
-    // block sums: u32 of the first scan, then (8-byte aligned) uint2 of the second
-    uint32_t *sa = static_cast<uint32_t *>(tmp);
-    uint2 *sg = reinterpret_cast<uint2 *>(sa + ((nba + 1) & ~1u));
+    // block sums: those of the first scan, then those of the second
+    uint2 *sa = static_cast<uint2 *>(tmp);
+    uint2 *sg = sa + scan_tmp_elems(na);
     if (!na || !ng || nba > SCAN_CARRY_BLOCKS || nbg > SCAN_CARRY_BLOCKS) {
-        hipError_t e = scan_u32(app, app_ex, na, sa, s);
+        hipError_t e = scan_impl<uint2>(app, app_ex, na, sa, status, 0, 0, 0, s, live);
         if (e != hipSuccess)
             return e;
         return scan_pairs(grp, grp_ex, ng, sg, status, cap_prims, cap_pairs, s);
@@ -300,7 +313,7 @@ hipError_t scan_fans_and_pairs(const uint32_t *app, uint32_t *app_ex, uint64_t n
     hipLaunchKernelGGL(k_scan_reduce2, dim3(nba + nbg), dim3(SCAN_THREADS), 0, s, app, na, sa,
                        nba, grp, ng, sg);
     hipLaunchKernelGGL(k_scan_down2, dim3(nba + nbg), dim3(SCAN_THREADS), 0, s, app, na, sa,
-                       app_ex, nba, grp, ng, sg, grp_ex, status, cap_prims, cap_pairs);
+                       app_ex, nba, live, grp, ng, sg, grp_ex, status, cap_prims, cap_pairs);
     return hipGetLastError();
 }
 

@@ -32,8 +32,9 @@ __device__ inline void mvp_apply(const float *m, float x, float y, float z, Pos4
     o.w = r[3];
 }
 
-// Carried varyings: var[0 .. lay.na) avar, var[lay.na .. lay.na+lay.np) pvar.
-template <int NV>
+// Carried varyings: var[0 .. lay.na) avar, var[lay.na .. lay.na+lay.np) pvar.  POS = false:
+// the varyings only (the position comes from the post-transform vertices).
+template <int NV, bool POS = true>
 __device__ inline void fetch_vertex(const GeomArgs &a, int32_t index, Pos4 &p, float *var)
 {
     p.x = p.y = p.z = p.w = 0.0f;
@@ -74,7 +75,8 @@ __device__ inline void fetch_vertex(const GeomArgs &a, int32_t index, Pos4 &p, f
                 var[k] = src[3 + k];
         break;
     case SWR_VS_MVP_POS_RGB_UV:
-        mvp_apply(a.mvp, src[0], src[1], src[2], p);
+        if (POS)
+            mvp_apply(a.mvp, src[0], src[1], src[2], p);
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
             if (k < na && k < 3)
@@ -84,7 +86,8 @@ __device__ inline void fetch_vertex(const GeomArgs &a, int32_t index, Pos4 &p, f
         }
         break;
     case SWR_VS_MVP_POS_UV:
-        mvp_apply(a.mvp, src[0], src[1], src[2], p);
+        if (POS)
+            mvp_apply(a.mvp, src[0], src[1], src[2], p);
 #pragma unroll
         for (int k = 0; k < NV; ++k)
             if (k >= na && k < na + np && k - na < 2)

@@ -152,6 +152,27 @@ def make_scenes(ref):
     return out
 
 
+# ---- the full 1M-triangle 1080p soup (the benchmark's frame) --------------------------------
+FRAME1M = dict(ntri=1_000_000, seed=2)
+
+
+def make_frame1m(ref):
+    """Digests of the benchmark's frame drawn by the reference: the depth-checker program
+    (RGBA8 + depth) and the order-sensitive hash program (hash + per-pixel count), Block and
+    Span."""
+    out = {}
+    soup = scenes.soup_scene(FRAME1M["ntri"], seed=FRAME1M["seed"])
+    out["input"] = scene_digest(soup)
+    for mode in (O.RASTER_SPAN, O.RASTER_BLOCK):
+        for prog in (O.PS_DEPTH_CHECKER, O.PS_HASH):
+            pipe = O.Pipeline(1920, 1080, raster_mode=mode, mvp=soup.mvp, ps_program=prog)
+            t = images(ref, pipe, soup)
+            out[f"m{mode}_p{prog}"] = digest(t.t0, t.t1)
+            if prog == O.PS_HASH:
+                out[f"m{mode}_frags"] = int(t.t1.astype(np.int64).sum())
+    return out
+
+
 # ---- invocation logs of small scenes ----------------------------------------------------------
 LOG_KEEP = 1500  # records stored in full; the rest of a log is pinned by its digest
 
@@ -274,6 +295,10 @@ def main():
     if not O.ref_available():
         O.build()
     ref = O.CpuRenderer("ref")
+    if sys.argv[1:] == ["frame1m"]:  # only the 1M-triangle frame's digests
+        np.savez_compressed(os.path.join(HERE, "frame1m.npz"),
+                            **{k: np.array(v) for k, v in make_frame1m(ref).items()})
+        return
     np.savez_compressed(os.path.join(HERE, "setup.npz"), **make_setup(ref))
     np.savez_compressed(os.path.join(HERE, "clip.npz"), **make_clip(ref))
     sc = make_scenes(ref)
@@ -282,6 +307,8 @@ def main():
     np.savez_compressed(os.path.join(HERE, "logs.npz"), **make_logs(ref))
     np.savez_compressed(os.path.join(HERE, "lines.npz"), **make_lines(ref))
     np.savez_compressed(os.path.join(HERE, "obj.npz"), **make_obj())
+    np.savez_compressed(os.path.join(HERE, "frame1m.npz"),
+                        **{k: np.array(v) for k, v in make_frame1m(ref).items()})
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

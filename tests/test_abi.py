@@ -4,6 +4,7 @@ import ctypes as C
 import os
 import re
 import subprocess
+import tempfile
 
 import pytest
 
@@ -29,8 +30,9 @@ def test_library_exports_every_declared_symbol():
 
 
 def _c_layout(code: str) -> dict:
-    src = os.path.join("/tmp", "swr_layout_probe.c")
-    exe = os.path.join("/tmp", "swr_layout_probe")
+    tmp = tempfile.mkdtemp(prefix="swr_layout_")
+    src = os.path.join(tmp, "probe.c")
+    exe = os.path.join(tmp, "probe")
     with open(src, "w") as f:
         f.write('#include <stdio.h>\n#include <stddef.h>\n#include "swr_hip.h"\n'
                 '#include "swr_oracle.h"\nint main(void){\n' + code + "return 0;}\n")

@@ -17,7 +17,7 @@ import pytest
 
 import oracle as O
 from softwarerenderer_amd import scenes
-from helpers import run_cpu, run_gpu, sorted_log
+from helpers import assert_same, run_cpu, run_gpu, sorted_log
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 MODES = [O.RASTER_SPAN, O.RASTER_BLOCK, O.RASTER_ADAPTIVE]
@@ -101,6 +101,47 @@ def test_scene_digests(cpu_oracle, scene_cases):
         t = run_cpu(cpu_oracle, pipe, sc)
         assert digest(t.t0, t.t1) == str(d[name]), name
     assert int(d["benchmark_frags"]) == 240_235_639
+
+
+# ---- the benchmark's 1M-triangle 1080p frame ------------------------------------------------
+
+FRAME1M_CASES = [(mode, prog) for mode in (O.RASTER_SPAN, O.RASTER_BLOCK)
+                 for prog in (O.PS_DEPTH_CHECKER, O.PS_HASH)]
+
+
+@pytest.fixture(scope="module")
+def frame1m():
+    d = load("frame1m.npz")
+    soup = scenes.soup_scene(1_000_000, seed=2)
+    assert scene_digest(soup) == str(d["input"]), "soup generator changed"
+    return d, soup
+
+
+def test_frame1m_digests(cpu_oracle, frame1m):
+    """The oracle draws the whole benchmark frame (1M triangles, 1080p) as the reference
+    did, in Span and Block, with the depth-checker and the order-sensitive hash program."""
+    d, soup = frame1m
+    for mode, prog in FRAME1M_CASES:
+        pipe = O.Pipeline(1920, 1080, raster_mode=mode, mvp=soup.mvp, ps_program=prog)
+        t = run_cpu(cpu_oracle, pipe, soup)
+        assert digest(t.t0, t.t1) == str(d[f"m{mode}_p{prog}"]), (mode, prog)
+        if prog == O.PS_HASH:
+            assert int(t.t1.astype(np.int64).sum()) == int(d[f"m{mode}_frags"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode,prog", FRAME1M_CASES)
+def test_gpu_frame1m(gpu, cpu_oracle, frame1m, mode, prog):
+    """The gfx950 path on the benchmark's full frame: word for word against the oracle on
+    the same inputs, and against the reference's digest.  At this size the per-tile lists
+    run long (about 52 primitives per 8x8 tile on average) and the first synchronous draw
+    grows its arenas and draws again."""
+    d, soup = frame1m
+    pipe = O.Pipeline(1920, 1080, raster_mode=mode, mvp=soup.mvp, ps_program=prog)
+    b = run_gpu(pipe, soup)
+    assert digest(b.t0, b.t1) == str(d[f"m{mode}_p{prog}"]), (mode, prog)
+    a = run_cpu(cpu_oracle, pipe, soup)
+    assert_same(a, b, f"frame1m mode={mode} prog={prog}")
 
 
 def _log_cases():

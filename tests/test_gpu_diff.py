@@ -50,6 +50,54 @@ def test_visibility_matches_oracle(gpu, cpu_oracle):
     r.close()
 
 
+def _fp64_check(r, clip, F, attr, seed):
+    """Forward image and both gradients of `r` against the FP64 reference on the visibility
+    buffer the GPU drew: image 1e-5 absolute; gradients 1e-4 relative as a norm and, per
+    element, 1e-4 of the element or of 1% of the largest one on 99.9% of the elements."""
+    import torch
+    img = r.forward(torch.from_numpy(clip).cuda(), torch.from_numpy(attr).cuda(), sync=True)
+    img = img.cpu().numpy()
+    ids = r.ids.cpu().numpy().view(np.uint32)
+    depth = r.depth.cpu().numpy()
+    rng = np.random.default_rng(seed)
+    g = rng.normal(size=img.shape).astype(np.float32)
+    gt = torch.from_numpy(g).cuda()
+    gclip, gattr = r.backward(gt)
+    gclip, gattr = gclip.cpu().numpy(), gattr.cpu().numpy()
+    adj = r.adj.cpu().numpy()
+    ref_img, ref_gclip, ref_gattr = O.diff_reference(clip, F, adj, attr, ids, depth, g)
+    assert np.abs(img - ref_img).max() <= 1e-5
+    # the backward reusing the forward's decisions (two kernels) and the one walking them
+    # again (one kernel) both match the reference
+    for gc_, ga_ in ((gclip, gattr), backward_unstored(r, gt)):
+        assert rel_err(ga_, ref_gattr) <= 1e-4
+        assert rel_err(gc_, ref_gclip) <= 1e-4
+        for got, ref in ((gc_, ref_gclip), (ga_, ref_gattr)):
+            tol = 1e-4 * np.maximum(np.abs(ref), 1e-2 * np.abs(ref).max())
+            assert (np.abs(got - ref) <= tol).mean() >= 0.999
+    return ids
+
+
+def test_fp64_at_the_differentiation_workload_size(gpu):
+    """The differentiation workload's mesh and image size: the 40,962-vertex icosphere
+    (subdivision 6) at 512x512, four of its orbit views, against the FP64 reference."""
+    V, F, clip, attr = make_case(views=4, size=512, subdiv=6, seed=6)
+    r = gpu_renderer(F, V.shape[0], 4, 512)
+    ids = _fp64_check(r, clip, F, attr, seed=8)
+    assert (ids > 0).sum() > 250_000
+    r.close()
+
+
+def test_fp64_on_a_large_heightfield(gpu):
+    """One heightfield of the batched-differentiation workload's upper range: 256x256 cells
+    (131,072 triangles) at 1024x1024, against the FP64 reference."""
+    pos, F, clip, col = _terrain_case(256, 1, 1024, seed=9)
+    r = gpu_renderer(F, pos.shape[0], 1, 1024)
+    ids = _fp64_check(r, clip, F, col, seed=10)
+    assert (ids > 0).sum() > 300_000
+    r.close()
+
+
 @pytest.mark.parametrize("subdiv,size,views", [(2, 48, 2), (4, 128, 3)])
 def test_forward_and_gradients_match_fp64(gpu, subdiv, size, views):
     import torch
@@ -149,6 +197,48 @@ def test_autograd_interior_position_gradient_matches_finite_differences(gpu):
             if abs(fd - an) <= 0.03 * abs(an) + 0.02 * float(gp.abs().max()):
                 ok += 1
     assert checked >= 24 and ok >= 0.9 * checked, (ok, checked)
+    r.close()
+
+
+def test_colour_and_interior_gradients_fd_at_workload_size(gpu):
+    """Per coordinate, central differences (h = 1e-3) at the differentiation workload's size
+    (subdivision 6, 512x512): colours of visible vertices over four views (the image is
+    linear in them), and positions of vertices well inside the visible disc of one view."""
+    import torch
+    r, Vt, F, mvps, pos, col, loss, gp, gc, rng = _loss_setup(6, 512, 4, 11)
+    h = 1e-3
+    vis = np.flatnonzero(gc.abs().sum(1).cpu().numpy() > 0)
+    assert len(vis) > 1000
+    checked = ok = 0
+    for n in rng.choice(vis, 16, replace=False):
+        for k in range(3):
+            d = torch.zeros_like(col)
+            d[n, k] = 1.0
+            with torch.no_grad():
+                fd = (loss(pos, col + h * d) - loss(pos, col - h * d)).item() / (2 * h)
+            an = float(gc[n, k])
+            checked += 1
+            ok += abs(fd - an) <= 1e-3 * abs(an) + 1e-4 * float(gc.abs().max())
+    assert ok == checked, (ok, checked)
+    r.close()
+    r, Vt, F, mvps, pos, col, loss, gp, gc, rng = _loss_setup(6, 512, 1, 12)
+    inv = np.linalg.inv(mvps[0].double().cpu().numpy())
+    c0 = inv @ np.array([0.0, 0.0, -1.0, 1.0])
+    c1 = inv @ np.array([0.0, 0.0, 1.0, 1.0])
+    to_cam = c0[:3] / c0[3] - c1[:3] / c1[3]
+    to_cam /= np.linalg.norm(to_cam)
+    cand = np.flatnonzero(Vt @ to_cam > 0.8)
+    ok = checked = 0
+    for n in rng.choice(cand, 12, replace=False):
+        for k in range(3):
+            an = float(gp[n, k])
+            d = torch.zeros_like(pos)
+            d[n, k] = 1.0
+            with torch.no_grad():
+                fd = (loss(pos + h * d, col) - loss(pos - h * d, col)).item() / (2 * h)
+            checked += 1
+            ok += abs(fd - an) <= 0.03 * abs(an) + 0.02 * float(gp.abs().max())
+    assert checked == 36 and ok >= 0.9 * checked, (ok, checked)
     r.close()
 
 
