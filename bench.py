@@ -64,15 +64,29 @@ def parse():
     return p.parse_args()
 
 
+def source_revision() -> str:
+    """A digest of the kernel and C-ABI sources (softwarerenderer_amd/csrc): counter figures
+    recorded for other sources do not describe this build."""
+    d = os.path.join(ROOT, "softwarerenderer_amd", "csrc")
+    h = hashlib.sha256()
+    for name in sorted(os.listdir(d)):
+        with open(os.path.join(d, name), "rb") as f:
+            h.update(name.encode() + b"\0" + f.read())
+    return h.hexdigest()[:16]
+
+
 def _counters(workload: str) -> dict:
-    """The raster kernel's rocprofv3 counter figures recorded in profiles/traffic.json."""
+    """The raster kernel's rocprofv3 counter figures recorded in profiles/traffic.json, when
+    they were recorded for this workload and these sources (else nothing)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return {}
-    return t if t.get("workload") == workload else {}
+    if t.get("workload") != workload or t.get("source") != source_revision():
+        return {}
+    return t
 
 
 def measured_traffic(workload: str):
@@ -622,6 +636,7 @@ def main():
                      "achieved": round(raster_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(raster_gbs / HBM_PEAK_GBS, 5),
                      "traffic": measured_traffic("soup-1M-1080p-block"),
+                     "source": source_revision(),
                      "algorithmic_bytes": raster_bytes,
                      "valu_issue": valu_issue("soup-1M-1080p-block", raster_ms)},
         "parity": parity,
@@ -665,27 +680,61 @@ def frame_parity(sc, args, color, depth) -> dict:
             "checked": "last timed frame, RGBA8 + depth words, sha256"}
 
 
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(sc, runs: int):
-    """The reference CPU path (oracle/_ref, or the oracle where that is absent) timed on
-    this host's cores on the same scene: one warm-up, then the median of `runs` draws."""
+    """The reference CPU path timed on this host's cores on the same scene, as
+    Benchmark.cpp:101-104 times drawElements: one warm-up, then the median of `runs` draws,
+    targets cleared outside the timed region.  Two builds of the reference's own sources
+    (oracle/_ref): serial, and with its OpenMP pragmas (Rasterizer.h:257, 369, 396) on
+    OMP_NUM_THREADS threads (the usable cores when unset).  `value` is the faster one; the
+    oracle (a C port) stands in where the reference build is absent."""
     import oracle as O
 
-    kind = "ref" if O.ref_available() else "oracle"
-    cpu = O.CpuRenderer(kind)
+    ncores = len(os.sched_getaffinity(0))
     pipe = O.Pipeline(sc.width, sc.height, raster_mode=O.RASTER_BLOCK, mvp=sc.mvp,
                       ps_program=O.PS_DEPTH_CHECKER)
-    times = []
-    for i in range(runs + 1):
-        t = pipe.targets()
-        s = time.perf_counter()
-        cpu.draw_elements(pipe, t, [(sc.vertices, sc.stride)], O.DRAW_TRIANGLE, sc.indices)
-        if i:
-            times.append(time.perf_counter() - s)
-    med = statistics.median(times)
-    return {"value": round(sc.triangles / med / 1e6, 4), "unit": "Mtri/s", "cores": 1,
-            "kind": "reference" if kind == "ref" else "port",
-            "sample": f"full frame ({sc.triangles} triangles, 1920x1080 Block), median of "
-                      f"{runs} serial draws after one warm-up; {med * 1e3:.0f} ms per frame"}
+
+    def time_kind(kind):
+        cpu = O.CpuRenderer(kind)
+        times = []
+        for i in range(runs + 1):
+            t = pipe.targets()
+            s = time.perf_counter()
+            cpu.draw_elements(pipe, t, [(sc.vertices, sc.stride)], O.DRAW_TRIANGLE, sc.indices)
+            if i:
+                times.append(time.perf_counter() - s)
+        return statistics.median(times)
+
+    legs = {}
+    if O.ref_available():
+        legs["serial"] = (time_kind("ref"), 1, "reference")
+        if os.path.exists(O.REF_OMP_SO):
+            os.environ.setdefault("OMP_NUM_THREADS", str(ncores))
+            legs["openmp"] = (time_kind("ref_omp"), int(os.environ["OMP_NUM_THREADS"]),
+                              "reference")
+    else:
+        legs["serial"] = (time_kind("oracle"), 1, "port")
+    best = min(legs, key=lambda k: legs[k][0])
+    med, cores, kind = legs[best]
+    return {"value": round(sc.triangles / med / 1e6, 4), "unit": "Mtri/s", "cores": cores,
+            "kind": kind,
+            "sample": f"full frame ({sc.triangles} triangles, 1920x1080 Block, depth-checker), "
+                      f"median of {runs} draws after one warm-up, {best} build: "
+                      f"{med * 1e3:.0f} ms per frame",
+            "legs": {k: {"ms_per_frame": round(v[0] * 1e3, 1), "threads": v[1],
+                         "mtri_per_s": round(sc.triangles / v[0] / 1e6, 4)}
+                     for k, v in legs.items()},
+            "host": {"nproc": ncores, "cpu_model": _cpu_model()}}
 
 
 if __name__ == "__main__":

@@ -325,7 +325,14 @@ typedef struct swr_tuning {
                                 depth-checker and visibility programs: 1, 2, 4, 8 (program's) */
     int32_t xcd_order;       /* 1: raster tile blocks grouped per XCD (default), -1: linear */
     int32_t expand_chunk;    /* tile pairs per expansion workgroup: 256 .. 4096 (1024) */
-    int32_t reserved[4];
+    int32_t vertex_pass;     /* 1: shade every vertex of attribute 0 once into post-transform
+                                vertices, -1: shade each triangle corner where it is used,
+                                0: automatic (the pass when the buffer's vertex count is known
+                                and at most three per triangle) */
+    int32_t bin_size;        /* screen bins of 8, 16 or 32 pixels (8) */
+    int32_t waves_per_bin;   /* bins of 16 or 32 pixels: waves sharing a bin's 8x8 blocks:
+                                1, 2, 4 (4) */
+    int32_t reserved;
 } swr_tuning;
 int  swr_set_tuning(swr_context *ctx, const swr_tuning *t);
 int  swr_get_tuning(swr_context *ctx, swr_tuning *t);

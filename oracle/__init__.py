@@ -19,6 +19,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libswr_ref.so")
+REF_OMP_SO = os.path.join(HERE, "_ref", "libswr_ref_omp.so")
 REF_OBJ_SO = os.path.join(HERE, "_ref", "libswr_refobj.so")
 REF_SRC = "/root/reference/src/renderer"
 
@@ -214,10 +215,11 @@ class OracleTexture:
 
 
 class CpuRenderer:
-    """The oracle (kind='oracle') or the reference build (kind='ref')."""
+    """The oracle (kind='oracle'), the reference build (kind='ref') or the reference's
+    OpenMP build (kind='ref_omp')."""
 
     def __init__(self, kind: str = "oracle"):
-        path = ORACLE_SO if kind == "oracle" else REF_SO
+        path = {"oracle": ORACLE_SO, "ref": REF_SO, "ref_omp": REF_OMP_SO}[kind]
         if not os.path.exists(path):
             raise FileNotFoundError(f"{path} is not built (run oracle.build())")
         self.kind = kind

@@ -346,16 +346,20 @@ class Rasterizer:
         self._check(self._lib.swr_reserve(self._ctx, primitives, tile_pairs), "reserve")
 
     def setTuning(self, raster_waves: int = 0, raster_group: int = 0, xcd_order: int = 0,
-                  expand_chunk: int = 0):
+                  expand_chunk: int = 0, vertex_pass: int = 0, bin_size: int = 0,
+                  waves_per_bin: int = 0):
         """Launch shapes of the draw path (0 = built-in default); results never change."""
-        t = _abi.Tuning(raster_waves, raster_group, xcd_order, expand_chunk)
+        t = _abi.Tuning(raster_waves, raster_group, xcd_order, expand_chunk, vertex_pass,
+                        bin_size, waves_per_bin, 0)
         self._check(self._lib.swr_set_tuning(self._ctx, C.byref(t)), "setTuning")
 
     def tuning(self) -> dict:
         t = _abi.Tuning()
         self._check(self._lib.swr_get_tuning(self._ctx, C.byref(t)), "tuning")
         return {"raster_waves": t.raster_waves, "raster_group": t.raster_group,
-                "xcd_order": t.xcd_order, "expand_chunk": t.expand_chunk}
+                "xcd_order": t.xcd_order, "expand_chunk": t.expand_chunk,
+                "vertex_pass": t.vertex_pass, "bin_size": t.bin_size,
+                "waves_per_bin": t.waves_per_bin}
 
     # drawing through the IRasterizer seam (Rasterizer.h:99-141)
     def _list(self, fn, what, vertices, indices):

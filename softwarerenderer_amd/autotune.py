@@ -9,6 +9,11 @@ leaves the fastest one set on the rasterizer.
 
 The draw must be asynchronous and enqueue on the rasterizer's stream; the tuner binds that
 stream to torch's current stream for the duration, so its HIP events bracket the draws.
+
+The default search runs in two stages.  First the binning shape: screen bins of 8 pixels with
+1, 2, 4 or 8 bins (one wave each) per raster workgroup, or bins of 16 or 32 pixels with 1, 2
+or 4 waves sharing a bin's 8x8 blocks, each with and without the vertex pass.  Then, for the
+fastest shape, primitives per row-walk group, XCD-grouped bin order and the expansion chunk.
 """
 from __future__ import annotations
 
@@ -17,22 +22,44 @@ from typing import Callable, Iterable, Optional
 
 import torch
 
-KNOBS = ("raster_waves", "raster_group", "xcd_order", "expand_chunk")
+KNOBS = ("raster_waves", "raster_group", "xcd_order", "expand_chunk", "vertex_pass",
+         "bin_size", "waves_per_bin")
 
-# raster workgroup size in 8x8 tiles, primitives per raster row-walk group (0: the pixel
-# program's default), XCD-grouped tile order on/off, tile pairs per expansion workgroup
-DEFAULT_SPACE = {
-    "raster_waves": (1, 2, 4, 8),
+# stage 1: (bin size, raster workgroup shape) x vertex pass
+SHAPES = ([dict(bin_size=8, raster_waves=w) for w in (1, 2, 4, 8)] +
+          [dict(bin_size=b, waves_per_bin=w) for b in (16, 32) for w in (1, 2, 4)])
+VERTEX_PASS = (1, -1)
+# stage 2, for the fastest shape: primitives per raster row-walk group (0: the pixel
+# program's default), XCD-grouped bin order on/off, tile pairs per expansion workgroup
+REFINE = {
     "raster_group": (0, 1, 2, 4),
     "xcd_order": (1, -1),
     "expand_chunk": (512, 1024, 2048),
 }
 
 
-def configs(space: Optional[dict] = None) -> Iterable[dict]:
-    space = dict(DEFAULT_SPACE, **(space or {}))
-    for vals in itertools.product(*(space[k] for k in KNOBS)):
-        yield dict(zip(KNOBS, vals))
+def full(cfg: dict) -> dict:
+    """A complete setting: every knob, 0 (the built-in default) where `cfg` has none."""
+    return {k: cfg.get(k, 0) for k in KNOBS}
+
+
+def configs(space: dict) -> Iterable[dict]:
+    """Every combination of the knobs `space` lists (the others at their defaults)."""
+    keys = [k for k in KNOBS if k in space]
+    for vals in itertools.product(*(space[k] for k in keys)):
+        yield full(dict(zip(keys, vals)))
+
+
+def staged_configs(best_shape: Optional[dict] = None) -> Iterable[dict]:
+    if best_shape is None:
+        for shape in SHAPES:
+            for vp in VERTEX_PASS:
+                yield full(dict(shape, vertex_pass=vp))
+    else:
+        for cfg in configs(REFINE):
+            yield full(dict(cfg, **{k: best_shape[k] for k in
+                                    ("bin_size", "raster_waves", "waves_per_bin",
+                                     "vertex_pass")}))
 
 
 def time_draws(draw: Callable[[], None], reps: int, warmup: int) -> float:
@@ -50,32 +77,43 @@ def time_draws(draw: Callable[[], None], reps: int, warmup: int) -> float:
     return e0.elapsed_time(e1) / reps
 
 
+def _sweep(rasterizer, draw, cfgs, reps, warmup, rounds, best_t):
+    for rnd in range(rounds):
+        order = range(len(cfgs)) if rnd % 2 == 0 else reversed(range(len(cfgs)))
+        for i in order:
+            rasterizer.setTuning(**cfgs[i])
+            key = tuple(sorted(cfgs[i].items()))
+            best_t[key] = min(best_t.get(key, float("inf")), time_draws(draw, reps, warmup))
+
+
 def autotune(rasterizer, draw: Callable[[], None], space: Optional[dict] = None,
              reps: int = 8, warmup: int = 2, rounds: int = 2) -> dict:
-    """Times `draw` under every configuration of `space` (``DEFAULT_SPACE`` overridden per
-    knob), `rounds` passes in alternating order to cancel drift, keeps each configuration's
-    best time, sets the fastest on `rasterizer` and returns
+    """Times `draw` under every configuration of `space` (each listed knob's values, the
+    others at their defaults), or of the two-stage default search when `space` is None,
+    `rounds` passes in alternating order to cancel drift, keeps each configuration's best
+    time, sets the fastest on `rasterizer` and returns
     ``{"best": cfg, "ms": t, "default_ms": t0, "table": [(cfg, t), ...]}``."""
     stream = torch.cuda.current_stream()
     rasterizer.setStream(stream.cuda_stream)
     rasterizer.synchronize()
     start = rasterizer.tuning()
-    cfgs = list(configs(space))
-    best_t = {i: float("inf") for i in range(len(cfgs))}
+    best_t: dict = {}
     try:
         rasterizer.setTuning()
         default_ms = time_draws(draw, reps, warmup)
-        for rnd in range(rounds):
-            order = range(len(cfgs)) if rnd % 2 == 0 else reversed(range(len(cfgs)))
-            for i in order:
-                rasterizer.setTuning(**cfgs[i])
-                best_t[i] = min(best_t[i], time_draws(draw, reps, warmup))
+        if space is not None:
+            _sweep(rasterizer, draw, list(configs(space)), reps, warmup, rounds, best_t)
+        else:
+            _sweep(rasterizer, draw, list(staged_configs()), reps, warmup, rounds, best_t)
+            shape = dict(min(best_t, key=best_t.get))
+            _sweep(rasterizer, draw, list(staged_configs(shape)), reps, warmup, rounds, best_t)
     except BaseException:
         rasterizer.synchronize()
         rasterizer.setTuning(**start)
         raise
     rasterizer.synchronize()
-    i_best = min(best_t, key=best_t.get)
-    rasterizer.setTuning(**cfgs[i_best])
-    table = sorted(((cfgs[i], t) for i, t in best_t.items()), key=lambda ct: ct[1])
-    return {"best": cfgs[i_best], "ms": best_t[i_best], "default_ms": default_ms, "table": table}
+    key = min(best_t, key=best_t.get)
+    best = dict(key)
+    rasterizer.setTuning(**best)
+    table = sorted(((dict(k), t) for k, t in best_t.items()), key=lambda ct: ct[1])
+    return {"best": best, "ms": best_t[key], "default_ms": default_ms, "table": table}

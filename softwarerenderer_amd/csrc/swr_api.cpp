@@ -68,6 +68,7 @@ struct swr_context {
         bigq,       // per-primitive first pair index
         clipq,      // triangles queued for the clipper
         clipq_cnt,  // its per-region counters
+        chunk_kept, // pairs kept per expansion chunk
         vwin, vmask,  // post-transform vertices of the vertex pass
         zscratch[2];  // scratch of the differentiation kernels
     uint64_t prim_cap = 0, pair_cap = 0;
@@ -99,7 +100,7 @@ struct swr_batch {
     uint32_t nscenes = 0, batches = 0;
     uint64_t vtris = 0;          // virtual triangles: every scene padded to whole batches
     uint64_t tris = 0;           // input triangles
-    uint64_t tiles = 0;          // sum of the scenes' tiles
+    uint64_t bins[3] = {0, 0, 0};  // sum of the scenes' bins of 8, 16 and 32 pixels
     unsigned tmask = 7;          // bit i: every scene provides target i
 };
 
@@ -409,6 +410,7 @@ struct Arenas {
     float *prims = nullptr;
     uint32_t *bigq = nullptr, *pk_a = nullptr, *pv_a = nullptr, *pk_b = nullptr, *pv_b = nullptr;
     uint32_t *hist = nullptr, *hist_ex = nullptr, *hist_tmp = nullptr;
+    uint32_t *chunk_kept = nullptr;
     uint64_t prim_cap = 0, pair_cap = 0;
     uint32_t G = 1;  // radix sort workgroups
 };
@@ -438,6 +440,13 @@ uint32_t sort_blocks(uint64_t pair_cap)
 }
 
 size_t hist_bytes(uint32_t G) { return (size_t)256 * G * 4; }
+// kept-pair counts of the expansion chunks (256 slots at the least)
+size_t chunk_kept_bytes(uint64_t pair_cap) { return (pair_cap / 256 + 1) * 4; }
+// the expansion chunk launch_expand uses for a tuning value
+int expand_chunk_of(int tuned)
+{
+    return (tuned == 256 || tuned == 512 || tuned == 2048 || tuned == 4096) ? tuned : 1024;
+}
 size_t hist_tmp_bytes(uint32_t G) { return scan_tmp_elems((uint64_t)256 * G) * 4; }
 
 // The triangle draw: drawElements (window_input = false), the IRasterizer seam, or a batch
@@ -449,9 +458,12 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
     const size_t T = bt ? bt->vtris : count / 3;
     std::memset(&c->stats, 0, sizeof c->stats);
     c->stats.input_primitives = bt ? bt->tris : T;
-    c->stats.tiles = bt ? bt->tiles
-                        : (uint64_t)((c->ps.width + TILE - 1) / TILE) *
-                              (uint64_t)((c->ps.height + TILE - 1) / TILE);
+    // screen bins of 8 << lvl pixels (the tuning's bin size)
+    const int lvl = c->tune.bin_size == 32 ? 2 : (c->tune.bin_size == 16 ? 1 : 0);
+    const int bin_shift = 3 + lvl, bin = 1 << bin_shift;
+    c->stats.tiles = bt ? bt->bins[lvl]
+                        : (uint64_t)((c->ps.width + bin - 1) / bin) *
+                              (uint64_t)((c->ps.height + bin - 1) / bin);
     if (T == 0 || c->ps.program == SWR_PS_NULL)
         return SWR_OK;  // nothing observable happens
     if (T > (size_t)(1u << 31) / 8)
@@ -464,8 +476,8 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
                     c->ps.pvar_count);
     const swr_pixel_shader_desc &ps = c->ps;
     const int W = bt ? 0 : ps.width, H = bt ? 0 : ps.height;
-    const int tiles_x = (W + TILE - 1) / TILE, tiles_y = (H + TILE - 1) / TILE;
-    const uint32_t tiles = bt ? (uint32_t)bt->tiles : (uint32_t)tiles_x * (uint32_t)tiles_y;
+    const int tiles_x = (W + bin - 1) / bin, tiles_y = (H + bin - 1) / bin;
+    const uint32_t tiles = bt ? (uint32_t)bt->bins[lvl] : (uint32_t)tiles_x * (uint32_t)tiles_y;
     if (tiles == 0)
         return SWR_OK;
     if (!sync && !bt) {
@@ -560,9 +572,9 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
     // average, 1 on, -1 off)
     uint32_t vcount = 0;
     if (!bt && !window_input && c->vs.program != SWR_VS_DUMMY && attr_bytes &&
-        attr_bytes >= need && c->attr[0].stride > 0 && c->tune.reserved[0] >= 0) {
+        attr_bytes >= need && c->attr[0].stride > 0 && c->tune.vertex_pass >= 0) {
         const uint64_t nv = (attr_bytes - need) / (uint64_t)c->attr[0].stride + 1;
-        const bool want = c->tune.reserved[0] > 0 || nv <= 3 * (uint64_t)T;
+        const bool want = c->tune.vertex_pass > 0 || nv <= 3 * (uint64_t)T;
         if (want && nv < (1ull << 31)) {
             vcount = (uint32_t)nv;
             HIP_TRY(c, ensure(c, c->vwin, (size_t)vcount * 16));
@@ -582,6 +594,7 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
             HIP_TRY(c, ensure(c, c->hist, hist_bytes(A.G)));
             HIP_TRY(c, ensure(c, c->hist_ex, hist_bytes(A.G)));
             HIP_TRY(c, ensure(c, c->hist_tmp, hist_tmp_bytes(A.G)));
+            HIP_TRY(c, ensure(c, c->chunk_kept, chunk_kept_bytes(c->pair_cap)));
             A.prims = (float *)c->prims.p;
             A.bigq = (uint32_t *)c->bigq.p;
             A.pk_a = (uint32_t *)c->pk_a.p;
@@ -591,6 +604,7 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
             A.hist = (uint32_t *)c->hist.p;
             A.hist_ex = (uint32_t *)c->hist_ex.p;
             A.hist_tmp = (uint32_t *)c->hist_tmp.p;
+            A.chunk_kept = (uint32_t *)c->chunk_kept.p;
             A.prim_cap = c->prim_cap;
             A.pair_cap = c->pair_cap;
         }
@@ -628,6 +642,7 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
         g.height = H;
         g.tiles_x = tiles_x;
         g.tiles_y = tiles_y;
+        g.bin_shift = bin_shift;
         g.lay = L;
         g.window_input = window_input ? 1 : 0;
         g.wverts = (const swr_rasterizer_vertex *)dwv;
@@ -647,6 +662,7 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
         g.tiles = tiles;
         g.sentinel = pair_sentinel(tiles);
         g.prim_pb = A.bigq;
+        g.chunk_kept = A.chunk_kept;
         g.clipq = A.clipq;
         g.clipq_cnt = A.clipq_cnt;
         if (bt) {
@@ -685,6 +701,8 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
         sb.scan_tmp = A.hist_tmp;
         sb.ranges = A.ranges;
         sb.blocks = G;
+        sb.chunk_kept = A.chunk_kept;
+        sb.chunk = (uint32_t)expand_chunk_of(c->tune.expand_chunk);
         HIP_TRY(c, sort_pairs(sb, g.status, tiles, c->stream));
         record(c, 4);
         HIP_TRY(c, launch_ranges(sb, g.status, tiles, c->stream));
@@ -699,6 +717,10 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
         r.status = g.status;
         r.tiles = (int)tiles;
         r.tiles_x = tiles_x;
+        r.bin_shift = bin_shift;
+        r.waves_per_bin = c->tune.waves_per_bin ? c->tune.waves_per_bin : 4;
+        if (r.waves_per_bin > (1 << (2 * lvl)))
+            r.waves_per_bin = 1 << (2 * lvl);
         r.width = W;
         r.height = H;
         r.pitch = ps.pitch;
@@ -840,6 +862,7 @@ int draw_lines_points(swr_context *c, int mode, const int32_t *indices, size_t c
     g.height = H;
     g.tiles_x = tiles_x;
     g.tiles_y = tiles_y;
+    g.bin_shift = 3;
     g.lay = L;
     g.window_input = window_input ? 1 : 0;
     g.wverts = (const swr_rasterizer_vertex *)dwv;
@@ -915,6 +938,7 @@ int draw_lines_points(swr_context *c, int mode, const int32_t *indices, size_t c
         r.status = g.status;
         r.tiles = (int)tiles;
         r.tiles_x = tiles_x;
+        r.bin_shift = 3;
         r.width = W;
         r.height = H;
         r.pitch = ps.pitch;
@@ -1088,7 +1112,7 @@ int swr_destroy(swr_context *c)
                       &c->scan_tmp, &c->prims, &c->pk_a, &c->pv_a, &c->pk_b, &c->pv_b,
                       &c->ranges, &c->hist, &c->hist_ex, &c->hist_tmp, &c->status, &c->tgt[0],
                       &c->tgt[1], &c->tgt[2], &c->log, &c->logcount, &c->bigq, &c->clipq,
-                      &c->clipq_cnt, &c->vwin, &c->vmask, &c->zscratch[0], &c->zscratch[1],
+                      &c->clipq_cnt, &c->chunk_kept, &c->vwin, &c->vmask, &c->zscratch[0], &c->zscratch[1],
                       &c->diff_table};
     for (DevBuf *b : bufs)
         if (b->p)
@@ -1298,7 +1322,7 @@ int swr_batch_create(swr_context *c, const swr_scene *sc, size_t n, swr_batch **
     (void)hipSetDevice(c->device);
     std::vector<SceneDev> tab(n);
     std::vector<uint32_t> bmap;
-    uint64_t tiles = 0, tris = 0;
+    uint64_t bins[3] = {0, 0, 0}, tris = 0;
     uint32_t batches = 0;
     unsigned tmask = 7;
     for (size_t i = 0; i < n; ++i) {
@@ -1351,19 +1375,23 @@ int swr_batch_create(swr_context *c, const swr_scene *sc, size_t n, swr_batch **
         d.width = s.width;
         d.height = s.height;
         d.pitch = s.pitch;
-        d.tiles_x = (s.width + TILE - 1) / TILE;
-        d.tiles_y = (s.height + TILE - 1) / TILE;
+        for (int l = 0; l < 3; ++l) {
+            const int bin = TILE << l;
+            d.bins_x[l] = (s.width + bin - 1) / bin;
+            d.bins_y[l] = (s.height + bin - 1) / bin;
+            d.bin0[l] = (uint32_t)bins[l];
+        }
         d.tris = (uint32_t)T;
         d.batch0 = batches;
-        d.tile0 = (uint32_t)tiles;
         const uint32_t nb = (uint32_t)((T + BATCH - 1) / BATCH);
         for (uint32_t b = 0; b < nb; ++b)
             bmap.push_back((uint32_t)i);
         batches += nb;
-        tiles += (uint64_t)d.tiles_x * (uint64_t)d.tiles_y;
+        for (int l = 0; l < 3; ++l)
+            bins[l] += (uint64_t)d.bins_x[l] * (uint64_t)d.bins_y[l];
         tris += T;
         if (T > (1u << 31) / 8 || (uint64_t)batches * BATCH > (1u << 31) / 8 ||
-            tiles >= 0xFFFFFFF0ull)
+            bins[0] >= 0xFFFFFFF0ull)
             return fail(c, SWR_ERR_CAPACITY, "batch too large at scene %zu", i);
     }
     swr_batch *b = new swr_batch();
@@ -1372,7 +1400,8 @@ int swr_batch_create(swr_context *c, const swr_scene *sc, size_t n, swr_batch **
     b->batches = batches;
     b->vtris = (uint64_t)batches * BATCH;
     b->tris = tris;
-    b->tiles = tiles;
+    for (int l = 0; l < 3; ++l)
+        b->bins[l] = bins[l];
     b->tmask = n ? tmask : 0;
     hipError_t e = ensure(c, b->scenes, std::max<size_t>(1, n) * sizeof(SceneDev));
     if (e == hipSuccess)
@@ -1505,7 +1534,9 @@ int swr_set_tuning(swr_context *c, const swr_tuning *t)
         return fail(c, SWR_ERR_INVALID_ARG, "null tuning");
     auto pow2_in = [](int v, int lo, int hi) { return v == 0 || (v >= lo && v <= hi && !(v & (v - 1))); };
     if (!pow2_in(t->raster_waves, 1, 8) || !pow2_in(t->raster_group, 1, 8) ||
-        !pow2_in(t->expand_chunk, 256, 4096) || t->xcd_order < -1 || t->xcd_order > 1)
+        !pow2_in(t->expand_chunk, 256, 4096) || t->xcd_order < -1 || t->xcd_order > 1 ||
+        t->vertex_pass < -1 || t->vertex_pass > 1 || !pow2_in(t->bin_size, 8, 32) ||
+        !pow2_in(t->waves_per_bin, 1, 4) || t->reserved != 0)
         return fail(c, SWR_ERR_INVALID_ARG, "tuning value out of range");
     c->tune = *t;
     return SWR_OK;

@@ -21,12 +21,15 @@ namespace swr {
 //   8..16  edge equations e0.abc e1.abc e2.abc (EdgeEquation.h:37-43)
 //   17..22 bottom-flat part: ox oy is1 is2 row_lo row_hi   (rows lo <= y < hi)
 //   23..28 top-flat part:    ox oy is1 is2 row_lo row_hi   (rows lo < y <= hi)
+//   17..20 Block primitives instead: the 8x8 blocks of the reference's block grid,
+//          bx0 by0 bx1 by1 (inclusive, in blocks; clamped to the target)
 //   32..   parameter planes a,b,c: [z][invw][avar 0..na)[pvar 0..np)
 constexpr int REC_KEY = 0;
 constexpr int REC_FLAGS = 1;
 constexpr int REC_TILES = 2;
 constexpr int REC_EDGE = 8;
 constexpr int REC_BOT = 17;
+constexpr int REC_BLOCKS = 17;
 constexpr int REC_TOP = 23;
 constexpr int REC_PARAM = 32;
 
@@ -132,16 +135,18 @@ __device__ inline int classify_block(const float *e, uint32_t flags, int x, int 
 // roundings, each at most 2^-24 of a partial sum bounded by M = |a xf| + |b yf| + |c| +
 // 7|a| + 7|b|.  A block is dropped only when an edge's exact maximum over the block (a
 // corner: the function is linear) is below -2^-19 M, more than twice that error bound.
-__device__ inline bool block_may_cover(const float *e, int x, int y)
+// ext = 7 is one 8x8 block; a larger ext tests a square of (ext + 1)^2 pixels (a bin of
+// blocks: the bound then covers every block's chain, which starts inside the square).
+__device__ inline bool block_may_cover(const float *e, int x, int y, int ext = 7)
 {
-    const double xf = (double)x + 0.5, yf = (double)y + 0.5;
+    const double xf = (double)x + 0.5, yf = (double)y + 0.5, s = (double)ext;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const double a = e[3 * k], b = e[3 * k + 1], c = e[3 * k + 2];
         if (!__builtin_isfinite(a) || !__builtin_isfinite(b) || !__builtin_isfinite(c))
             return true;
-        const double m = fabs(a * xf) + fabs(b * yf) + fabs(c) + 7.0 * (fabs(a) + fabs(b));
-        const double mx = a * xf + b * yf + c + fmax(0.0, 7.0 * a) + fmax(0.0, 7.0 * b);
+        const double m = fabs(a * xf) + fabs(b * yf) + fabs(c) + s * (fabs(a) + fabs(b));
+        const double mx = a * xf + b * yf + c + fmax(0.0, s * a) + fmax(0.0, s * b);
         if (mx < -m * 1.9073486328125e-06)  // 2^-19
             return false;
     }
@@ -177,7 +182,8 @@ constexpr uint32_t PAIR_PRIM_MASK = 0x3FFFFFFFu;
 // One scene of a batched draw: its inputs, its VertexProcessor / Rasterizer state and its
 // render targets.  Scene s owns the virtual triangles [batch0 * 1024, batch0 * 1024 + tris)
 // (each scene starts on a fresh 1024-triangle batch, so its draw order is exactly that of
-// a draw of its own) and the global tiles [tile0, tile0 + tiles_x * tiles_y).
+// a draw of its own) and, for each bin size 8 << L (L = 0, 1, 2), the global bins
+// [bin0[L], bin0[L] + bins_x[L] * bins_y[L]).
 struct SceneDev {
     const int32_t *indices;
     const uint8_t *attr0;
@@ -187,8 +193,10 @@ struct SceneDev {
     float vp_px, vp_py, vp_ox, vp_oy, dz_scale, dz_bias;
     int stride0, cull_mode, raster_mode;
     int sc_minx, sc_maxx, sc_miny, sc_maxy;
-    int width, height, pitch, tiles_x, tiles_y;
-    uint32_t tris, batch0, tile0, pad0;
+    int width, height, pitch;
+    int bins_x[3], bins_y[3];
+    uint32_t bin0[3];
+    uint32_t tris, batch0;
 };
 
 // ---- launch arguments ---------------------------------------------------------------
@@ -208,7 +216,8 @@ struct GeomArgs {
     int sc_minx, sc_maxx, sc_miny, sc_maxy;   // scissor as Rasterizer.h:83-86 stores it
     float vp_px, vp_py, vp_ox, vp_oy;         // VertexProcessor.cpp:50-53
     float dz_scale, dz_bias;                  // 0.5f*(f-n), 0.5f*(n+f)
-    int width, height, tiles_x, tiles_y;
+    int width, height, tiles_x, tiles_y;  // tiles_x, tiles_y, tiles: bins of 1 << bin_shift px
+    int bin_shift;              // 3, 4 or 5: screen bins of 8, 16 or 32 pixels
     ParamLayout lay;
     int window_input;           // 1: vertices are window-space RasterizerVertex (IRasterizer seam)
     const swr_rasterizer_vertex *wverts;
@@ -230,6 +239,7 @@ struct GeomArgs {
     uint32_t tiles;
     uint32_t sentinel;          // pair key of a tile the primitive cannot shade (dropped)
     uint32_t *prim_pb;          // [prim_cap] index of each primitive's first pair
+    uint32_t *chunk_kept;       // [pair_cap / expansion chunk] kept pairs per chunk (or null)
     uint32_t *clipq;            // [tris + 64*256] triangles that straddle a clip plane
     uint32_t *clipq_cnt;        // [64 * 16] per-region queue counters
     // post-transform vertices (the vertex pass; vwin == nullptr: off): window-space
@@ -259,7 +269,9 @@ struct RasterArgs {
     const uint32_t *ranges;     // [tiles*2] begin, end into vals
     const uint32_t *vals;       // primitive index per sorted pair
     const unsigned long long *status;
-    int tiles, tiles_x;
+    int tiles, tiles_x;         // bins of 1 << bin_shift pixels
+    int bin_shift;
+    int waves_per_bin;          // bins larger than 8x8: waves sharing one bin's blocks
     int width, height, pitch;
     int sc_minx, sc_maxx;       // Span x clipping (Rasterizer.h:377-378, 404-405)
     int na_set, np_set;         // declared counts (hash / log programs)
@@ -309,17 +321,17 @@ __device__ inline void apply_scene(GeomArgs &a, const SceneDev &s)
     a.sc_maxy = s.sc_maxy;
     a.width = s.width;
     a.height = s.height;
-    a.tiles_x = s.tiles_x;
-    a.tiles_y = s.tiles_y;
+    a.tiles_x = s.bins_x[a.bin_shift - 3];
+    a.tiles_y = s.bins_y[a.bin_shift - 3];
 }
 
-// The scene owning global tile `tile` (scenes sorted by tile0).
-__device__ inline uint32_t scene_of_tile(const SceneDev *sc, uint32_t n, uint32_t tile)
+// The scene owning global bin `tile` of bin level lvl (scenes sorted by bin0[lvl]).
+__device__ inline uint32_t scene_of_tile(const SceneDev *sc, uint32_t n, uint32_t tile, int lvl)
 {
     uint32_t lo = 0, hi = n;
     while (hi - lo > 1) {
         uint32_t mid = (lo + hi) >> 1;
-        if (sc[mid].tile0 <= tile)
+        if (sc[mid].bin0[lvl] <= tile)
             lo = mid;
         else
             hi = mid;

@@ -54,7 +54,8 @@ struct PrimGeo {
     Edge e0, e1, e2;
     float area2;
     uint32_t flags;
-    int tx0, ty0, tx1, ty1;
+    int tx0, ty0, tx1, ty1;  // screen bins
+    int kx0, ky0, kx1, ky1;  // Block: the 8x8 blocks of the block grid
     SpanPart bot, top;
 };
 
@@ -91,10 +92,10 @@ __device__ inline void span_part_rect(const GeomArgs &a, const SpanPart &p, long
     if (x1 > a.width - 1) x1 = a.width - 1;
     if (x0 > x1)
         return;
-    tx0 = imin(tx0, (int)(x0 >> 3));
-    tx1 = imax(tx1, (int)(x1 >> 3));
-    ty0 = imin(ty0, (int)(r0 >> 3));
-    ty1 = imax(ty1, (int)(r1 >> 3));
+    tx0 = imin(tx0, (int)(x0 >> a.bin_shift));
+    tx1 = imax(tx1, (int)(x1 >> a.bin_shift));
+    ty0 = imin(ty0, (int)(r0 >> a.bin_shift));
+    ty1 = imax(ty1, (int)(r1 >> a.bin_shift));
 }
 
 // Rasterizer::drawTriangleSpanTemplate, Rasterizer.h:318-357: vertex order and the two
@@ -155,7 +156,8 @@ __device__ inline void span_setup(const GeomArgs &a, const WinV &v0, const WinV 
     g.ty1 = ty1;
 }
 
-// Rasterizer::drawTriangleBlockTemplate, Rasterizer.h:234-258: the block grid as a tile rect.
+// Rasterizer::drawTriangleBlockTemplate, Rasterizer.h:234-258: the block grid as a rect of
+// screen bins.
 __device__ inline void block_setup(const GeomArgs &a, const WinV &v0, const WinV &v1,
                                    const WinV &v2, PrimGeo &g)
 {
@@ -186,17 +188,26 @@ __device__ inline void block_setup(const GeomArgs &a, const WinV &v0, const WinV
     } else {
         return;
     }
-    long long t0 = bx0 >> 3, t1 = bx1 >> 3, u0 = by0 >> 3, u1 = by1 >> 3;
-    if (t0 < 0) t0 = 0;
-    if (u0 < 0) u0 = 0;
-    if (t1 > a.tiles_x - 1) t1 = a.tiles_x - 1;
-    if (u1 > a.tiles_y - 1) u1 = a.tiles_y - 1;
-    if (t0 > t1 || u0 > u1)
+    // the block grid, clamped to the target's blocks (pixels outside are never written)
+    long long k0 = bx0 >> 3, k1 = bx1 >> 3, j0 = by0 >> 3, j1 = by1 >> 3;
+    const long long kmax = ((long long)a.width + TILE - 1) / TILE - 1;
+    const long long jmax = ((long long)a.height + TILE - 1) / TILE - 1;
+    if (k0 < 0) k0 = 0;
+    if (j0 < 0) j0 = 0;
+    if (k1 > kmax) k1 = kmax;
+    if (j1 > jmax) j1 = jmax;
+    if (k0 > k1 || j0 > j1)
         return;
-    g.tx0 = (int)t0;
-    g.tx1 = (int)t1;
-    g.ty0 = (int)u0;
-    g.ty1 = (int)u1;
+    g.kx0 = (int)k0;
+    g.kx1 = (int)k1;
+    g.ky0 = (int)j0;
+    g.ky1 = (int)j1;
+    // the bins holding those blocks
+    const int sh = a.bin_shift - 3;
+    g.tx0 = (int)(k0 >> sh);
+    g.tx1 = (int)(k1 >> sh);
+    g.ty0 = (int)(j0 >> sh);
+    g.ty1 = (int)(j1 >> sh);
 }
 
 // Setup shared by both passes.  Returns false when the triangle draws nothing.
@@ -231,10 +242,13 @@ __device__ inline bool prim_geometry(const GeomArgs &a, const WinV &v0, const Wi
     return g.tx0 <= g.tx1 && g.ty0 <= g.ty1;
 }
 
-// The (key, value) pair of one candidate tile.  Tiles in which the primitive cannot shade a
+// The (key, value) pair of one candidate bin.  Bins in which the primitive cannot shade a
 // pixel get the sentinel key and are dropped before the sort:
-//   Block: the reference's corner classification (0 = skip), then block_may_cover;
-//   Span: exact, the tile's eight scanline ranges are those the rasterizer computes.
+//   Block, 8x8 bins: the reference's corner classification (0 = skip), then
+//     block_may_cover; the class rides in the pair value;
+//   Block, larger bins: dropped when no pixel of the bin can pass all three edge tests
+//     (block_may_cover over the bin); the raster kernel classifies each 8x8 block;
+//   Span: exact, the bin's scanline ranges are those the rasterizer computes.
 __device__ inline void pair_for_tile(const GeomArgs &a, const float *rec, uint32_t flags,
                                      uint32_t pos, int tx, int ty, uint32_t &key, uint32_t &val)
 {
@@ -242,26 +256,31 @@ __device__ inline void pair_for_tile(const GeomArgs &a, const float *rec, uint32
     uint32_t tile0 = 0;
     if (a.nscenes) {
         const SceneDev &S = a.scenes[__float_as_uint(rec[REC_SCENE])];
-        tiles_x = S.tiles_x;
+        tiles_x = S.bins_x[a.bin_shift - 3];
         sc_minx = S.sc_minx;
         sc_maxx = S.sc_maxx;
-        tile0 = S.tile0;
+        tile0 = S.bin0[a.bin_shift - 3];
     }
     key = tile0 + (uint32_t)(ty * tiles_x + tx);
     val = pos;
-    const int bx = tx * TILE, by = ty * TILE;
+    const int size = 1 << a.bin_shift;
+    const int bx = tx << a.bin_shift, by = ty << a.bin_shift;
     if (flags & FLAG_BLOCK) {
-        int cls = classify_block(rec + REC_EDGE, flags, bx, by);
-        if (cls == 1 && !block_may_cover(rec + REC_EDGE, bx, by))
-            cls = 0;
-        key = cls ? key : a.sentinel;
-        val |= (uint32_t)cls << 30;
+        if (size == TILE) {
+            int cls = classify_block(rec + REC_EDGE, flags, bx, by);
+            if (cls == 1 && !block_may_cover(rec + REC_EDGE, bx, by))
+                cls = 0;
+            key = cls ? key : a.sentinel;
+            val |= (uint32_t)cls << 30;
+        } else {
+            key = block_may_cover(rec + REC_EDGE, bx, by, size - 1) ? key : a.sentinel;
+        }
     } else {
         bool any = false;
-        for (int r = 0; r < TILE && !any; ++r) {
+        for (int r = 0; r < size && !any; ++r) {
             int xl, xr;
             bool rowv = span_row_extent(rec, flags, by + r, sc_minx, sc_maxx, xl, xr);
-            any = rowv && imax(xl, bx) < imin(xr, bx + TILE);
+            any = rowv && imax(xl, bx) < imin(xr, bx + size);
         }
         key = any ? key : a.sentinel;
     }
@@ -301,7 +320,12 @@ __device__ __forceinline__ void write_prim(const GeomArgs &a, uint64_t pos, uint
     rec[REC_EDGE + 6] = g.e2.a;
     rec[REC_EDGE + 7] = g.e2.b;
     rec[REC_EDGE + 8] = g.e2.c;
-    if (!(g.flags & FLAG_BLOCK)) {
+    if (g.flags & FLAG_BLOCK) {
+        ru[REC_BLOCKS + 0] = (uint32_t)g.kx0;
+        ru[REC_BLOCKS + 1] = (uint32_t)g.ky0;
+        ru[REC_BLOCKS + 2] = (uint32_t)g.kx1;
+        ru[REC_BLOCKS + 3] = (uint32_t)g.ky1;
+    } else {
         const SpanPart *parts[2] = {&g.bot, &g.top};
         const uint32_t pf[2] = {FLAG_BOTTOM, FLAG_TOP};
         const int base[2] = {REC_BOT, REC_TOP};
@@ -346,13 +370,17 @@ __device__ __forceinline__ void write_prim(const GeomArgs &a, uint64_t pos, uint
 // (tile, primitive) pairs, load-balanced: each workgroup owns EXPAND_CHUNK consecutive pair
 // slots (256 to 4096; tunable).  Pair bases are increasing in draw order, so the primitives
 // owning a chunk are a contiguous run found by one binary search; each thread then searches
-// that run (in LDS) for the primitive of its slot and writes the slot's tile in row-major
-// rect order.
+// that run (in LDS) for the primitive of its slot and classifies the slot's tile (row-major
+// rect order).  The tiles the primitive cannot shade are dropped here: the kept pairs are
+// written in slot order to the start of the chunk and their count to chunk_kept[chunk], so
+// the sort reads only kept pairs (without chunk_kept the chunk is written whole, rejected
+// pairs keyed with the sentinel).
 template <int EXPAND_CHUNK>
 __global__ void __launch_bounds__(256) k_expand(GeomArgs a)
 {
     __shared__ uint32_t s_pb[EXPAND_CHUNK + 1];
     __shared__ uint32_t s_p0, s_p1;
+    __shared__ uint32_t s_w[4];
     if (a.status[ST_OVERFLOW] | a.status[ST_ERROR])
         return;
     const uint64_t M = a.status[ST_PAIRS], P = a.status[ST_PRIMS];
@@ -388,27 +416,58 @@ __global__ void __launch_bounds__(256) k_expand(GeomArgs a)
     for (uint32_t i = threadIdx.x; i <= np; i += blockDim.x)
         s_pb[i] = (p0 + i < P) ? a.prim_pb[p0 + i] : (uint32_t)M;
     __syncthreads();
-    for (uint64_t q = c0 + threadIdx.x; q < c1; q += blockDim.x) {
-        uint32_t lo = 0, hi = np;
-        while (hi - lo > 1) {
-            uint32_t mid = (lo + hi) >> 1;
-            if (s_pb[mid] <= (uint32_t)q)
-                lo = mid;
-            else
-                hi = mid;
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    uint32_t kept = 0;  // pairs kept by the earlier rounds (uniform)
+    for (uint64_t r0 = c0; r0 < c1; r0 += 256) {  // blockDim.x == 256
+        const uint64_t q = r0 + threadIdx.x;
+        uint32_t key = a.sentinel, val = 0;
+        if (q < c1) {
+            uint32_t lo = 0, hi = np;
+            while (hi - lo > 1) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (s_pb[mid] <= (uint32_t)q)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            const uint32_t pos = p0 + lo;
+            const uint32_t local = (uint32_t)q - s_pb[lo];
+            const float *rec = a.prims + (uint64_t)pos * (uint64_t)a.lay.stride;
+            const uint32_t *ru = reinterpret_cast<const uint32_t *>(rec);
+            const int tx0 = (int)ru[REC_TILES + 0], ty0 = (int)ru[REC_TILES + 1];
+            const uint32_t w0 = ru[REC_TILES + 2] - ru[REC_TILES + 0] + 1u;
+            const int tx = tx0 + (int)(local % w0), ty = ty0 + (int)(local / w0);
+            pair_for_tile(a, rec, ru[REC_FLAGS], pos, tx, ty, key, val);
         }
-        const uint32_t pos = p0 + lo;
-        const uint32_t local = (uint32_t)q - s_pb[lo];
-        const float *rec = a.prims + (uint64_t)pos * (uint64_t)a.lay.stride;
-        const uint32_t *ru = reinterpret_cast<const uint32_t *>(rec);
-        const int tx0 = (int)ru[REC_TILES + 0], ty0 = (int)ru[REC_TILES + 1];
-        const uint32_t w = ru[REC_TILES + 2] - ru[REC_TILES + 0] + 1u;
-        const int tx = tx0 + (int)(local % w), ty = ty0 + (int)(local / w);
-        uint32_t key, val;
-        pair_for_tile(a, rec, ru[REC_FLAGS], pos, tx, ty, key, val);
-        a.pair_keys[q] = key;
-        a.pair_vals[q] = val;
+        if (!a.chunk_kept) {
+            if (q < c1) {
+                a.pair_keys[q] = key;
+                a.pair_vals[q] = val;
+            }
+            continue;
+        }
+        // block-wide rank of the kept pairs of this round, in slot order
+        const bool keep = q < c1 && key != a.sentinel;
+        const unsigned long long m = __ballot(keep);
+        if (lane == 0)
+            s_w[w] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t pre = 0, tot = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            pre += k < (int)w ? s_w[k] : 0u;
+            tot += s_w[k];
+        }
+        __syncthreads();  // s_w is written again by the next round
+        if (keep) {
+            const uint64_t o = c0 + kept + pre + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            a.pair_keys[o] = key;
+            a.pair_vals[o] = val;
+        }
+        kept += tot;
     }
+    if (a.chunk_kept && threadIdx.x == 0)
+        a.chunk_kept[blockIdx.x] = kept;
 }
 
 __device__ inline uint64_t rect_tiles(const PrimGeo &g)
@@ -1243,7 +1302,7 @@ static hipError_t launch_expand_ch(const GeomArgs &a, hipStream_t s)
 
 hipError_t launch_expand(const GeomArgs &a, hipStream_t s, int chunk)
 {
-    switch (chunk) {
+    switch (chunk) {  // the chunk sizes swr_api.cpp's expand_chunk_of names
     case 256: return launch_expand_ch<256>(a, s);
     case 512: return launch_expand_ch<512>(a, s);
     case 2048: return launch_expand_ch<2048>(a, s);

@@ -10,6 +10,8 @@ struct SortBuffers {
     uint32_t *hist, *hist_ex, *scan_tmp;
     uint32_t *ranges;
     uint32_t blocks;
+    const uint32_t *chunk_kept = nullptr;  // pairs kept per expansion chunk (nullptr: one run)
+    uint32_t chunk = 0;                    // slots per expansion chunk
     uint32_t *sorted_keys, *sorted_vals;  // set by sort_pairs
 };
 

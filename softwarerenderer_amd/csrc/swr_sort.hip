@@ -320,8 +320,9 @@ hipError_t scan_fans_and_pairs(const uint2 *app, uint2 *app_ex, uint64_t na, uin
 // ---- radix sort of (tile, primitive) pairs ---------------------------------------------
 constexpr int RADIX_THREADS = 256;
 
-// Pass 0 reads all emitted pairs (ST_PAIRS) and drops the rejected ones (keys >= tiles);
-// later passes and the ranges read the kept ones (ST_KEPT).
+// Pass 0 reads the emitted pairs (ST_PAIRS slots: compacted per expansion chunk, or one run
+// with rejected pairs keyed >= tiles, which it drops); later passes and the ranges read the
+// kept ones (ST_KEPT).
 __device__ inline uint64_t pair_count(const unsigned long long *status, int word)
 {
     return (status[ST_OVERFLOW] | status[ST_ERROR]) ? 0 : status[word];
@@ -338,21 +339,104 @@ __device__ inline void chunk_of(const unsigned long long *status, int word, uint
     if (end > m) end = m;
 }
 
+// The expansion's output: chunk c of `size` slots holds its kept[c] pairs at its start
+// (the tiles a primitive cannot shade are already dropped).  Radix block b of G owns the
+// chunks [k0, k1) and reads their pairs in order as one run, RADIX_THREADS chunks at a
+// time: s_off holds the group's exclusive prefix of kept counts, a run position maps to its
+// chunk by a binary search.
+struct Chunks {
+    const uint32_t *kept;  // nullptr: the pairs are one contiguous run (chunk_of)
+    uint32_t size;
+};
+
+__device__ inline void chunks_of(const unsigned long long *status, uint32_t size, uint32_t G,
+                                 uint32_t b, uint64_t &k0, uint64_t &k1)
+{
+    const uint64_t m = pair_count(status, ST_PAIRS);
+    const uint64_t n = (m + size - 1) / size, per = (n + G - 1) / G;
+    k0 = (uint64_t)b * per;
+    k1 = k0 + per;
+    if (k0 > n) k0 = n;
+    if (k1 > n) k1 = n;
+}
+
+// Loads the kept counts of chunks [g0, g0 + RADIX_THREADS) ∩ [., k1) and their exclusive
+// prefix into s_off[0 .. RADIX_THREADS]; returns the group's chunk count.
+__device__ inline uint32_t load_chunk_group(const Chunks &ch, uint64_t g0, uint64_t k1,
+                                            uint32_t *s_off)
+{
+    const uint32_t nk = (uint32_t)(k1 - g0 < RADIX_THREADS ? k1 - g0 : RADIX_THREADS);
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    __shared__ uint32_t s_w[RADIX_THREADS / 64];
+    const uint32_t c = tid < nk ? ch.kept[g0 + tid] : 0u;
+    uint32_t inc = c;
+#pragma unroll
+    for (int sh = 1; sh < 64; sh <<= 1) {
+        const uint32_t o = __shfl_up(inc, sh, 64);
+        inc += lane >= (uint32_t)sh ? o : 0u;
+    }
+    if (lane == 63)
+        s_w[w] = inc;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < RADIX_THREADS / 64; ++k) {
+        pre += k < (int)w ? s_w[k] : 0u;
+        tot += s_w[k];
+    }
+    s_off[tid] = pre + inc - c;
+    if (tid == 0)
+        s_off[RADIX_THREADS] = tot;
+    __syncthreads();
+    return nk;
+}
+
+// The slot of run position v of the group (v < s_off[nk]).
+__device__ inline uint64_t chunk_slot(const Chunks &ch, uint64_t g0, uint32_t nk,
+                                      const uint32_t *s_off, uint32_t v)
+{
+    uint32_t lo = 0, hi = nk;  // last chunk with s_off <= v
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_off[mid] <= v)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return (g0 + lo) * (uint64_t)ch.size + (v - s_off[lo]);
+}
+
 __global__ void __launch_bounds__(RADIX_THREADS) k_radix_hist(const uint32_t *keys,
                                                             const unsigned long long *status,
                                                             int word, int shift, uint32_t tiles,
-                                                            uint32_t *hist)
+                                                            uint32_t *hist, Chunks ch)
 {
     __shared__ uint32_t h[256];
+    __shared__ uint32_t s_off[RADIX_THREADS + 1];
     const uint32_t G = gridDim.x, b = blockIdx.x;
     h[threadIdx.x] = 0;
     __syncthreads();
-    uint64_t beg, end;
-    chunk_of(status, word, G, b, beg, end);
-    for (uint64_t i = beg + threadIdx.x; i < end; i += RADIX_THREADS) {
-        uint32_t k = keys[i];
-        if (k < tiles)
-            atomicAdd(&h[(k >> shift) & 255u], 1u);
+    if (ch.kept) {
+        uint64_t k0, k1;
+        chunks_of(status, ch.size, G, b, k0, k1);
+        for (uint64_t g0 = k0; g0 < k1; g0 += RADIX_THREADS) {
+            const uint32_t nk = load_chunk_group(ch, g0, k1, s_off);
+            const uint32_t tot = s_off[RADIX_THREADS];
+            for (uint32_t v = threadIdx.x; v < tot; v += RADIX_THREADS) {
+                const uint32_t k = keys[chunk_slot(ch, g0, nk, s_off, v)];
+                if (k < tiles)
+                    atomicAdd(&h[(k >> shift) & 255u], 1u);
+            }
+            __syncthreads();  // s_off is reloaded by the next group
+        }
+    } else {
+        uint64_t beg, end;
+        chunk_of(status, word, G, b, beg, end);
+        for (uint64_t i = beg + threadIdx.x; i < end; i += RADIX_THREADS) {
+            uint32_t k = keys[i];
+            if (k < tiles)
+                atomicAdd(&h[(k >> shift) & 255u], 1u);
+        }
     }
     __syncthreads();
     hist[(size_t)threadIdx.x * G + b] = h[threadIdx.x];
@@ -364,36 +448,28 @@ __global__ void __launch_bounds__(RADIX_THREADS) k_radix_hist(const uint32_t *ke
 // in lower lanes of its wave (eight ballots find the lanes with the same digit).
 constexpr int RADIX_ITEMS = 4;
 
-__global__ void __launch_bounds__(RADIX_THREADS) k_radix_scatter(
-    const uint32_t *keys, const uint32_t *vals, uint32_t *okeys, uint32_t *ovals,
-    const unsigned long long *status, int word, int shift, uint32_t tiles, const uint32_t *hist_ex)
+// One round of RADIX_ITEMS * RADIX_THREADS pairs: run positions [base, end), position i at
+// slot slot_of(i); run[] carries each digit's next output position between rounds.
+template <class SlotOf>
+__device__ inline void scatter_round(const uint32_t *keys, const uint32_t *vals, uint32_t *okeys,
+                                     uint32_t *ovals, int shift, uint32_t tiles, uint64_t cap,
+                                     uint64_t base, uint64_t end, SlotOf slot_of, uint32_t *run,
+                                     uint32_t (*wcnt)[RADIX_THREADS / 64][256],
+                                     uint32_t (*off)[RADIX_THREADS / 64][256])
 {
     constexpr int NW = RADIX_THREADS / 64;
-    __shared__ uint32_t run[256];
-    __shared__ uint32_t wcnt[RADIX_ITEMS][NW][256];  // per (chunk, wave, digit) counts
-    __shared__ uint32_t off[RADIX_ITEMS][NW][256];   // their exclusive prefix from run
-    const uint32_t G = gridDim.x, b = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    run[tid] = hist_ex[(size_t)tid * G + b];
-#pragma unroll
-    for (int j = 0; j < RADIX_ITEMS; ++j)
-#pragma unroll
-        for (int q = 0; q < NW; ++q)
-            wcnt[j][q][tid] = 0;
-    __syncthreads();
-    uint64_t beg, end;
-    chunk_of(status, word, G, b, beg, end);
-    const uint64_t cap = pair_count(status, ST_PAIRS);
     const unsigned long long below = (1ull << lane) - 1ull;
-    for (uint64_t base = beg; base < end; base += (uint64_t)RADIX_THREADS * RADIX_ITEMS) {
+    {
         uint32_t k[RADIX_ITEMS], v[RADIX_ITEMS], d[RADIX_ITEMS], rank[RADIX_ITEMS];
         bool valid[RADIX_ITEMS];
 #pragma unroll
         for (int j = 0; j < RADIX_ITEMS; ++j) {
             const uint64_t i = base + (uint64_t)j * RADIX_THREADS + tid;
-            k[j] = i < end ? keys[i] : 0u;
+            const uint64_t slot = i < end ? slot_of(i) : 0;
+            k[j] = i < end ? keys[slot] : 0u;
             valid[j] = i < end && k[j] < tiles;
-            v[j] = valid[j] ? vals[i] : 0u;
+            v[j] = valid[j] ? vals[slot] : 0u;
             d[j] = (k[j] >> shift) & 255u;
         }
 #pragma unroll
@@ -435,6 +511,47 @@ __global__ void __launch_bounds__(RADIX_THREADS) k_radix_scatter(
     }
 }
 
+__global__ void __launch_bounds__(RADIX_THREADS) k_radix_scatter(
+    const uint32_t *keys, const uint32_t *vals, uint32_t *okeys, uint32_t *ovals,
+    const unsigned long long *status, int word, int shift, uint32_t tiles, const uint32_t *hist_ex,
+    Chunks ch)
+{
+    constexpr int NW = RADIX_THREADS / 64;
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[RADIX_ITEMS][NW][256];  // per (chunk, wave, digit) counts
+    __shared__ uint32_t off[RADIX_ITEMS][NW][256];   // their exclusive prefix from run
+    const uint32_t G = gridDim.x, b = blockIdx.x;
+    const int tid = threadIdx.x;
+    run[tid] = hist_ex[(size_t)tid * G + b];
+#pragma unroll
+    for (int j = 0; j < RADIX_ITEMS; ++j)
+#pragma unroll
+        for (int q = 0; q < NW; ++q)
+            wcnt[j][q][tid] = 0;
+    __syncthreads();
+    const uint64_t cap = pair_count(status, ST_PAIRS);
+    if (ch.kept) {
+        __shared__ uint32_t s_off[RADIX_THREADS + 1];
+        uint64_t k0, k1;
+        chunks_of(status, ch.size, G, b, k0, k1);
+        for (uint64_t g0 = k0; g0 < k1; g0 += RADIX_THREADS) {
+            const uint32_t nk = load_chunk_group(ch, g0, k1, s_off);
+            const uint32_t tot = s_off[RADIX_THREADS];
+            for (uint64_t base = 0; base < tot; base += (uint64_t)RADIX_THREADS * RADIX_ITEMS)
+                scatter_round(keys, vals, okeys, ovals, shift, tiles, cap, base, tot,
+                              [&](uint64_t i) { return chunk_slot(ch, g0, nk, s_off, (uint32_t)i); },
+                              run, wcnt, off);
+            __syncthreads();  // s_off is reloaded by the next group
+        }
+    } else {
+        uint64_t beg, end;
+        chunk_of(status, word, G, b, beg, end);
+        for (uint64_t base = beg; base < end; base += (uint64_t)RADIX_THREADS * RADIX_ITEMS)
+            scatter_round(keys, vals, okeys, ovals, shift, tiles, cap, base, end,
+                          [](uint64_t i) { return i; }, run, wcnt, off);
+    }
+}
+
 // Per-tile [begin, end) into the sorted kept pairs; tiles without pairs keep the zeros the
 // begin kernel wrote.
 __global__ void __launch_bounds__(256) k_ranges(const uint32_t *keys,
@@ -471,8 +588,10 @@ hipError_t sort_pairs(SortBuffers &sb, const unsigned long long *status, uint32_
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
         const int word = p == 0 ? ST_PAIRS : ST_KEPT;
+        // pass 0 reads the expansion's chunks (when it compacted them), later passes one run
+        Chunks ch{p == 0 ? sb.chunk_kept : nullptr, sb.chunk};
         hipLaunchKernelGGL(k_radix_hist, dim3(sb.blocks), dim3(RADIX_THREADS), 0, s, ka, status,
-                           word, shift, tiles, sb.hist);
+                           word, shift, tiles, sb.hist, ch);
         // pass 0's histogram total is the number of kept pairs (ST_KEPT)
         hipError_t e = scan_impl<uint32_t>(sb.hist, sb.hist_ex, nh, sb.scan_tmp,
                                            p == 0 ? (unsigned long long *)status : nullptr,
@@ -480,7 +599,7 @@ hipError_t sort_pairs(SortBuffers &sb, const unsigned long long *status, uint32_
         if (e != hipSuccess)
             return e;
         hipLaunchKernelGGL(k_radix_scatter, dim3(sb.blocks), dim3(RADIX_THREADS), 0, s, ka, va, kb,
-                           vb, status, word, shift, tiles, sb.hist_ex);
+                           vb, status, word, shift, tiles, sb.hist_ex, ch);
         uint32_t *t = ka;
         ka = kb;
         kb = t;

@@ -100,6 +100,26 @@ def test_batch_matches_single_draws_and_oracle(gpu, cpu_oracle):
         assert int(g.t1.sum()) > 0 or sc.triangles == 0
 
 
+@pytest.mark.parametrize("tune", [dict(bin_size=16, waves_per_bin=2),
+                                  dict(bin_size=32, waves_per_bin=4, raster_group=2)],
+                         ids=["bin16", "bin32"])
+@pytest.mark.parametrize("ps", [O.PS_HASH, O.PS_DEPTH_CHECKER])
+def test_batch_larger_bins(gpu, cpu_oracle, tune, ps):
+    """Every scene of a mixed batch with 16x16 and 32x32 screen bins (each scene's bins
+    numbered after the previous scenes'), against the oracle."""
+    from softwarerenderer_amd import api
+    cases = mvp_cases(ps=ps)
+    r = api.Rasterizer(0)
+    r.setTuning(**tune)
+    try:
+        got = run_batch(cases, rasterizer=r)
+    finally:
+        r.close()
+    for i, ((sc, pipe), g) in enumerate(zip(cases, got)):
+        if sc.triangles <= 40_000:
+            assert_same(run_cpu(cpu_oracle, pipe, sc), g, f"scene {i}: {tune} vs oracle")
+
+
 def test_batch_depth_checker(gpu, cpu_oracle):
     """The depth-checker program in a batch: each pixel chains its winning fragment's planes
     again when its tile is written, with its own scene's scissor and pitch."""

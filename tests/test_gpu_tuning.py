@@ -10,11 +10,21 @@ from helpers import assert_same, run_cpu, run_gpu
 pytestmark = pytest.mark.gpu
 
 TUNINGS = [
-    dict(raster_waves=1, raster_group=1, xcd_order=-1, expand_chunk=256),
-    dict(raster_waves=2, raster_group=2, xcd_order=1, expand_chunk=512),
-    dict(raster_waves=8, raster_group=4, xcd_order=1, expand_chunk=2048),
-    dict(raster_waves=4, raster_group=8, xcd_order=-1, expand_chunk=4096),
+    dict(raster_waves=1, raster_group=1, xcd_order=-1, expand_chunk=256, vertex_pass=1),
+    dict(raster_waves=2, raster_group=2, xcd_order=1, expand_chunk=512, vertex_pass=-1),
+    dict(raster_waves=8, raster_group=4, xcd_order=1, expand_chunk=2048, bin_size=16,
+         waves_per_bin=1),
+    dict(raster_waves=4, raster_group=8, xcd_order=-1, expand_chunk=4096, bin_size=32,
+         waves_per_bin=4),
+    dict(bin_size=16, waves_per_bin=4, raster_group=2, vertex_pass=1),
+    dict(bin_size=32, waves_per_bin=2, expand_chunk=256),
 ]
+KNOB_DEFAULTS = dict(raster_waves=0, raster_group=0, xcd_order=0, expand_chunk=0, vertex_pass=0,
+                     bin_size=0, waves_per_bin=0)
+
+
+def _tid(t):
+    return "-".join(f"{k[:3]}{v}" for k, v in t.items())
 
 
 @pytest.fixture(scope="module")
@@ -31,11 +41,11 @@ def _rasterizer(tune):
     from softwarerenderer_amd import api
     r = api.Rasterizer(0)
     r.setTuning(**tune)
-    assert r.tuning() == tune
+    assert r.tuning() == dict(KNOB_DEFAULTS, **tune)
     return r
 
 
-@pytest.mark.parametrize("tune", TUNINGS, ids=lambda t: "w{raster_waves}g{raster_group}x{xcd_order}c{expand_chunk}".format(**t))
+@pytest.mark.parametrize("tune", TUNINGS, ids=_tid)
 @pytest.mark.parametrize("prog", [O.PS_LOG, O.PS_DEPTH_CHECKER, O.PS_VISIBILITY])
 def test_sphere_tuned(gpu, cpu_oracle, sphere, tune, prog):
     pipe = O.Pipeline(640, 480, raster_mode=O.RASTER_BLOCK, mvp=sphere.mvp, ps_program=prog)
@@ -49,7 +59,23 @@ def test_sphere_tuned(gpu, cpu_oracle, sphere, tune, prog):
     assert_same(a, b, f"sphere prog={prog} tuning={tune}")
 
 
-@pytest.mark.parametrize("tune", TUNINGS[:2] + TUNINGS[3:])
+@pytest.mark.parametrize("tune", TUNINGS, ids=_tid)
+@pytest.mark.parametrize("mode", [O.RASTER_SPAN, O.RASTER_ADAPTIVE])
+def test_sphere_span_tuned(gpu, cpu_oracle, sphere, tune, mode):
+    """Span and Adaptive rows under every bin size: the hash program sees every fragment's
+    values in order."""
+    pipe = O.Pipeline(640, 480, raster_mode=mode, mvp=sphere.mvp, ps_program=O.PS_HASH,
+                      scissor=(13, 7, 600, 450))
+    a = run_cpu(cpu_oracle, pipe, sphere)
+    r = _rasterizer(tune)
+    try:
+        b = run_gpu(pipe, sphere, rasterizer=r)
+    finally:
+        r.close()
+    assert_same(a, b, f"sphere span mode={mode} tuning={tune}")
+
+
+@pytest.mark.parametrize("tune", TUNINGS, ids=_tid)
 def test_clip_tuned(gpu, cpu_oracle, clip, tune):
     pipe = O.Pipeline(clip.width, clip.height, raster_mode=O.RASTER_SPAN, cull_mode=O.CULL_NONE,
                       vs_program=O.VS_RAW, vs_counts=(1, 3, 2), ps_program=O.PS_HASH)
@@ -67,7 +93,7 @@ def test_soup_tuned(gpu, cpu_oracle):
     pipe = O.Pipeline(1920, 1080, raster_mode=O.RASTER_BLOCK, mvp=sc.mvp,
                       ps_program=O.PS_DEPTH_CHECKER)
     a = run_cpu(cpu_oracle, pipe, sc)
-    for tune in (TUNINGS[0], TUNINGS[2]):
+    for tune in TUNINGS:
         r = _rasterizer(tune)
         try:
             b = run_gpu(pipe, sc, rasterizer=r)
@@ -78,7 +104,9 @@ def test_soup_tuned(gpu, cpu_oracle):
 
 @pytest.mark.parametrize("bad", [dict(raster_waves=3), dict(raster_waves=16),
                                  dict(raster_group=5), dict(expand_chunk=128),
-                                 dict(expand_chunk=3000), dict(xcd_order=2)])
+                                 dict(expand_chunk=3000), dict(xcd_order=2),
+                                 dict(vertex_pass=2), dict(bin_size=24), dict(bin_size=64),
+                                 dict(waves_per_bin=8), dict(waves_per_bin=3)])
 def test_invalid_tuning(gpu, bad):
     from softwarerenderer_amd import _abi, api
     r = api.Rasterizer(0)
@@ -117,8 +145,8 @@ def test_autotune_small(gpu, cpu_oracle, sphere):
 
         v.drawElements(api.DrawMode.Triangle, idx)
         res = autotune(r, draw, space={"raster_waves": (2, 4), "raster_group": (0, 2),
-                                       "expand_chunk": (1024,)}, reps=2, warmup=1, rounds=1)
-        assert len(res["table"]) == 2 * 2 * 2 * 1
+                                       "bin_size": (8, 16)}, reps=2, warmup=1, rounds=1)
+        assert len(res["table"]) == 2 * 2 * 2
         assert r.tuning() == res["best"]
         assert res["ms"] > 0
         draw()
