@@ -225,6 +225,19 @@ int  swr_draw_elements(swr_context *ctx, int mode, size_t count, const int32_t *
 int  swr_draw_elements_async(swr_context *ctx, int mode, size_t count, const int32_t *indices);
 int  swr_reserve(swr_context *ctx, size_t primitives, size_t tile_pairs);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * triangle draw of swr_draw_elements_async (same arguments, same results, same
+ * device-buffers-only rule, failures reported by swr_synchronize) whose working memory is
+ * carved from one fixed array per GPU of SWR_NOALLOC_SCRATCH_BYTES instead of the context's
+ * growable arenas: it calls no hipMalloc, hipFree or memset.  Draws whose input-sized
+ * arenas (about 64 bytes per triangle) and records (one per input triangle, plus an eighth)
+ * do not fit fail at once with SWR_ERR_CAPACITY; tile pairs take what remains, and a draw
+ * that emits more fails at the next swr_synchronize with SWR_ERR_CAPACITY (nothing drawn).
+ * Every call on one GPU uses that GPU's one array: calls that overlap on one GPU (two
+ * streams, two contexts, two threads) race and must be ordered by the caller. */
+#define SWR_NOALLOC_SCRATCH_BYTES (64 << 20)
+int  swr_draw_elements_noalloc(swr_context *ctx, int mode, size_t count, const int32_t *indices);
 
 /* ---- batched draws --------------------------------------------------------------------
  * No counterpart in the reference: a batch stands for a sequence of independent

@@ -100,6 +100,7 @@ SIGNATURES = {
     "swr_draw_elements": (_I, [_P, _I, _S, _P]),
     "swr_draw_elements_async": (_I, [_P, _I, _S, _P]),
     "swr_reserve": (_I, [_P, _S, _S]),
+    "swr_draw_elements_noalloc": (_I, [_P, _I, _S, _P]),
     "swr_get_draw_stats": (_I, [_P, C.POINTER(DrawStats)]),
     "swr_set_profiling": (_I, [_P, _I]),
     "swr_batch_create": (_I, [_P, C.POINTER(Scene), _S, C.POINTER(C.c_void_p)]),

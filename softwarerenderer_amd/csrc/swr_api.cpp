@@ -18,6 +18,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -449,11 +450,110 @@ int expand_chunk_of(int tuned)
 }
 size_t hist_tmp_bytes(uint32_t G) { return scan_tmp_elems((uint64_t)256 * G) * 4; }
 
+// ---- the no-allocation draw's scratch ------------------------------------------------------
+// One array per GPU, in the code object's device memory: swr_draw_elements_noalloc carves all
+// of its arenas from it and allocates nothing.  In host code `scratch` names a host-side
+// stand-in, not device memory: its device address is looked up once per device
+// (hipGetSymbolAddress with that device current, at swr_create) and only that address is
+// handed to kernels.
+static __device__ __align__(256) unsigned char scratch[SWR_NOALLOC_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[MAX_DEVICES] = {};
+bool g_scratch_done[MAX_DEVICES] = {};
+
+// The calling thread's current device is `device`.  Returns hipSuccess once the address is
+// known (the lookup runs once per device).
+hipError_t resolve_scratch(int device)
+{
+    if (device < 0 || device >= MAX_DEVICES)
+        return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_done[device])
+        return g_scratch_ptr[device] ? hipSuccess : hipErrorInvalidSymbol;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_done[device] = true;
+    g_scratch_ptr[device] = e == hipSuccess ? p : nullptr;
+    if (e != hipSuccess)
+        (void)hipGetLastError();
+    return e;
+}
+
+void *scratch_of(int device)
+{
+    if (device < 0 || device >= MAX_DEVICES)
+        return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
+// Lays every arena of a draw out in [base, base + bytes): the input-sized ones first, then
+// primitive records and tile pairs from what remains.  Returns false when the input-sized
+// arenas and a minimal record / pair capacity do not fit.
+bool carve_arenas(unsigned char *base, size_t bytes, size_t T, uint64_t ngrp, uint32_t tiles,
+                  int stride_words, Arenas &A)
+{
+    const FixedSizes f = fixed_sizes(T, ngrp, tiles);
+    size_t off = 0;
+    auto take = [&](size_t n) -> unsigned char * {
+        unsigned char *p = base + off;
+        off += (std::max<size_t>(n, 1) + 255) & ~(size_t)255;
+        return p;
+    };
+    const size_t fixed_total = 2 * ((f.app + 255) & ~(size_t)255) + ((f.live + 255) & ~(size_t)255) +
+                               ((f.clipq + 255) & ~(size_t)255) +
+                               ((f.clipq_cnt + 255) & ~(size_t)255) +
+                               2 * ((f.grp + 255) & ~(size_t)255) +
+                               ((f.scan_tmp + 255) & ~(size_t)255) + ((f.ranges + 255) & ~(size_t)255);
+    if (fixed_total >= bytes)
+        return false;
+    A.app = (uint2 *)take(f.app);
+    A.app_ex = (uint2 *)take(f.app);
+    A.live = (uint32_t *)take(f.live);
+    A.clipq = (uint32_t *)take(f.clipq);
+    A.clipq_cnt = (uint32_t *)take(f.clipq_cnt);
+    A.grp = (uint2 *)take(f.grp);
+    A.grp_ex = (uint2 *)take(f.grp);
+    A.scan_tmp = take(f.scan_tmp);
+    A.ranges = (uint32_t *)take(f.ranges);
+    // records: every input triangle and an eighth more for clip fans; pairs: the rest
+    const size_t rec_bytes = (size_t)stride_words * 4 + 4;  // record + its first-pair index
+    A.prim_cap = T + T / 8 + 256;
+    const size_t slack = 12 * 256;  // alignment of the arenas below
+    if (off + A.prim_cap * rec_bytes + slack >= bytes)
+        return false;
+    size_t rest = bytes - off - A.prim_cap * rec_bytes - slack;
+    uint64_t pairs = rest / 16;
+    A.G = sort_blocks(pairs);
+    const size_t hb = 2 * hist_bytes(A.G) + hist_tmp_bytes(A.G) + chunk_kept_bytes(pairs) + 256;
+    if (hb >= rest)
+        return false;
+    pairs = (rest - hb) / 16;
+    if (pairs < 4096)
+        return false;
+    A.pair_cap = pairs;
+    A.prims = (float *)take(A.prim_cap * (size_t)stride_words * 4);
+    A.bigq = (uint32_t *)take(A.prim_cap * 4);
+    A.pk_a = (uint32_t *)take(A.pair_cap * 4);
+    A.pv_a = (uint32_t *)take(A.pair_cap * 4);
+    A.pk_b = (uint32_t *)take(A.pair_cap * 4);
+    A.pv_b = (uint32_t *)take(A.pair_cap * 4);
+    A.hist = (uint32_t *)take(hist_bytes(A.G));
+    A.hist_ex = (uint32_t *)take(hist_bytes(A.G));
+    A.hist_tmp = (uint32_t *)take(hist_tmp_bytes(A.G));
+    A.chunk_kept = (uint32_t *)take(chunk_kept_bytes(A.pair_cap));
+    return off <= bytes;
+}
+
 // The triangle draw: drawElements (window_input = false), the IRasterizer seam, or a batch
 // of scenes (bt != nullptr: inputs, state and targets come from the scene table).
+// noalloc: the asynchronous draw whose arenas are carved from the device's scratch array
+// (nothing is allocated, freed or set by a memset; sizes beyond the array fail at once).
 int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool window_input,
                    const swr_rasterizer_vertex *wverts, size_t nverts, bool sync,
-                   const swr_batch *bt = nullptr)
+                   const swr_batch *bt = nullptr, bool noalloc = false)
 {
     const size_t T = bt ? bt->vtris : count / 3;
     std::memset(&c->stats, 0, sizeof c->stats);
@@ -543,7 +643,16 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
     const uint32_t batches = (uint32_t)((T + BATCH - 1) / BATCH);
     const uint64_t ngrp = (uint64_t)batches * 2048;
     Arenas A;
-    {
+    if (noalloc) {
+        unsigned char *base = static_cast<unsigned char *>(scratch_of(c->device));
+        if (!base)
+            return fail(c, SWR_ERR_HIP, "the device address of the scratch array is unknown");
+        if (!carve_arenas(base, SWR_NOALLOC_SCRATCH_BYTES, T, ngrp, tiles, L.stride, A))
+            return fail(c, SWR_ERR_CAPACITY,
+                        "%zu triangles on %u tiles do not fit the %d-byte scratch array of "
+                        "swr_draw_elements_noalloc",
+                        T, tiles, SWR_NOALLOC_SCRATCH_BYTES);
+    } else {
         const FixedSizes f = fixed_sizes(T, ngrp, tiles);
         HIP_TRY(c, ensure(c, c->app, f.app));
         HIP_TRY(c, ensure(c, c->clipq, f.clipq));
@@ -571,7 +680,7 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
     // count is known (tuning: 0 automatic = when vertices are referenced at least once on
     // average, 1 on, -1 off)
     uint32_t vcount = 0;
-    if (!bt && !window_input && c->vs.program != SWR_VS_DUMMY && attr_bytes &&
+    if (!bt && !window_input && !noalloc && c->vs.program != SWR_VS_DUMMY && attr_bytes &&
         attr_bytes >= need && c->attr[0].stride > 0 && c->tune.vertex_pass >= 0) {
         const uint64_t nv = (attr_bytes - need) / (uint64_t)c->attr[0].stride + 1;
         const bool want = c->tune.vertex_pass > 0 || nv <= 3 * (uint64_t)T;
@@ -583,7 +692,7 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
     }
 
     for (int attempt = 0; attempt < 4; ++attempt) {
-        {
+        if (!noalloc) {
             HIP_TRY(c, ensure(c, c->prims, c->prim_cap * (size_t)L.stride * 4));
             HIP_TRY(c, ensure(c, c->bigq, c->prim_cap * 4));
             HIP_TRY(c, ensure(c, c->pk_a, c->pair_cap * 4));
@@ -1077,6 +1186,9 @@ int swr_create(swr_context **out, int device)
     }
     std::memset(c->status_host, 0, ST_WORDS * 8);
     c->stream = c->own;
+    // the no-allocation draw's scratch array: its address on this device, once per device
+    // (a failure only disables swr_draw_elements_noalloc, which then reports it)
+    (void)resolve_scratch(device);
     // the status words start at zero (the sticky word is only ever cleared by the host)
     if (ensure(c, c->status, ST_WORDS * 8) != hipSuccess ||
         hipMemsetAsync(c->status.p, 0, c->status.bytes, c->stream) != hipSuccess ||
@@ -1274,6 +1386,16 @@ int swr_draw_elements_async(swr_context *c, int mode, size_t count, const int32_
         return fail(c, SWR_ERR_UNSUPPORTED, "async draws take triangles");
     (void)hipSetDevice(c->device);
     return draw_triangles(c, indices, count, false, nullptr, 0, false);
+}
+
+int swr_draw_elements_noalloc(swr_context *c, int mode, size_t count, const int32_t *indices)
+{
+    if (!c || (count && !indices))
+        return fail(c, SWR_ERR_INVALID_ARG, "null indices");
+    if (mode != SWR_DRAW_TRIANGLE)
+        return fail(c, SWR_ERR_UNSUPPORTED, "the no-allocation draw takes triangles");
+    (void)hipSetDevice(c->device);
+    return draw_triangles(c, indices, count, false, nullptr, 0, false, nullptr, true);
 }
 
 int swr_reserve(swr_context *c, size_t primitives, size_t tile_pairs)
