@@ -212,9 +212,9 @@ def diff_bench(args, dev):
 
     def step(rr, m, tgt):
         clip = torch.einsum("vij,nj->vni", m, hom).contiguous()
-        img = rr.forward(clip, col)
-        g = img - tgt  # dL/dimage of L = 1/2 |image - target|^2
-        gclip, gattr = rr.backward(g)
+        rr.forward(clip, col)
+        # L = 1/2 |image - target|^2: dL/dimage = image - target, formed in the backward
+        gclip, gattr = rr.backward_l2(tgt)
         gpos = torch.einsum("vni,vij->nj", gclip, m)[:, :3]
         return gpos, gattr
 

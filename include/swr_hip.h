@@ -304,11 +304,14 @@ int  swr_batch_destroy(swr_batch *batch);
 #define SWR_DIFF_CULLED 1          /* every visible triangle faces the same way (the visibility
                                       draw culled one winding): pixels of two triangles that
                                       share a vertex are never separated by a silhouette */
+#define SWR_DIFF_L2_TARGET 2       /* swr_diff_backward: grad_image holds a target image T and
+                                      the loss is 1/2 |image - T|^2, so dL/dimage = image - T
+                                      (formed where it is read, from the forward's image) */
 typedef struct swr_diff_desc {
     int32_t views, width, height;
     int32_t vertex_count, triangle_count;
     int32_t channels;              /* 1..4 */
-    int32_t flags;                 /* SWR_DIFF_CULLED                                      */
+    int32_t flags;                 /* SWR_DIFF_CULLED, SWR_DIFF_L2_TARGET                  */
     int32_t scenes;                /* 0 or 1: one mesh seen by every view; > 1: see above  */
     const float    *clip;          /* [views][vertex_count][4] clip-space x, y, z, w       */
     const int32_t  *triangles;     /* [triangle_count][3]                                  */
@@ -318,7 +321,9 @@ typedef struct swr_diff_desc {
     const uint32_t *ids;           /* [views][height][width]: triangle + 1, 0 = background */
     const float    *depth;         /* [views][height][width]                               */
     float *bary;                   /* [views][height][width][2]: forward out, backward in  */
-    float *color;                  /* [views][height][width][channels], before antialiasing */
+    float *color;                  /* optional [views][height][width][channels]: forward out,
+                                      the colour before antialiasing (NULL: not written; the
+                                      kernels recompute the few they need)                 */
     float *image;                  /* [views][height][width][channels]: forward out        */
     const int32_t *scene_views;    /* scenes > 1: host [scenes + 1]                        */
     const int32_t *scene_vertices; /* scenes > 1: host [scenes + 1]                        */
@@ -341,7 +346,7 @@ typedef struct swr_tuning {
     int32_t vertex_pass;     /* 1: shade every vertex of attribute 0 once into post-transform
                                 vertices, -1: shade each triangle corner where it is used,
                                 0: automatic (the pass when the buffer's vertex count is known
-                                and at most three per triangle) */
+                                and each vertex is referenced at least twice on average) */
     int32_t bin_size;        /* screen bins of 8, 16 or 32 pixels (8) */
     int32_t waves_per_bin;   /* bins of 16 or 32 pixels: waves sharing a bin's 8x8 blocks:
                                 1, 2, 4 (4) */

@@ -677,13 +677,13 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
     }
 
     // the vertex pass: every vertex of attribute 0 shaded once, when the buffer's vertex
-    // count is known (tuning: 0 automatic = when vertices are referenced at least once on
-    // average, 1 on, -1 off)
+    // count is known (tuning: 0 automatic = when each vertex is referenced at least twice on
+    // average, so that the pass saves more vertex-program runs than it adds; 1 on, -1 off)
     uint32_t vcount = 0;
     if (!bt && !window_input && !noalloc && c->vs.program != SWR_VS_DUMMY && attr_bytes &&
         attr_bytes >= need && c->attr[0].stride > 0 && c->tune.vertex_pass >= 0) {
         const uint64_t nv = (attr_bytes - need) / (uint64_t)c->attr[0].stride + 1;
-        const bool want = c->tune.vertex_pass > 0 || nv <= 3 * (uint64_t)T;
+        const bool want = c->tune.vertex_pass > 0 || 2 * nv <= 3 * (uint64_t)T;
         if (want && nv < (1ull << 31)) {
             vcount = (uint32_t)nv;
             HIP_TRY(c, ensure(c, c->vwin, (size_t)vcount * 16));
@@ -752,6 +752,8 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
         g.tiles_x = tiles_x;
         g.tiles_y = tiles_y;
         g.bin_shift = bin_shift;
+        // 16x16 bins carry their blocks' classes when primitive indices fit 24 bits
+        g.cls4 = (bin_shift == 4 && A.prim_cap <= (uint64_t)PAIR_CLS4_PRIM_MASK + 1) ? 1 : 0;
         g.lay = L;
         g.window_input = window_input ? 1 : 0;
         g.wverts = (const swr_rasterizer_vertex *)dwv;
@@ -827,6 +829,7 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
         r.tiles = (int)tiles;
         r.tiles_x = tiles_x;
         r.bin_shift = bin_shift;
+        r.cls4 = g.cls4;
         r.waves_per_bin = c->tune.waves_per_bin ? c->tune.waves_per_bin : 4;
         if (r.waves_per_bin > (1 << (2 * lvl)))
             r.waves_per_bin = 1 << (2 * lvl);

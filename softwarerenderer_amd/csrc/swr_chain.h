@@ -28,6 +28,11 @@ namespace swr {
 // (round to nearest even, subnormals kept, no contraction).
 __host__ __device__ inline float chain_add(float v, float a, int n)
 {
+    if (n <= 16) {  // short chains: the additions themselves are cheaper
+        for (int k = 0; k < n; ++k)
+            v = v + a;
+        return v;
+    }
     int binade = 0x7fffffff;  // exponent of the binade the evenness below refers to
     bool even = false;        // v / u is known to be even (a tie's additions are regular)
     while (n > 0) {

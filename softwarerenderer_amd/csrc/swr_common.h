@@ -175,8 +175,29 @@ __device__ inline bool span_row_extent(const float *rec, uint32_t flags, int y, 
     return inb || intop;
 }
 
+// Span: true when some scanline of the 8x8 block at (bx, by) has pixels of the primitive
+// inside the block's columns (the ranges the rasterizer computes, Rasterizer.h:370-380,
+// 397-407).
+__device__ inline bool span_block_any(const float *rec, uint32_t flags, int bx, int by,
+                                      int sc_minx, int sc_maxx)
+{
+    bool any = false;
+    for (int r = 0; r < TILE && !any; ++r) {
+        int xl, xr;
+        const bool rowv = span_row_extent(rec, flags, by + r, sc_minx, sc_maxx, xl, xr);
+        any = rowv && imax(xl, bx) < imin(xr, bx + TILE);
+    }
+    return any;
+}
+
 // pair value: primitive index | classification << 30 (Block primitives)
 constexpr uint32_t PAIR_PRIM_MASK = 0x3FFFFFFFu;
+// 16x16 bins whose pair values carry the classes of the bin's four 8x8 blocks: primitive
+// index (24 bits) | class of block b << (24 + 2 b), blocks in row-major order (Block: the
+// corner classification of Rasterizer.h:270-304 after block_may_cover; Span: 1 when some
+// scanline of the block has pixels of the primitive)
+constexpr int PAIR_CLS4_SHIFT = 24;
+constexpr uint32_t PAIR_CLS4_PRIM_MASK = 0x00FFFFFFu;
 
 // ---- batched draws ------------------------------------------------------------------
 // One scene of a batched draw: its inputs, its VertexProcessor / Rasterizer state and its
@@ -218,6 +239,7 @@ struct GeomArgs {
     float dz_scale, dz_bias;                  // 0.5f*(f-n), 0.5f*(n+f)
     int width, height, tiles_x, tiles_y;  // tiles_x, tiles_y, tiles: bins of 1 << bin_shift px
     int bin_shift;              // 3, 4 or 5: screen bins of 8, 16 or 32 pixels
+    int cls4;                   // 16x16 bins: pair values carry the four block classes
     ParamLayout lay;
     int window_input;           // 1: vertices are window-space RasterizerVertex (IRasterizer seam)
     const swr_rasterizer_vertex *wverts;
@@ -271,6 +293,7 @@ struct RasterArgs {
     const unsigned long long *status;
     int tiles, tiles_x;         // bins of 1 << bin_shift pixels
     int bin_shift;
+    int cls4;                   // 16x16 bins: pair values carry the four block classes
     int waves_per_bin;          // bins larger than 8x8: waves sharing one bin's blocks
     int width, height, pitch;
     int sc_minx, sc_maxx;       // Span x clipping (Rasterizer.h:377-378, 404-405)

@@ -67,7 +67,8 @@ struct DiffArgs {
     float *bary;
     float *color;
     float *image;
-    const float *gimage;
+    const float *gimage;  // dL/dimage, or the target image T when l2_target
+    int l2_target;        // dL/dimage = image - T
     float *gclip;
     float *gattr;  // per-view partial attribute gradients [V][N][C], summed afterwards
     float *gattr_out;
@@ -92,6 +93,14 @@ struct DiffArgs {
     const int *svert, *sview;
     int S;
 };
+
+// dL/d(image channel c of pixel i): the caller's image gradient, or image - target for the
+// fused L2 loss (SWR_DIFF_L2_TARGET), formed in the same fp32 operation torch would use.
+__device__ inline float grad_image(const DiffArgs &d, size_t i, int C, int c)
+{
+    const float g = d.gimage[i * C + c];
+    return d.l2_target ? d.image[i * C + c] - g : g;
+}
 
 // Clip / gradient row of vertex v (global number) as seen by `view`.
 __device__ inline size_t crow(const DiffArgs &d, int view, int v)
@@ -462,6 +471,32 @@ __device__ inline void bary_color(const DiffArgs &d, const Bary &B, double *col)
                  B.b[2] * d.attr[(size_t)B.v[2] * C + c];
 }
 
+// The colour interp_pixel gives pixel (px, py) of `view` before antialiasing (the attributes
+// at its barycentrics, 0 on the background, rounded to fp32): read from the colour buffer
+// when the caller keeps one, else computed again by the same operations (same bits).
+template <int C>
+__device__ inline void pixel_color(const DiffArgs &d, int view, size_t i, int px, int py,
+                                   double *out)
+{
+    if (d.color) {
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+            out[c] = d.color[i * C + c];
+        return;
+    }
+    double col[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+        col[c] = 0.0;
+    Bary B;
+    const int tri = id_tri(d, view, d.ids[i]);
+    if (tri >= 0 && bary_at(d, view, tri, px, py, B))
+        bary_color<C>(d, B, col);
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+        out[c] = (double)(float)col[c];
+}
+
 // Gradient accumulation of one workgroup (one 16x16 tile of one view): an open-addressing
 // table in LDS keyed by vertex, flushed to global memory once per distinct vertex.  The
 // pixels of a tile touch a few dozen vertices, so this replaces most global float atomics
@@ -708,7 +743,8 @@ __device__ inline void interp_pixel(const DiffArgs &d, const PixelIdx &pi, bool 
     reinterpret_cast<float2 *>(d.bary)[i] = bb;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-        d.color[i * C + c] = (float)col[c];
+        if (d.color)
+            d.color[i * C + c] = (float)col[c];
         d.image[i * C + c] = (float)col[c];  // the blend pass adds the antialiasing
     }
     if (threadIdx.x == 0) {
@@ -1010,8 +1046,8 @@ __global__ void __launch_bounds__(256) k_diff_blend(DiffArgs d)
     const int view = pi.view, px = pi.px, py = pi.py;
     double cp[C], out[C];
 #pragma unroll
-    for (int c = 0; c < C; ++c)
-        out[c] = cp[c] = d.color[i * C + c];
+    for (int c = 0; c < C; ++c)  // the image holds the colour until this thread blends it
+        out[c] = cp[c] = d.image[i * C + c];
     for (int k = 0; k < 4; ++k) {
         int qx, qy;
         if (!neighbour(d, px, py, k, qx, qy))
@@ -1023,9 +1059,7 @@ __global__ void __launch_bounds__(256) k_diff_blend(DiffArgs d)
         double src[C];
         if (pr.front_is_p) {
             const size_t iq = (size_t)view * HW + (size_t)qy * d.W + qx;
-#pragma unroll
-            for (int c = 0; c < C; ++c)
-                src[c] = d.color[iq * C + c];
+            pixel_color<C>(d, view, iq, qx, qy, src);
         } else {
             Bary B;
             H3 drdt;
@@ -1069,10 +1103,12 @@ __device__ inline void backward_pixel(const DiffArgs &d, const VertexAcc &A, con
     double gp[C], gc[C], cp[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-        gp[c] = d.gimage[i * C + c];
+        gp[c] = grad_image(d, i, C, c);
         gc[c] = PART == BWD_EDGES ? 0.0 : gp[c];
-        cp[c] = PART == BWD_INTERIOR ? 0.0 : d.color[i * C + c];
+        cp[c] = 0.0;
     }
+    if (PART != BWD_INTERIOR)
+        pixel_color<C>(d, view, i, px, py, cp);
     bool touched = PART != BWD_EDGES;
 
     // antialiasing: colour gradients (gather) and silhouette crossings (scatter)
@@ -1092,7 +1128,7 @@ __device__ inline void backward_pixel(const DiffArgs &d, const VertexAcc &A, con
                 const double w = pv.a * (0.5 - pv.t);
 #pragma unroll
                 for (int c = 0; c < C; ++c)
-                    gc[c] += w * d.gimage[iq * C + c];
+                    gc[c] += w * grad_image(d, iq, C, c);
                 touched = true;
             }
             continue;
@@ -1104,9 +1140,7 @@ __device__ inline void backward_pixel(const DiffArgs &d, const VertexAcc &A, con
         double dedt = 0.0;  // d(w gp . E)/dt through the crossing point
         Bary B;
         if (t_low) {
-#pragma unroll
-            for (int c = 0; c < C; ++c)
-                src[c] = d.color[iq * C + c];
+            pixel_color<C>(d, view, iq, qx, qy, src);
         } else {
             H3 drdt;
             if (!bary_at_ray(d, view, pr.tri, crossing_ray(d, pr, pv.t, px, py, qx, qy, drdt), B))
@@ -1222,7 +1256,7 @@ __global__ void __launch_bounds__(256) k_diff_bwd_interior(DiffArgs d)
             double gc[C];
 #pragma unroll
             for (int c = 0; c < C; ++c)
-                gc[c] = d.gimage[pi.i * C + c];
+                gc[c] = grad_image(d, pi.i, C, c);
             bary_grads<C>(d, B, gc, g);
             tri = t;
 #pragma unroll
@@ -1377,11 +1411,12 @@ static int diff_run(swr_context *c, const swr_diff_desc *dd, const float *gimage
     const size_t px = (size_t)q.views * q.width * q.height;
     if (px == 0)
         return SWR_OK;
-    const void *need[] = {q.clip, q.triangles, q.adjacency, q.attributes, q.ids, q.depth, q.bary,
-                          q.color};
+    const void *need[] = {q.clip, q.triangles, q.adjacency, q.attributes, q.ids, q.depth, q.bary};
     for (const void *p : need)
         if (!p || !context_is_device_ptr(p))
             return context_fail(c, SWR_ERR_INVALID_ARG, "swr_diff buffers must be device memory");
+    if (q.color && !context_is_device_ptr(q.color))
+        return context_fail(c, SWR_ERR_INVALID_ARG, "swr_diff buffers must be device memory");
     if (which == 0 && (!q.image || !context_is_device_ptr(q.image)))
         return context_fail(c, SWR_ERR_INVALID_ARG, "swr_diff_forward needs image");
     if (which == 1 && (!gimage || !gclip || !gattr || !context_is_device_ptr(gimage) ||
@@ -1410,6 +1445,9 @@ static int diff_run(swr_context *c, const swr_diff_desc *dd, const float *gimage
     d.color = q.color;
     d.image = q.image;
     d.gimage = gimage;
+    d.l2_target = (q.flags & SWR_DIFF_L2_TARGET) ? 1 : 0;
+    if (which == 1 && d.l2_target && (!q.image || !context_is_device_ptr(q.image)))
+        return context_fail(c, SWR_ERR_INVALID_ARG, "SWR_DIFF_L2_TARGET needs the forward's image");
     d.gclip = gclip;
     d.gattr = nullptr;
     d.gattr_out = gattr;
@@ -1489,7 +1527,7 @@ static int diff_run(swr_context *c, const swr_diff_desc *dd, const float *gimage
                              (uint64_t)(uintptr_t)q.adjacency, (uint64_t)(uintptr_t)q.ids,
                              (uint64_t)(uintptr_t)q.depth,     (uint64_t)(uintptr_t)pbuf,
                              ((uint64_t)q.views << 32) | (uint32_t)q.width,
-                             ((uint64_t)q.height << 40) | ((uint64_t)(q.flags & 0xFF) << 32) |
+                             ((uint64_t)q.height << 40) | ((uint64_t)(q.flags & SWR_DIFF_CULLED) << 32) |
                                  (uint32_t)q.triangle_count,
                              q.scenes > 1 ? table_gen : 0};
     uint64_t *held = context_diff_tag(c);

@@ -246,8 +246,10 @@ __device__ inline bool prim_geometry(const GeomArgs &a, const WinV &v0, const Wi
 // pixel get the sentinel key and are dropped before the sort:
 //   Block, 8x8 bins: the reference's corner classification (0 = skip), then
 //     block_may_cover; the class rides in the pair value;
-//   Block, larger bins: dropped when no pixel of the bin can pass all three edge tests
-//     (block_may_cover over the bin); the raster kernel classifies each 8x8 block;
+//   16x16 bins with cls4: the class of each of the bin's four blocks (in the block grid)
+//     rides in the pair value, Span primitives a bit per block with pixels;
+//   Block, other larger bins: dropped when no pixel of the bin can pass all three edge
+//     tests (block_may_cover over the bin); the raster kernel classifies each 8x8 block;
 //   Span: exact, the bin's scanline ranges are those the rasterizer computes.
 __device__ inline void pair_for_tile(const GeomArgs &a, const float *rec, uint32_t flags,
                                      uint32_t pos, int tx, int ty, uint32_t &key, uint32_t &val)
@@ -272,9 +274,34 @@ __device__ inline void pair_for_tile(const GeomArgs &a, const float *rec, uint32
                 cls = 0;
             key = cls ? key : a.sentinel;
             val |= (uint32_t)cls << 30;
+        } else if (a.cls4) {
+            const uint32_t *ru = reinterpret_cast<const uint32_t *>(rec);
+            uint32_t packed = 0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int kx = (bx >> 3) + (b & 1), ky = (by >> 3) + (b >> 1);
+                int cls = 0;
+                if (kx >= (int)ru[REC_BLOCKS] && kx <= (int)ru[REC_BLOCKS + 2] &&
+                    ky >= (int)ru[REC_BLOCKS + 1] && ky <= (int)ru[REC_BLOCKS + 3]) {
+                    cls = classify_block(rec + REC_EDGE, flags, kx << 3, ky << 3);
+                    if (cls == 1 && !block_may_cover(rec + REC_EDGE, kx << 3, ky << 3))
+                        cls = 0;
+                }
+                packed |= (uint32_t)cls << (2 * b);
+            }
+            key = packed ? key : a.sentinel;
+            val |= packed << PAIR_CLS4_SHIFT;
         } else {
             key = block_may_cover(rec + REC_EDGE, bx, by, size - 1) ? key : a.sentinel;
         }
+    } else if (a.cls4) {
+        uint32_t packed = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            packed |= span_block_any(rec, flags, bx + (b & 1) * TILE, by + (b >> 1) * TILE,
+                                     sc_minx, sc_maxx) ? 1u << (2 * b) : 0u;
+        key = packed ? key : a.sentinel;
+        val |= packed << PAIR_CLS4_SHIFT;
     } else {
         bool any = false;
         for (int r = 0; r < size && !any; ++r) {
@@ -311,15 +338,17 @@ __device__ __forceinline__ void write_prim(const GeomArgs &a, uint64_t pos, uint
     ru[REC_TILES + 1] = (uint32_t)g.ty0;
     ru[REC_TILES + 2] = (uint32_t)g.tx1;
     ru[REC_TILES + 3] = (uint32_t)g.ty1;
-    rec[REC_EDGE + 0] = g.e0.a;
-    rec[REC_EDGE + 1] = g.e0.b;
-    rec[REC_EDGE + 2] = g.e0.c;
-    rec[REC_EDGE + 3] = g.e1.a;
-    rec[REC_EDGE + 4] = g.e1.b;
-    rec[REC_EDGE + 5] = g.e1.c;
-    rec[REC_EDGE + 6] = g.e2.a;
-    rec[REC_EDGE + 7] = g.e2.b;
-    rec[REC_EDGE + 8] = g.e2.c;
+    if (g.flags & FLAG_BLOCK) {  // Span coverage comes from the flat parts alone
+        rec[REC_EDGE + 0] = g.e0.a;
+        rec[REC_EDGE + 1] = g.e0.b;
+        rec[REC_EDGE + 2] = g.e0.c;
+        rec[REC_EDGE + 3] = g.e1.a;
+        rec[REC_EDGE + 4] = g.e1.b;
+        rec[REC_EDGE + 5] = g.e1.c;
+        rec[REC_EDGE + 6] = g.e2.a;
+        rec[REC_EDGE + 7] = g.e2.b;
+        rec[REC_EDGE + 8] = g.e2.c;
+    }
     if (g.flags & FLAG_BLOCK) {
         ru[REC_BLOCKS + 0] = (uint32_t)g.kx0;
         ru[REC_BLOCKS + 1] = (uint32_t)g.ky0;
@@ -961,7 +990,7 @@ __device__ __forceinline__ int clip_two_planes(const GeomArgs &a, int cm, const 
 // itself emits a primitive for t (a triangle the clip queue takes is not live).  Their scan
 // gives the appended fans' draw-order slots and, with the live flags, the compacted list.
 template <int NV, bool EMIT, bool CLIPQ = false, bool LISTED = false>
-__device__ void process_triangle(const GeomArgs &a0, uint32_t t)
+__device__ void process_triangle(const GeomArgs &a0, uint32_t t, uint32_t usid = 0)
 {
     GeomArgs a = a0;
     FanCtx c;
@@ -978,8 +1007,10 @@ __device__ void process_triangle(const GeomArgs &a0, uint32_t t)
     uint32_t ntris = a.tris, nbatches = a.batches;  // extent of this triangle's draw
     bool live = true;
     if (a0.nscenes) {
-        uint32_t sid = a0.batch_scene[c.b];
-        if (!CLIPQ && !LISTED)  // a 256-thread block lies inside one 1024-triangle batch
+        // a 256-thread block of the main pass lies inside one 1024-triangle batch; the listed
+        // emit hands in the wave-uniform scene of its waterfall step
+        uint32_t sid = LISTED ? usid : a0.batch_scene[c.b];
+        if (!CLIPQ && !LISTED)
             sid = __builtin_amdgcn_readfirstlane(sid);
         const SceneDev &S = a0.scenes[sid];
         apply_scene(a, S);
@@ -1200,14 +1231,31 @@ __global__ void __launch_bounds__(256) k_geom_emit(GeomArgs a)
 }
 
 // The emit pass over the live triangles only, in draw order (the list the scan compacted).
-template <int NV>
+// In a batch of scenes a wave's triangles may come from several scenes: the wave takes them
+// one scene at a time (a waterfall over the scene of its first pending lane), so the scene's
+// state stays wave-uniform (scalar loads and registers).
+template <int NV, bool SCENES>
 __global__ void __launch_bounds__(256) k_geom_emit_listed(GeomArgs a)
 {
     if (a.status[ST_OVERFLOW] | a.status[ST_ERROR])
         return;
     const uint32_t n = (uint32_t)a.status[ST_LIVE];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        process_triangle<NV, true, false, true>(a, a.live[i]);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t t = a.live[i];
+        if (!SCENES) {
+            process_triangle<NV, true, false, true>(a, t);
+            continue;
+        }
+        const uint32_t sid = a.batch_scene[t >> 10];
+        bool pending = true;
+        while (pending) {
+            const uint32_t s0 = __builtin_amdgcn_readfirstlane(sid);
+            if (sid == s0) {
+                process_triangle<NV, true, false, true>(a, t, s0);
+                pending = false;
+            }
+        }
+    }
 }
 
 // The queued triangles that straddle a clip plane, densely packed (order does not matter:
@@ -1331,7 +1379,10 @@ static void emit_kernels(const GeomArgs &a, hipStream_t s, hipStream_t side, boo
         hipLaunchKernelGGL(k_clip_emit<NV == 0 ? 1 : NV>, gq, bq, 0, side ? side : s, a);
     if (a.live) {  // at most 2048 workgroups walk the live list
         const dim3 gl(g.x < 2048 ? g.x : 2048);
-        hipLaunchKernelGGL(k_geom_emit_listed<NV>, gl, blk, 0, s, a);
+        if (a.nscenes)
+            hipLaunchKernelGGL((k_geom_emit_listed<NV, true>), gl, blk, 0, s, a);
+        else
+            hipLaunchKernelGGL((k_geom_emit_listed<NV, false>), gl, blk, 0, s, a);
     } else {
         hipLaunchKernelGGL(k_geom_emit<NV>, g, blk, 0, s, a);
     }

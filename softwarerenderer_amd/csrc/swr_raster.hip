@@ -63,30 +63,18 @@ struct chained_planes<P, N, decltype((void)P::CHAINED)> {
     static constexpr int value = P::CHAINED < N ? P::CHAINED : N;
 };
 
-// Span: true when some scanline of the 8x8 block at (bx, by) has pixels of the primitive
-// inside the block's columns (the ranges the rasterizer computes, Rasterizer.h:370-380,
-// 397-407).
-__device__ inline bool span_block_any(const float *rec, uint32_t flags, int bx, int by,
-                                      int sc_minx, int sc_maxx)
-{
-    bool any = false;
-    for (int r = 0; r < TILE && !any; ++r) {
-        int xl, xr;
-        const bool rowv = span_row_extent(rec, flags, by + r, sc_minx, sc_maxx, xl, xr);
-        any = rowv && imax(xl, bx) < imin(xr, bx + TILE);
-    }
-    return any;
-}
-
 // The raster kernel.  A workgroup takes either raster_waves bins of 8x8 pixels (one wave per
 // bin: the bin is one block and its list carries each primitive's block class), or one bin of
 // 16x16 or 32x32 pixels whose 4 or 16 blocks its waves share (wave w takes blocks w, w + W,
 // ...): each wave reads the bin's list and keeps, in a per-wave ring in LDS, the primitives
 // its block classifies as drawn (Rasterizer.h:270-304 with block_may_cover, or the Span
 // rows), in list order.
-template <template <class> class ProgT, int NZ, int NW, int NA, int NP, int GSX>
+// MODE 0: 8x8 bins; 1: 16x16 bins whose pair values carry the block classes; 2: larger bins
+// whose blocks the kernel classifies.
+template <template <class> class ProgT, int NZ, int NW, int NA, int NP, int GSX, int MODE>
 __global__ void __launch_bounds__(512) k_raster(RasterArgs a0)
 {
+    constexpr bool FILT = MODE != 0;
     using L = Layout<NZ, NW, NA, NP>;
     using Prog = ProgT<L>;
     constexpr int N = L::N;
@@ -97,15 +85,17 @@ __global__ void __launch_bounds__(512) k_raster(RasterArgs a0)
     static_assert(GS >= 1 && GS <= 8, "a group is at most 8 primitives (one per row slot)");
     constexpr int MAXW = 8;  // waves per workgroup: blockDim.x / 64, at most 8
     constexpr uint32_t RING = 128;  // per-wave ring of a larger bin's drawn primitives
+    constexpr int RW = FILT ? MAXW : 1;
     __shared__ float s_rows[MAXW][GS * 64];
     __shared__ unsigned long long s_cov[MAXW][GS];
-    __shared__ uint32_t s_ring[MAXW][RING];
+    __shared__ uint32_t s_ring[RW][RING];
 
     if (a0.fold && blockIdx.x == 0 && threadIdx.x == 0)
         fold_status(const_cast<unsigned long long *>(a0.status));
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wpb = (int)(blockDim.x >> 6);
-    const int S = 1 << (a0.bin_shift - 3);  // 8x8 blocks per bin side
+    // 8x8 blocks per bin side: 1 for 8x8 bins (FILT = false), else 2 or 4
+    const int S = FILT ? 1 << (a0.bin_shift - 3) : 1;
     const unsigned blk = a0.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const int tile = __builtin_amdgcn_readfirstlane(S == 1 ? (int)blk * wpb + wv : (int)blk);
     if (tile >= a0.tiles)
@@ -137,7 +127,7 @@ __global__ void __launch_bounds__(512) k_raster(RasterArgs a0)
     const int slot = lane >> 3, row = lane & 7;
     float *rows = s_rows[wv];
     unsigned char *covb = reinterpret_cast<unsigned char *>(s_cov[wv]);
-    uint32_t *ring = s_ring[wv];
+    uint32_t *ring = s_ring[FILT ? wv : 0];
     const unsigned long long below = (1ull << lane) - 1ull;
 
     for (int bi = S == 1 ? 0 : wv; bi < S * S; bi += S == 1 ? 1 : wpb) {
@@ -161,7 +151,7 @@ __global__ void __launch_bounds__(512) k_raster(RasterArgs a0)
         uint32_t head = 0, tail = 0;   // ring of the block's drawn primitives (S > 1)
         for (;;) {
             int cnt;
-            if (S == 1) {
+            if (!FILT) {
                 if (pos >= end)
                     break;
                 cnt = (int)((end - pos) < (uint32_t)GS ? (end - pos) : (uint32_t)GS);
@@ -170,7 +160,12 @@ __global__ void __launch_bounds__(512) k_raster(RasterArgs a0)
                     const uint32_t i = pos + (uint32_t)lane;
                     uint32_t pv = 0;
                     bool keep = false;
-                    if (i < end) {
+                    if (i < end && MODE == 1) {
+                        pv = a.vals[i];
+                        const uint32_t cls = (pv >> (PAIR_CLS4_SHIFT + 2 * bi)) & 3u;
+                        keep = cls != 0;
+                        pv = (pv & PAIR_CLS4_PRIM_MASK) | cls << 30;
+                    } else if (i < end) {
                         pv = a.vals[i] & PAIR_PRIM_MASK;
                         const float *rec = a.prims + (size_t)pv * (size_t)a.stride;
                         const uint32_t flags = __float_as_uint(rec[REC_FLAGS]);
@@ -207,8 +202,8 @@ __global__ void __launch_bounds__(512) k_raster(RasterArgs a0)
             const bool act = slot < cnt;
             uint32_t pv = 0u;
             if (act)
-                pv = S == 1 ? a.vals[pos + slot] : ring[(head + (uint32_t)slot) & (RING - 1)];
-            if (S == 1)
+                pv = !FILT ? a.vals[pos + slot] : ring[(head + (uint32_t)slot) & (RING - 1)];
+            if (!FILT)
                 pos += (uint32_t)cnt;
             else
                 head += (uint32_t)cnt;
@@ -340,8 +335,15 @@ static hipError_t launch(const RasterArgs &a, hipStream_t s, int waves)
     }
     if (!blocks)
         return hipSuccess;
-    hipLaunchKernelGGL((k_raster<ProgT, NZ, NW, NA, NP, GSX>), dim3(blocks), dim3(64 * w), 0, s,
-                       a);
+    if (a.bin_shift > 3 && a.cls4)
+        hipLaunchKernelGGL((k_raster<ProgT, NZ, NW, NA, NP, GSX, 1>), dim3(blocks), dim3(64 * w),
+                           0, s, a);
+    else if (a.bin_shift > 3)
+        hipLaunchKernelGGL((k_raster<ProgT, NZ, NW, NA, NP, GSX, 2>), dim3(blocks), dim3(64 * w),
+                           0, s, a);
+    else
+        hipLaunchKernelGGL((k_raster<ProgT, NZ, NW, NA, NP, GSX, 0>), dim3(blocks),
+                           dim3(64 * w), 0, s, a);
     return hipGetLastError();
 }
 

@@ -257,8 +257,9 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_down2(
 }
 
 // Scans of up to this many blocks skip the pass over the block sums: each block of the
-// down pass adds up the sums before it (at most SCAN_CARRY_BLOCKS / SCAN_THREADS loads).
-constexpr uint32_t SCAN_CARRY_BLOCKS = 1024;
+// down pass adds up the sums before it (at most SCAN_CARRY_BLOCKS / SCAN_THREADS loads per
+// thread, one round of independent L2 loads).
+constexpr uint32_t SCAN_CARRY_BLOCKS = 8192;
 
 template <typename T>
 hipError_t scan_impl(const T *in, T *out, uint64_t n, T *tmp, unsigned long long *status,

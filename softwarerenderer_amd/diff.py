@@ -91,8 +91,7 @@ class DiffRenderer:
         self.ids = torch.zeros((V, height, width), dtype=torch.int32, device=dev)
         self.depth = torch.ones((V, height, width), dtype=torch.float32, device=dev)
         self.bary = torch.zeros((V, height, width, 2), dtype=torch.float32, device=dev)
-        self.color = torch.zeros((V, height, width, channels), dtype=torch.float32, device=dev)
-        self.image = torch.zeros_like(self.color)
+        self.image = torch.zeros((V, height, width, channels), dtype=torch.float32, device=dev)
         self.flags = 1 if cull_mode != api.CullMode.None_ else 0  # SWR_DIFF_CULLED
         self.r = api.Rasterizer(device)
         self.r.setPixelShader(api.VisibilityShader())
@@ -132,8 +131,9 @@ class DiffRenderer:
                                             self.adj.data_ptr())
         d.attributes = attr.data_ptr()
         d.ids, d.depth = self.ids.data_ptr(), self.depth.data_ptr()
-        d.bary, d.color, d.image = (self.bary.data_ptr(), self.color.data_ptr(),
-                                    self.image.data_ptr())
+        # no colour buffer: the kernels recompute the few colours before antialiasing they
+        # need instead of the forward writing one per pixel
+        d.bary, d.image = self.bary.data_ptr(), self.image.data_ptr()
         if self.S > 1:
             d.scenes = self.S
             d.scene_views, d.scene_vertices, d.scene_triangles = (
@@ -167,11 +167,22 @@ class DiffRenderer:
     def backward(self, grad_image: torch.Tensor):
         """(grad_clip shaped like the forward's clip, grad_attr [N, C]) of the last
         forward."""
+        return self._backward(grad_image, l2=False)
+
+    def backward_l2(self, target: torch.Tensor):
+        """The gradients of the loss 1/2 |image - target|^2 for the last forward: the same
+        as backward(image - target), with the difference formed inside the backward
+        kernels (SWR_DIFF_L2_TARGET) instead of as a separate image-sized tensor."""
+        return self._backward(target, l2=True)
+
+    def _backward(self, grad_image: torch.Tensor, l2: bool):
         self._use_current_stream()
         g = grad_image.detach().contiguous().float()
         gclip = torch.zeros(self._clip_shape, dtype=torch.float32, device=self.device)
         gattr = torch.zeros((self.N, self.C), dtype=torch.float32, device=self.device)
         d = self._desc(self._attr_keep)
+        if l2:
+            d.flags |= 2  # SWR_DIFF_L2_TARGET
         self.r._check(self.r._lib.swr_diff_backward(self.r._ctx, C.byref(d), g.data_ptr(),
                                                     gclip.data_ptr(), gattr.data_ptr()),
                       "swr_diff_backward")

@@ -288,6 +288,26 @@ def test_autograd_full_loss_direction(gpu):
     r.close()
 
 
+def test_fused_l2_backward_matches_explicit_gradient(gpu):
+    """backward_l2(target) (SWR_DIFF_L2_TARGET: dL/dimage = image - target formed where the
+    kernels read it) equals backward(image - target), up to the order of float atomics."""
+    import torch
+    size, views = 96, 3
+    V, F, clip, attr = make_case(views=views, size=size, subdiv=3, seed=31)
+    r = gpu_renderer(F, V.shape[0], views, size)
+    clip_t, attr_t = torch.from_numpy(clip).cuda(), torch.from_numpy(attr).cuda()
+    target = torch.from_numpy(np.random.default_rng(2).random((views, size, size, 3))).float().cuda()
+    img = r.forward(clip_t, attr_t, sync=True)
+    gc1, ga1 = r.backward(img - target)
+    r.forward(clip_t, attr_t)
+    gc2, ga2 = r.backward_l2(target)
+    r.check()
+    assert torch.allclose(gc2, gc1, rtol=1e-5, atol=1e-6 * gc1.abs().max().item())
+    assert torch.allclose(ga2, ga1, rtol=1e-5, atol=1e-6 * ga1.abs().max().item())
+    assert gc1.abs().max().item() > 0
+    r.close()
+
+
 def test_graph_replay_matches_eager(gpu):
     """A forward + backward step captured in a HIP graph replays to the eager results."""
     import torch
