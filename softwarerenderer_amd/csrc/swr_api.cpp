@@ -42,8 +42,6 @@ struct swr_context {
     int device = 0;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
-    hipStream_t side = nullptr;               // the clip queue's emit (fork / join)
-    hipEvent_t fork = nullptr, join = nullptr;
     std::string err;
     int err_code = SWR_OK;                    // code of the last failure (with err)
 
@@ -799,7 +797,7 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
             HIP_TRY(c, scan_pairs(g.grp, A.grp_ex, ngrp, (uint2 *)A.scan_tmp, g.status,
                                   A.prim_cap, A.pair_cap, c->stream));
         record(c, 2);
-        HIP_TRY(c, launch_geom_emit(g, c->stream, c->side, c->fork, c->join));
+        HIP_TRY(c, launch_geom_emit(g, c->stream));
         HIP_TRY(c, launch_expand(g, c->stream, c->tune.expand_chunk));
         record(c, 3);
         SortBuffers sb;
@@ -1205,13 +1203,6 @@ int swr_create(swr_context **out, int device)
     }
     for (int i = 0; i <= S_COUNT; ++i)
         (void)hipEventCreate(&c->ev[i]);
-    // the side stream is optional: without it the clip queue's emit stays on the draw stream
-    if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess) {
-        (void)hipGetLastError();
-        c->side = nullptr;
-    }
     *out = c;
     return SWR_OK;
 }
@@ -1237,14 +1228,6 @@ int swr_destroy(swr_context *c)
             (void)hipEventDestroy(c->ev[i]);
     if (c->status_host)
         (void)hipHostFree(c->status_host);
-    if (c->side) {
-        (void)hipStreamSynchronize(c->side);
-        (void)hipStreamDestroy(c->side);
-    }
-    if (c->fork)
-        (void)hipEventDestroy(c->fork);
-    if (c->join)
-        (void)hipEventDestroy(c->join);
     if (c->own)
         (void)hipStreamDestroy(c->own);
     delete c;

@@ -26,8 +26,6 @@
 
 namespace swr {
 
-constexpr int MAX_NODES = 32;
-
 // ---- primitive setup ------------------------------------------------------------------
 struct Edge {
     float a, b, c;
@@ -514,32 +512,70 @@ struct FanCtx {
     uint32_t app_n;                 // count pass: fan triangles appended (k >= 1)
 };
 
-// One fan triangle (p0, p[k+1], p[k+2]): transform, cull, setup, count or emit.
+// Facing cull and setup of one fan triangle (VertexProcessor::cullTriangles,
+// VertexProcessor.cpp:330-343).  A triangle the cull mode keeps with facing >= 0 has its
+// vertices 0 and 2 swapped (VertexProcessor.cpp:342): w0 / w2 come back swapped and `swap`
+// tells the caller to swap the varyings too.  Returns false when it draws nothing.
+__device__ __forceinline__ bool fan_geometry(const GeomArgs &a, WinV &w0, const WinV &w1,
+                                             WinV &w2, bool cull, PrimGeo &g, bool &swap)
+{
+    swap = false;
+    if (cull) {
+        float facing = (w0.x - w1.x) * (w2.y - w1.y) - (w2.x - w1.x) * (w0.y - w1.y);
+        if (facing < 0) {
+            if (a.cull_mode == SWR_CULL_CW)
+                return false;
+        } else {
+            if (a.cull_mode == SWR_CULL_CCW)
+                return false;
+            swap = true;
+        }
+    }
+    // component by component: a select between the two structs would address them in memory
+    const WinV s0 = w0, s2 = w2;
+    w0.x = swap ? s2.x : s0.x;
+    w0.y = swap ? s2.y : s0.y;
+    w0.z = swap ? s2.z : s0.z;
+    w0.w = swap ? s2.w : s0.w;
+    w2.x = swap ? s0.x : s2.x;
+    w2.y = swap ? s0.y : s2.y;
+    w2.z = swap ? s0.z : s2.z;
+    w2.w = swap ? s0.w : s2.w;
+    return prim_geometry(a, w0, w1, w2, g);
+}
+
+// The emit pass's write of one fan triangle at record `pos` with its pairs from `pb`.
+template <int NV>
+__device__ __forceinline__ void fan_write(const GeomArgs &a, uint64_t pos, uint64_t pb,
+                                          uint32_t nt, uint32_t key, uint32_t src, uint32_t sid,
+                                          const PrimGeo &g, const WinV &w0, const WinV &w1,
+                                          const WinV &w2, const float *v0, const float *v1,
+                                          const float *v2, bool swap)
+{
+    if (pos >= a.prim_cap || pb + nt > a.pair_cap) {  // totals are checked; this guards wrap
+        atomicOr(&a.status[ST_OVERFLOW], 4ull);
+        return;
+    }
+    // swap by value: selecting between the two arrays by pointer would force them to memory
+    constexpr int NVS = NV > 0 ? NV : 1;
+    float x0[NVS], x2[NVS];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        x0[q] = swap ? v2[q] : v0[q];
+        x2[q] = swap ? v0[q] : v2[q];
+    }
+    write_prim<NV>(a, pos, key, src, sid, g, w0, w1, w2, x0, v1, x2, pb);
+}
+
+// One fan triangle (p0, p[k+1], p[k+2]): cull, setup, count or emit.
 template <int NV, bool EMIT>
 __device__ __forceinline__ void fan_triangle(const GeomArgs &a, FanCtx &c, int k, WinV w0, WinV w1,
                                     WinV w2, const float *v0, const float *v1, const float *v2,
                                     bool cull)
 {
-    bool swap = false;
-    if (cull) {
-        // VertexProcessor::cullTriangles, VertexProcessor.cpp:330-343
-        float facing = (w0.x - w1.x) * (w2.y - w1.y) - (w2.x - w1.x) * (w0.y - w1.y);
-        if (facing < 0) {
-            if (a.cull_mode == SWR_CULL_CW)
-                return;
-        } else {
-            if (a.cull_mode == SWR_CULL_CCW)
-                return;
-            swap = true;  // indices i and i+2 swap (VertexProcessor.cpp:342)
-        }
-    }
-    {
-        const WinV s0 = w0, s2 = w2;
-        w0 = swap ? s2 : s0;
-        w2 = swap ? s0 : s2;
-    }
+    bool swap;
     PrimGeo g;
-    if (!prim_geometry(a, w0, w1, w2, g))
+    if (!fan_geometry(a, w0, w1, w2, cull, g, swap))
         return;
     uint32_t nt = (uint32_t)rect_tiles(g);
     if (!EMIT) {
@@ -564,135 +600,7 @@ __device__ __forceinline__ void fan_triangle(const GeomArgs &a, FanCtx &c, int k
         c.cnt1 += 1;
         c.cntp1 += nt;
     }
-    if (pos >= a.prim_cap || pb + nt > a.pair_cap) {  // totals are checked; this guards wrap
-        atomicOr(&a.status[ST_OVERFLOW], 4ull);
-        return;
-    }
-    // swap by value: selecting between the two arrays by pointer would force them to memory
-    constexpr int NVS = NV > 0 ? NV : 1;
-    float x0[NVS], x2[NVS];
-#pragma unroll
-    for (int q = 0; q < NV; ++q) {
-        x0[q] = swap ? v2[q] : v0[q];
-        x2[q] = swap ? v0[q] : v2[q];
-    }
-    write_prim<NV>(a, pos, key, c.src, c.sid, g, w0, w1, w2, x0, v1, x2, pb);
-}
-
-// Threads per workgroup of the clip-queue kernels.
-constexpr int CLIPQ_THREADS = 64;
-
-// Polygon nodes of the general clipper.  The walk indexes them with run-time values, which
-// per-thread arrays turn into scratch memory, a dependent global round trip per access, and
-// the few queued triangles that take this path are the queue kernels' long pole.  With few
-// varyings they live in LDS instead, one column per thread (node-major, so the lanes of a
-// wave touch consecutive words); wide varyings keep per-thread arrays.
-template <int NV, bool EMIT, bool LDS = (NV <= 8)>
-struct ClipNodes {
-    Pos4 P[MAX_NODES];
-    float V[EMIT ? MAX_NODES : 1][NV];
-    uint8_t poly[2][MAX_CLIP];
-    __device__ Pos4 &p(int i) { return P[i]; }
-    __device__ float &v(int i, int k) { return V[EMIT ? i : 0][k]; }
-    __device__ uint8_t &g(int b, int i) { return poly[b][i]; }
-};
-
-template <int NV, bool EMIT>
-struct ClipNodes<NV, EMIT, true> {
-    __device__ static Pos4 (&sp())[MAX_NODES][CLIPQ_THREADS]
-    {
-        __shared__ Pos4 s[MAX_NODES][CLIPQ_THREADS];
-        return s;
-    }
-    __device__ static float (&sv())[EMIT ? MAX_NODES : 1][NV > 0 ? NV : 1][CLIPQ_THREADS]
-    {
-        __shared__ float s[EMIT ? MAX_NODES : 1][NV > 0 ? NV : 1][CLIPQ_THREADS];
-        return s;
-    }
-    __device__ static uint8_t (&sg())[2][MAX_CLIP][CLIPQ_THREADS]
-    {
-        __shared__ uint8_t s[2][MAX_CLIP][CLIPQ_THREADS];
-        return s;
-    }
-    __device__ Pos4 &p(int i) { return sp()[i][threadIdx.x]; }
-    __device__ float &v(int i, int k) { return sv()[EMIT ? i : 0][k][threadIdx.x]; }
-    __device__ uint8_t &g(int b, int i) { return sg()[b][i][threadIdx.x]; }
-};
-
-// Clipped triangles that neither early-out nor fit the register paths (rare path, kept
-// out of line).  Called only from the clip-queue kernels (CLIPQ_THREADS threads each).
-template <int NV, bool EMIT>
-__device__ inline void clip_path(const GeomArgs &a, FanCtx &c, int cm, const Pos4 *in3,
-                                       const float (*var3)[NV])
-{
-    ClipNodes<NV, EMIT> N;
-    int cur = 0;  // node lists: g(cur, .) is the polygon, g(cur ^ 1, .) the next one
-    for (int q = 0; q < 3; ++q) {
-        N.p(q) = in3[q];
-        if (EMIT)
-            for (int k = 0; k < NV; ++k)
-                N.v(q, k) = var3[q][k];
-        N.g(cur, q) = (uint8_t)q;
-    }
-    int n = 3, nn = 3;
-    // VertexProcessor.cpp:233-250 with PolyClipper::clipToPlane (PolyClipper.cpp:45-81)
-    for (int p = 0; p < 6; ++p) {
-        if (!(cm & (1 << p)) || n < 3)
-            continue;
-        int m = 0;
-        const int nxt = cur ^ 1;
-        int iprev = N.g(cur, 0);
-        float dprev = plane_dot(p, N.p(iprev));
-        for (int i = 1; i <= n; ++i) {
-            int ic = (i == n) ? N.g(cur, 0) : N.g(cur, i);
-            float dp = plane_dot(p, N.p(ic));
-            if (dprev >= 0) {
-                if (m >= MAX_CLIP) { m = -1; break; }
-                N.g(nxt, m++) = (uint8_t)iprev;
-            }
-            if (sgnf(dp) != sgnf(dprev)) {
-                if (m >= MAX_CLIP || nn >= MAX_NODES) { m = -1; break; }
-                float tt = dp < 0 ? dprev / (dprev - dp) : -dprev / (dp - dprev);
-                const Pos4 A = N.p(iprev), B = N.p(ic);
-                Pos4 R;
-                R.x = lerp1(A.x, B.x, tt);
-                R.y = lerp1(A.y, B.y, tt);
-                R.z = lerp1(A.z, B.z, tt);
-                R.w = lerp1(A.w, B.w, tt);
-                N.p(nn) = R;
-                if (EMIT) {
-                    // Helper::interpolateVertex lerps the vertex shader's counts only
-                    for (int k = 0; k < NV; ++k) {
-                        bool lerped = k < a.lay.na ? (k < a.vs_na) : ((k - a.lay.na) < a.vs_np);
-                        N.v(nn, k) = lerped ? lerp1(N.v(iprev, k), N.v(ic, k), tt) : 0.0f;
-                    }
-                }
-                N.g(nxt, m++) = (uint8_t)nn++;
-            }
-            iprev = ic;
-            dprev = dp;
-        }
-        if (m < 0) {
-            atomicOr(&a.status[ST_ERROR], 2ull);
-            n = 0;
-            break;
-        }
-        cur = nxt;
-        n = m;
-    }
-    if (!EMIT)
-        c.app_n = n > 3 ? (uint32_t)(n - 3) : 0u;
-    for (int k = 0; k + 2 < n; ++k) {
-        const int ia = N.g(cur, 0), ib = N.g(cur, k + 1), ic = N.g(cur, k + 2);
-        float va[NV > 0 ? NV : 1], vb[NV > 0 ? NV : 1], vc[NV > 0 ? NV : 1];
-        for (int q = 0; q < NV; ++q) {
-            va[q] = EMIT ? N.v(ia, q) : 0.0f;
-            vb[q] = EMIT ? N.v(ib, q) : 0.0f;
-            vc[q] = EMIT ? N.v(ic, q) : 0.0f;
-        }
-        fan_triangle<NV, EMIT>(a, c, k, to_window(a, N.p(ia)), to_window(a, N.p(ib)),
-                               to_window(a, N.p(ic)), va, vb, vc, true);
-    }
+    fan_write<NV>(a, pos, pb, nt, key, c.src, c.sid, g, w0, w1, w2, v0, v1, v2, swap);
 }
 
 // All three vertices are outside the first plane the clipper applies (the lowest set bit of
@@ -853,143 +761,16 @@ __device__ __forceinline__ int clip_one_plane(const GeomArgs &a, const Pos4 *P,
     return quad ? 2 : 1;
 }
 
-// One Sutherland-Hodgman pass (PolyClipper.cpp:52-78) of a convex polygon held in
-// registers (n <= NIN vertices) against plane p, in the clipper's output order: for
-// i = 0..n-1, vertex i if it is inside, then the intersection with edge (i, i+1 mod n) if the
-// sign changes there.  Output slot j takes the candidate of rank j through select chains, so
-// no register array is indexed by a runtime value and nothing spills to scratch.  Returns
-// the output count, or -1 when a vertex lies on the plane (the queue clipper handles the
-// duplicate vertices that creates).
-template <int NIN, int NV, bool EMIT>
-__device__ __forceinline__ int clip_poly_reg(const GeomArgs &a, int p, int n, const Pos4 *P,
-                                             const float (*Vv)[NV], Pos4 *Po, float (*Vo)[NV])
-{
-    constexpr int NOUT = NIN + 1;
-    float dp[NIN];
-    bool in[NIN];
-    bool onplane = false;
-#pragma unroll
-    for (int i = 0; i < NIN; ++i) {
-        dp[i] = plane_dot(p, P[i]);
-        onplane = onplane || (i < n && dp[i] == 0.0f);
-        in[i] = dp[i] > 0.0f;
-    }
-    if (onplane)
-        return -1;
-    CVert<NV> cand[2 * NIN];
-    bool emit[2 * NIN];
-#pragma unroll
-    for (int i = 0; i < NIN; ++i) {
-        const bool wrap = i + 1 >= n;
-        const int nx = i + 1 < NIN ? i + 1 : 0;
-        const Pos4 Pn = wrap ? P[0] : P[nx];
-        const float dn = wrap ? dp[0] : dp[nx];
-        const bool innx = wrap ? in[0] : in[nx];
-        cand[2 * i].p = P[i];
-#pragma unroll
-        for (int k = 0; k < NV; ++k)
-            cand[2 * i].v[k] = Vv[i][k];
-        emit[2 * i] = i < n && in[i];
-        // I(v_i, v_next): t = dcur < 0 ? dprev / (dprev - dcur) : -dprev / (dcur - dprev)
-        const float dpr = dp[i];
-        const float t = dn < 0 ? dpr / (dpr - dn) : -dpr / (dn - dpr);
-        CVert<NV> &x = cand[2 * i + 1];
-        x.p.x = lerp1(P[i].x, Pn.x, t);
-        x.p.y = lerp1(P[i].y, Pn.y, t);
-        x.p.z = lerp1(P[i].z, Pn.z, t);
-        x.p.w = lerp1(P[i].w, Pn.w, t);
-#pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            const bool lerped = k < a.lay.na ? (k < a.vs_na) : ((k - a.lay.na) < a.vs_np);
-            const float vn = wrap ? Vv[0][k] : Vv[nx][k];
-            x.v[k] = (EMIT && lerped) ? lerp1(Vv[i][k], vn, t) : 0.0f;
-        }
-        emit[2 * i + 1] = i < n && in[i] != innx;
-    }
-    int rank[2 * NIN];
-    int m = 0;
-#pragma unroll
-    for (int c = 0; c < 2 * NIN; ++c) {
-        rank[c] = m;
-        m += emit[c] ? 1 : 0;
-    }
-#pragma unroll
-    for (int j = 0; j < NOUT; ++j) {
-        Pos4 q = P[0];
-        float qv[NV > 0 ? NV : 1];
-#pragma unroll
-        for (int k = 0; k < NV; ++k)
-            qv[k] = 0.0f;
-#pragma unroll
-        for (int c = 0; c < 2 * NIN; ++c) {
-            const bool sel = emit[c] && rank[c] == j;
-            q.x = sel ? cand[c].p.x : q.x;
-            q.y = sel ? cand[c].p.y : q.y;
-            q.z = sel ? cand[c].p.z : q.z;
-            q.w = sel ? cand[c].p.w : q.w;
-#pragma unroll
-            for (int k = 0; k < NV; ++k)
-                qv[k] = sel ? cand[c].v[k] : qv[k];
-        }
-        Po[j] = q;
-#pragma unroll
-        for (int k = 0; k < NV; ++k)
-            Vo[j][k] = qv[k];
-    }
-    return m;
-}
-
-// Exactly two planes selected and finite coordinates: clip in registers, plane by plane in
-// the order of VertexProcessor.cpp:237-242 (lowest bit first).  The polygon has at most 5
-// vertices.  Writes the window-space polygon to W / V and returns the fan count (0 when
-// fully clipped), or -1 to leave the triangle to the queue clipper.
-template <int NV, bool EMIT>
-__device__ __forceinline__ int clip_two_planes(const GeomArgs &a, int cm, const Pos4 *P,
-                                               const float (*var)[NV], WinV *W, float (*V)[NV])
-{
-    const int p1 = __builtin_ctz(cm);
-    const int rest = cm & (cm - 1);
-    if (!rest || (rest & (rest - 1)))
-        return -1;
-    const int p2 = __builtin_ctz(rest);
-#pragma unroll
-    for (int q = 0; q < 3; ++q)
-        if (!(finitef(P[q].x) && finitef(P[q].y) && finitef(P[q].z) && finitef(P[q].w)))
-            return -1;
-    Pos4 P1[4];
-    float V1[4][NV > 0 ? NV : 1];
-    const int n1 = clip_poly_reg<3, NV, EMIT>(a, p1, 3, P, var, P1, V1);
-    if (n1 < 0)
-        return -1;
-    if (n1 < 3)
-        return 0;  // clipToPlane returns fewer than 3 vertices: fully clipped
-    Pos4 P2[5];
-    float V2[5][NV > 0 ? NV : 1];
-    const int n2 = clip_poly_reg<4, NV, EMIT>(a, p2, n1, P1, V1, P2, V2);
-    if (n2 < 0)
-        return -1;
-    if (n2 < 3)
-        return 0;
-#pragma unroll
-    for (int i = 0; i < 5; ++i) {
-        W[i] = to_window(a, P2[i]);
-#pragma unroll
-        for (int k = 0; k < NV; ++k)
-            V[i][k] = V2[i][k];
-    }
-    return n2 - 2;
-}
-
-// CLIPQ = false: the main pass over all input triangles; triangles that need the clipper
-// go to a queue (count pass) or are left to the queue kernel (emit pass).
-// CLIPQ = true: one queued triangle, clipped.
+// The main pass over all input triangles.  Triangles inside every plane, fully clipped or
+// crossing one plane are finished here; the others go to the clip queue (count pass) or are
+// left to the queue kernel (emit pass), which clips them a wave at a time (clip_queued).
 // LISTED: the emit pass over the compacted list of triangles the count pass found live (a
 // workgroup's triangles may then come from several batches and scenes).
 //
 // The count pass writes app[t] = (fan triangles appended, live): live when the main pass
 // itself emits a primitive for t (a triangle the clip queue takes is not live).  Their scan
 // gives the appended fans' draw-order slots and, with the live flags, the compacted list.
-template <int NV, bool EMIT, bool CLIPQ = false, bool LISTED = false>
+template <int NV, bool EMIT, bool LISTED = false>
 __device__ void process_triangle(const GeomArgs &a0, uint32_t t, uint32_t usid = 0)
 {
     GeomArgs a = a0;
@@ -1010,7 +791,7 @@ __device__ void process_triangle(const GeomArgs &a0, uint32_t t, uint32_t usid =
         // a 256-thread block of the main pass lies inside one 1024-triangle batch; the listed
         // emit hands in the wave-uniform scene of its waterfall step
         uint32_t sid = LISTED ? usid : a0.batch_scene[c.b];
-        if (!CLIPQ && !LISTED)
+        if (!LISTED)
             sid = __builtin_amdgcn_readfirstlane(sid);
         const SceneDev &S = a0.scenes[sid];
         apply_scene(a, S);
@@ -1042,9 +823,9 @@ __device__ void process_triangle(const GeomArgs &a0, uint32_t t, uint32_t usid =
     }
     constexpr int NVS = NV > 0 ? NV : 1;
     float var3[3][NVS];
-    // the polygon handed to the fan loop below: 3 to 5 window-space vertices
-    WinV W[5];
-    float V[5][NVS];
+    // the polygon handed to the fan loop below: 3 or 4 window-space vertices
+    WinV W[4];
+    float V[4][NVS];
     int nfan = 0;
     bool cull = true;
 
@@ -1090,7 +871,7 @@ __device__ void process_triangle(const GeomArgs &a0, uint32_t t, uint32_t usid =
             atomicOr(&a.status[ST_ERROR], 1ull);
         } else {
             int cmv = -1;  // clip mask from the post-transform vertices
-            if (!CLIPQ && a.vwin)
+            if (a.vwin)
                 cmv = a.vmask[ids[0]] | a.vmask[ids[1]] | a.vmask[ids[2]];
             Pos4 P[3];
             if (cmv == 0) {
@@ -1117,19 +898,7 @@ __device__ void process_triangle(const GeomArgs &a0, uint32_t t, uint32_t usid =
                 fetch_vertex<EMIT ? NV : 0>(a, ids[q], P[q], var3[q]);
             int cm = clip_mask(P[0]) | clip_mask(P[1]) | clip_mask(P[2]);
             float dp1[3];
-            if (CLIPQ) {
-                // two planes in registers (the common queued case); anything else through
-                // the general clipper
-                int r2 = -1;
-                if constexpr (NVS <= 8)
-                    r2 = clip_two_planes<NVS, EMIT>(a, cm, P, var3, W, V);
-                if (r2 >= 0) {
-                    nfan = r2;
-                    c.app_n = nfan > 1 ? (uint32_t)(nfan - 1) : 0u;
-                } else {
-                    clip_path<NVS, EMIT>(a, c, cm, P, var3);
-                }
-            } else if (cm == 0) {
+            if (cm == 0) {
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
                     W[q] = to_window(a, P[q]);
@@ -1150,8 +919,7 @@ __device__ void process_triangle(const GeomArgs &a0, uint32_t t, uint32_t usid =
                 }
                 return;
             }
-            if (!CLIPQ)
-                c.app_n = nfan == 2 ? 1u : 0u;
+            c.app_n = nfan == 2 ? 1u : 0u;
         }
     }
 fans:
@@ -1159,13 +927,13 @@ fans:
     // vertex arrays stay in registers
 #pragma unroll 1
     for (int k = 0; k < nfan; ++k) {
-        const WinV wb = k == 0 ? W[1] : (k == 1 ? W[2] : W[3]);
-        const WinV wc = k == 0 ? W[2] : (k == 1 ? W[3] : W[4]);
+        const WinV wb = k == 0 ? W[1] : W[2];
+        const WinV wc = k == 0 ? W[2] : W[3];
         float vb[NVS], vc[NVS];
 #pragma unroll
         for (int q = 0; q < NVS; ++q) {
-            vb[q] = k == 0 ? V[1][q] : (k == 1 ? V[2][q] : V[3][q]);
-            vc[q] = k == 0 ? V[2][q] : (k == 1 ? V[3][q] : V[4][q]);
+            vb[q] = k == 0 ? V[1][q] : V[2][q];
+            vc[q] = k == 0 ? V[2][q] : V[3][q];
         }
         fan_triangle<NV, EMIT>(a, c, k, W[0], wb, wc, V[0], vb, vc, cull);
     }
@@ -1173,7 +941,7 @@ fans:
         a.grp[(size_t)c.b * 2048 + c.j] = make_uint2(c.cnt0, c.cntp0);
         a.grp[(size_t)c.b * 2048 + 1024 + c.j] = make_uint2(c.cnt1, c.cntp1);
         if (!a.window_input)
-            a.app[t] = make_uint2(c.app_n, (!CLIPQ && c.cnt0 + c.cnt1 > 0) ? 1u : 0u);
+            a.app[t] = make_uint2(c.app_n, c.cnt0 + c.cnt1 > 0 ? 1u : 0u);
     }
 }
 
@@ -1243,7 +1011,7 @@ __global__ void __launch_bounds__(256) k_geom_emit_listed(GeomArgs a)
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t t = a.live[i];
         if (!SCENES) {
-            process_triangle<NV, true, false, true>(a, t);
+            process_triangle<NV, true, true>(a, t);
             continue;
         }
         const uint32_t sid = a.batch_scene[t >> 10];
@@ -1251,38 +1019,235 @@ __global__ void __launch_bounds__(256) k_geom_emit_listed(GeomArgs a)
         while (pending) {
             const uint32_t s0 = __builtin_amdgcn_readfirstlane(sid);
             if (sid == s0) {
-                process_triangle<NV, true, false, true>(a, t, s0);
+                process_triangle<NV, true, true>(a, t, s0);
                 pending = false;
             }
         }
     }
 }
 
-// The queued triangles that straddle a clip plane, densely packed (order does not matter:
-// each writes its own (batch, phase, j) counts and draw-order positions).  Each block
-// prefix-sums the region counts, then maps a dense index to (region, slot).
+// ---- the clip queue: a wave per queued triangle -------------------------------------------
+// Triangles that cross two or more planes (about 0.3% of a dense frame) are clipped by whole
+// waves, one triangle per wave at a time, lane k holding polygon vertex k.  A pass of
+// PolyClipper::clipToPlane (PolyClipper.cpp:45-81) walks the edges (v_k, v_k+1 mod n) in
+// order and outputs v_k if it is inside (dpPrev >= 0), then the edge's intersection if the
+// plane value changes sign along it.  Here every lane takes its own edge at once: vertex k's
+// output slot is the number of inside lanes below k plus the number of crossing lanes below
+// k (two ballots), its intersection goes right after it, and the new polygon is staged in
+// LDS and read back a vertex per lane.  The fan triangles (VertexProcessor.cpp:254-261) are
+// set up a fan per lane; the emit pass places fan k >= 1 after the drawn fans below it (a
+// ballot count) and its pairs after theirs (a wave prefix sum), as a walk in fan order
+// would.  A polygon that outgrows MAX_CLIP vertices fails the draw, as the oracle's clipper
+// does.
+constexpr int CLIPQ_THREADS = 64;  // one wave per workgroup
+static_assert(MAX_CLIP <= CLIPQ_THREADS, "a polygon vertex per lane");
+
+template <int NV>
+struct ClipStage {
+    Pos4 p[MAX_CLIP];
+    float v[MAX_CLIP][NV];
+};
+
+// wave-wide inclusive prefix sum
+__device__ inline uint32_t wave_incl_sum(uint32_t x, uint32_t lane)
+{
+#pragma unroll
+    for (int sh = 1; sh < 64; sh <<= 1) {
+        const uint32_t o = __shfl_up(x, sh, 64);
+        x += lane >= (uint32_t)sh ? o : 0u;
+    }
+    return x;
+}
+
+__device__ inline WinV shfl_win(const WinV &w, int src)
+{
+    WinV r;
+    r.x = __shfl(w.x, src);
+    r.y = __shfl(w.y, src);
+    r.z = __shfl(w.z, src);
+    r.w = __shfl(w.w, src);
+    return r;
+}
+
+// One queued triangle t, clipped by the calling wave (all 64 lanes, t uniform).  The count
+// pass writes its (batch, phase, j) counts and appended-fan count; the emit pass its records.
+template <int NV, bool EMIT>
+__device__ void clip_queued(const GeomArgs &a0, uint32_t t, ClipStage<NV> &st)
+{
+    const uint32_t lane = threadIdx.x;
+    t = __builtin_amdgcn_readfirstlane(t);
+    GeomArgs a = a0;
+    const uint32_t b = t >> 10, j = t & 1023u;
+    uint32_t src = t, kb = b, sid = 0, ntris = a.tris, nbatches = a.batches;
+    if (a0.nscenes) {
+        sid = a0.batch_scene[b];
+        const SceneDev &S = a0.scenes[sid];
+        apply_scene(a, S);
+        kb = b - S.batch0;
+        src = t - S.batch0 * (uint32_t)BATCH;
+        ntris = S.tris;
+        nbatches = (S.tris + BATCH - 1) / BATCH;
+    }
+    // the draw-order slots the scans gave this triangle (it is live: the count pass checked
+    // its indices before queueing it)
+    uint32_t slot1 = 0;
+    uint64_t pos0 = 0, pairs0 = 0, pos1 = 0, pairs1 = 0;
+    if (EMIT) {
+        const uint32_t nbt = (kb + 1 < nbatches) ? (uint32_t)BATCH : ntris - kb * (uint32_t)BATCH;
+        slot1 = nbt + (a.app_ex[t].x - a.app_ex[b * BATCH].x);
+        const uint2 g0 = a.grp_ex[(size_t)b * 2048 + j];
+        const uint2 g1 = a.grp_ex[(size_t)b * 2048 + 1024 + j];
+        pos0 = g0.x;
+        pairs0 = g0.y;
+        pos1 = g1.x;
+        pairs1 = g1.y;
+    }
+    // lanes 0-2: the input vertices
+    Pos4 P = {0.0f, 0.0f, 0.0f, 0.0f};
+    float var[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+        var[k] = 0.0f;
+    int cml = 0;
+    if (lane < 3) {
+        fetch_vertex<EMIT ? NV : 0>(a, a.indices[3ull * src + lane], P, var);
+        cml = clip_mask(P);
+    }
+    const int cm = __builtin_amdgcn_readfirstlane(__shfl(cml, 0) | __shfl(cml, 1) | __shfl(cml, 2));
+    // VertexProcessor.cpp:233-250: the planes of the mask in order, while 3 or more remain
+    int n = 3;
+    for (int p = 0; p < 6; ++p) {
+        if (!(cm & (1 << p)) || n < 3)
+            continue;
+        const bool act = (int)lane < n;
+        const int nx = (int)lane + 1 < n ? (int)lane + 1 : 0;
+        const float dp = plane_dot(p, P);
+        const float dn = __shfl(dp, nx);
+        const bool vin = act && dp >= 0;
+        const bool cross = act && sgnf(dn) != sgnf(dp);
+        const unsigned long long m_in = __ballot(vin), m_cr = __ballot(cross);
+        const unsigned long long below = (1ull << lane) - 1ull;
+        const int m = __popcll(m_in) + __popcll(m_cr);
+        if (m > MAX_CLIP) {
+            if (lane == 0)
+                atomicOr(&a.status[ST_ERROR], 2ull);
+            n = 0;
+            break;
+        }
+        const int slot = __popcll(m_in & below) + __popcll(m_cr & below);
+        Pos4 Pn;
+        Pn.x = __shfl(P.x, nx);
+        Pn.y = __shfl(P.y, nx);
+        Pn.z = __shfl(P.z, nx);
+        Pn.w = __shfl(P.w, nx);
+        float vn[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            vn[k] = EMIT ? __shfl(var[k], nx) : 0.0f;
+        __syncthreads();  // the previous pass has read the stage
+        if (vin) {
+            st.p[slot] = P;
+#pragma unroll
+            for (int k = 0; k < NV; ++k)
+                if (EMIT)
+                    st.v[slot][k] = var[k];
+        }
+        if (cross) {
+            const float tt = dn < 0 ? dp / (dp - dn) : -dp / (dn - dp);
+            Pos4 R;
+            R.x = lerp1(P.x, Pn.x, tt);
+            R.y = lerp1(P.y, Pn.y, tt);
+            R.z = lerp1(P.z, Pn.z, tt);
+            R.w = lerp1(P.w, Pn.w, tt);
+            const int o = slot + (vin ? 1 : 0);
+            st.p[o] = R;
+            // Helper::interpolateVertex lerps the vertex shader's counts only
+#pragma unroll
+            for (int k = 0; k < NV; ++k) {
+                const bool lerped = k < a.lay.na ? (k < a.vs_na) : ((k - a.lay.na) < a.vs_np);
+                if (EMIT)
+                    st.v[o][k] = lerped ? lerp1(var[k], vn[k], tt) : 0.0f;
+            }
+        }
+        __syncthreads();
+        n = m;
+        if ((int)lane < n) {
+            P = st.p[lane];
+#pragma unroll
+            for (int k = 0; k < NV; ++k)
+                if (EMIT)
+                    var[k] = st.v[lane][k];
+        }
+    }
+    // fan k = (p0, p[k+1], p[k+2]) on lane k
+    const int nf = n >= 3 ? n - 2 : 0;
+    const WinV wl = to_window(a, P);
+    const int i1 = (int)lane + 1 < 64 ? (int)lane + 1 : 63;
+    const int i2 = (int)lane + 2 < 64 ? (int)lane + 2 : 63;
+    WinV w0 = shfl_win(wl, 0), w2 = shfl_win(wl, i2);
+    const WinV w1 = shfl_win(wl, i1);
+    float v0[NV], v1[NV], v2[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        v0[k] = EMIT ? __shfl(var[k], 0) : 0.0f;
+        v1[k] = EMIT ? __shfl(var[k], i1) : 0.0f;
+        v2[k] = EMIT ? __shfl(var[k], i2) : 0.0f;
+    }
+    bool swap = false, drawn = false;
+    PrimGeo g;
+    uint32_t nt = 0;
+    if ((int)lane < nf) {
+        drawn = fan_geometry(a, w0, w1, w2, true, g, swap);
+        if (drawn)
+            nt = (uint32_t)rect_tiles(g);
+    }
+    const unsigned long long m_drawn = __ballot(drawn);
+    const uint32_t nt1 = lane >= 1 ? nt : 0u;  // pairs of the appended fans
+    const uint32_t incl = wave_incl_sum(nt1, lane);
+    if (!EMIT) {
+        const uint32_t tot1 = __shfl(incl, 63);
+        if (lane == 0) {
+            a.grp[(size_t)b * 2048 + j] = make_uint2(drawn ? 1u : 0u, nt);
+            a.grp[(size_t)b * 2048 + 1024 + j] =
+                make_uint2((uint32_t)__popcll(m_drawn & ~1ull), tot1);
+            a.app[t] = make_uint2(n > 3 ? (uint32_t)(n - 3) : 0u, 0u);
+        }
+        return;
+    }
+    if (!drawn)
+        return;
+    uint64_t pos = pos0, pb = pairs0;
+    uint32_t key = (kb << 15) + j;
+    if (lane >= 1) {
+        pos = pos1 + (uint64_t)__popcll(m_drawn & ((1ull << lane) - 1ull) & ~1ull);
+        pb = pairs1 + (incl - nt1);
+        key = (kb << 15) + slot1 + (lane - 1);
+    }
+    fan_write<NV>(a, pos, pb, nt, key, src, sid, g, w0, w1, w2, v0, v1, v2, swap);
+}
+
+// The queued triangles, densely packed (order does not matter: each writes its own (batch,
+// phase, j) counts and draw-order positions).  Each workgroup prefix-sums the region counts,
+// then takes dense indices blockIdx.x, blockIdx.x + gridDim.x, ... and maps each to its
+// (region, slot).
 template <int NV, bool EMIT>
 __device__ void run_clip_queue(const GeomArgs &a)
 {
     __shared__ uint32_t s_off[CLIPQ_REGIONS + 1];
-    static_assert(CLIPQ_REGIONS == 64, "one region per lane of the first wave");
-    if (threadIdx.x < 64) {
+    __shared__ ClipStage<NV> st;
+    static_assert(CLIPQ_REGIONS == CLIPQ_THREADS, "one region per lane");
+    {
         // the 64 counters load in parallel, one per lane; an inclusive scan across the wave
         const uint32_t lane = threadIdx.x;
         const uint32_t cnt = a.clipq_cnt[lane * CLIPQ_STRIDE];
-        uint32_t incl = cnt;
-#pragma unroll
-        for (int sh = 1; sh < 64; sh <<= 1) {
-            const uint32_t o = __shfl_up(incl, sh, 64);
-            incl += lane >= (uint32_t)sh ? o : 0u;
-        }
+        const uint32_t incl = wave_incl_sum(cnt, lane);
         s_off[lane] = incl - cnt;
         if (lane == 63)
             s_off[CLIPQ_REGIONS] = incl;
     }
     __syncthreads();
     const uint32_t n = s_off[CLIPQ_REGIONS], cap = clipq_region_cap(a.tris);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
         uint32_t lo = 0, hi = CLIPQ_REGIONS;  // last region with s_off <= i
         while (hi - lo > 1) {
             uint32_t mid = (lo + hi) >> 1;
@@ -1291,13 +1256,13 @@ __device__ void run_clip_queue(const GeomArgs &a)
             else
                 hi = mid;
         }
-        process_triangle<NV, EMIT, true>(a, a.clipq[lo * cap + (i - s_off[lo])]);
+        clip_queued<NV, EMIT>(a, a.clipq[lo * cap + (i - s_off[lo])], st);
     }
 }
 
 __global__ void __launch_bounds__(CLIPQ_THREADS) k_clip_count(GeomArgs a)
 {
-    run_clip_queue<0, false>(a);
+    run_clip_queue<1, false>(a);
 }
 
 template <int NV>
@@ -1309,12 +1274,13 @@ __global__ void __launch_bounds__(CLIPQ_THREADS) k_clip_emit(GeomArgs a)
 }
 
 // ---- host launchers -----------------------------------------------------------------------
-// queue kernels: enough threads for a tenth of the triangles (they loop beyond), at least
-// one block, at most 2048
+// queue kernels: a workgroup (one wave) per queued triangle, looping beyond the grid; up to
+// 4096 workgroups (16 waves per CU), so that a 1M-triangle draw whose queue holds up to 0.4%
+// of its triangles gives each workgroup one
 static uint32_t clip_blocks(uint64_t tris)
 {
-    const uint64_t b = (tris / 10 + CLIPQ_THREADS - 1) / CLIPQ_THREADS + 1;
-    return (uint32_t)(b < 2048 ? b : 2048);
+    const uint64_t b = tris / 64 + 64;
+    return (uint32_t)(b < 4096 ? b : 4096);
 }
 hipError_t launch_begin(const GeomArgs &a, hipStream_t s)
 {
@@ -1368,15 +1334,15 @@ int emit_class(int nvary)
     return 32;
 }
 
-// The clipped triangles' emit (k_clip_emit) and the others' (k_geom_emit) write disjoint
-// records and pairs, so with a side stream the clip queue runs beside the main emit: its
-// few, long-latency waves fill what the main emit leaves idle.
+// The clipped triangles' emit (k_clip_emit) and the others' write disjoint records, one after
+// the other on the draw stream.  (The queue emit is short: a second stream beside the main
+// emit measured slower, its fork and join costing more than the overlap saved.)
 template <int NV>
-static void emit_kernels(const GeomArgs &a, hipStream_t s, hipStream_t side, bool q)
+static void emit_kernels(const GeomArgs &a, hipStream_t s, bool q)
 {
     dim3 g((a.tris + 255) / 256), blk(256), gq(clip_blocks(a.tris)), bq(CLIPQ_THREADS);
     if (q)
-        hipLaunchKernelGGL(k_clip_emit<NV == 0 ? 1 : NV>, gq, bq, 0, side ? side : s, a);
+        hipLaunchKernelGGL(k_clip_emit<NV == 0 ? 1 : NV>, gq, bq, 0, s, a);
     if (a.live) {  // at most 2048 workgroups walk the live list
         const dim3 gl(g.x < 2048 ? g.x : 2048);
         if (a.nscenes)
@@ -1388,34 +1354,17 @@ static void emit_kernels(const GeomArgs &a, hipStream_t s, hipStream_t side, boo
     }
 }
 
-hipError_t launch_geom_emit(const GeomArgs &a, hipStream_t s, hipStream_t side, hipEvent_t fork,
-                            hipEvent_t join)
+hipError_t launch_geom_emit(const GeomArgs &a, hipStream_t s)
 {
     if (!a.tris)
         return hipSuccess;
     const bool q = !a.window_input;
-    const bool split = q && side && fork && join;
-    if (split) {
-        hipError_t e = hipEventRecord(fork, s);
-        if (e == hipSuccess)
-            e = hipStreamWaitEvent(side, fork, 0);
-        if (e != hipSuccess)
-            return e;
-    }
-    hipStream_t sd = split ? side : nullptr;
     switch (emit_class(a.lay.na + a.lay.np)) {
-    case 0: emit_kernels<0>(a, s, sd, q); break;
-    case 3: emit_kernels<3>(a, s, sd, q); break;
-    case 5: emit_kernels<5>(a, s, sd, q); break;
-    case 8: emit_kernels<8>(a, s, sd, q); break;
-    default: emit_kernels<32>(a, s, sd, q); break;
-    }
-    if (split) {
-        hipError_t e = hipEventRecord(join, side);
-        if (e == hipSuccess)
-            e = hipStreamWaitEvent(s, join, 0);
-        if (e != hipSuccess)
-            return e;
+    case 0: emit_kernels<0>(a, s, q); break;
+    case 3: emit_kernels<3>(a, s, q); break;
+    case 5: emit_kernels<5>(a, s, q); break;
+    case 8: emit_kernels<8>(a, s, q); break;
+    default: emit_kernels<32>(a, s, q); break;
     }
     return hipGetLastError();
 }

@@ -18,10 +18,7 @@ struct SortBuffers {
 // geometry (swr_geometry.hip)
 hipError_t launch_begin(const GeomArgs &a, hipStream_t s);
 hipError_t launch_geom_count(const GeomArgs &a, hipStream_t s);
-// side, fork, join: a second stream and two events; the clip queue's emit runs on `side`
-// between a fork and a join (all null: one stream)
-hipError_t launch_geom_emit(const GeomArgs &a, hipStream_t s, hipStream_t side, hipEvent_t fork,
-                            hipEvent_t join);
+hipError_t launch_geom_emit(const GeomArgs &a, hipStream_t s);
 hipError_t launch_expand(const GeomArgs &a, hipStream_t s, int chunk);
 
 int emit_class(int nvary);  // varyings carried per vertex: 0, 3, 5, 8 or 32
