@@ -17,7 +17,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_SO = os.path.join(HERE, "liboracle.so")
+# SWR_ORACLE_LIB: another build of the same sources (the sanitizer build of `make san`)
+ORACLE_SO = os.environ.get("SWR_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libswr_ref.so")
 REF_OMP_SO = os.path.join(HERE, "_ref", "libswr_ref_omp.so")
 REF_OBJ_SO = os.path.join(HERE, "_ref", "libswr_refobj.so")
