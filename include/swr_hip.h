@@ -133,6 +133,11 @@ typedef struct swr_vertex_shader_desc {
  *                         triangle planes: SWR_ERR_UNSUPPORTED).  log2 of the level of
  *                         detail is correctly rounded (the reference calls the C library's
  *                         log2f, which may round the other way by one ulp)
+ *   SWR_PS_MODULE         a program compiled outside the library: `module` (below) with
+ *                         its own drawPixel, built against include/swr/pixel_program.h;
+ *                         the module's statics replace the four counts of the descriptor;
+ *                         `uniforms` (device memory) is handed to every drawPixel call.
+ *                         No target is required: the program writes what it writes.
  * "key" identifies the primitive in draw order: (batch << 15) + slot, where slot is the
  * triangle's position in the batch's index list after clipping (VertexProcessor.cpp:254-261);
  * for swr_draw_triangle_list it is the triangle's position in the list.
@@ -145,6 +150,27 @@ typedef struct swr_vertex_shader_desc {
 #define SWR_PS_LOG             5
 #define SWR_PS_VISIBILITY      6
 #define SWR_PS_TEXTURE         7
+#define SWR_PS_MODULE          8
+
+/* A pixel program compiled outside the library: the device counterpart of a class derived
+ * from PixelShaderBase (PixelShaderBase.h:36-119), whose static drawPixel(const PixelData &)
+ * the rasterizer calls for every covered pixel in the reference's order (for each pixel, its
+ * primitives in draw order).  include/swr/pixel_program.h builds it from such a class; the
+ * module's entry point (SWR_PIXEL_MODULE) fills this struct, which swr_set_pixel_shader
+ * copies.  The two launchers enqueue the raster stage of a triangle draw and of a point or
+ * line draw; `args` is the library's raster argument block, of args_bytes bytes. */
+#define SWR_PIXEL_MODULE_ABI   1u
+typedef struct swr_pixel_module {
+    uint32_t abi;              /* SWR_PIXEL_MODULE_ABI                                  */
+    uint32_t args_bytes;       /* size of the raster argument block it was built with   */
+    int32_t  interpolate_z;    /* PixelShaderBase::InterpolateZ                         */
+    int32_t  interpolate_w;    /* PixelShaderBase::InterpolateW                         */
+    int32_t  avar_count;       /* PixelShaderBase::AVarCount (0..SWR_MAX_AVARS)         */
+    int32_t  pvar_count;       /* PixelShaderBase::PVarCount (0..SWR_MAX_PVARS)         */
+    int    (*launch_triangles)(const void *args, void *stream, int waves);
+    int    (*launch_fragments)(const void *args, void *stream);
+    const char *name;
+} swr_pixel_module;
 
 #define SWR_LOG_WORDS          (7 + SWR_MAX_AVARS + SWR_MAX_PVARS)
 #define SWR_HASH_SEED          2166136261u
@@ -166,6 +192,8 @@ typedef struct swr_pixel_shader_desc {
     float   checker_cells;     /* SWR_PS_DEPTH_CHECKER uniform  */
     int32_t reserved;
     const void *texture;       /* SWR_PS_TEXTURE: a swr_texture */
+    const swr_pixel_module *module;  /* SWR_PS_MODULE                 */
+    const void *uniforms;      /* SWR_PS_MODULE: device memory handed to drawPixel */
 } swr_pixel_shader_desc;
 
 /* ---- context ------------------------------------------------------------------------ */

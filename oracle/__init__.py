@@ -50,7 +50,8 @@ class PixelShaderDesc(C.Structure):
                 ("pitch", C.c_int32), ("target0", C.c_void_p), ("target1", C.c_void_p),
                 ("target2", C.c_void_p), ("log", C.c_void_p), ("log_capacity", C.c_uint64),
                 ("log_count", C.c_void_p), ("checker_cells", C.c_float),
-                ("reserved", C.c_int32), ("texture", C.c_void_p)]
+                ("reserved", C.c_int32), ("texture", C.c_void_p),
+                ("module", C.c_void_p), ("uniforms", C.c_void_p)]
 
 
 class PipelineState(C.Structure):

@@ -36,7 +36,20 @@ class PixelShaderDesc(C.Structure):
                 ("pitch", C.c_int32), ("target0", C.c_void_p), ("target1", C.c_void_p),
                 ("target2", C.c_void_p), ("log", C.c_void_p), ("log_capacity", C.c_uint64),
                 ("log_count", C.c_void_p), ("checker_cells", C.c_float),
-                ("reserved", C.c_int32), ("texture", C.c_void_p)]
+                ("reserved", C.c_int32), ("texture", C.c_void_p),
+                ("module", C.c_void_p), ("uniforms", C.c_void_p)]
+
+
+PIXEL_MODULE_ABI = 1
+
+
+class PixelModule(C.Structure):
+    """swr_pixel_module: a pixel program compiled outside the library."""
+    _fields_ = [("abi", C.c_uint32), ("args_bytes", C.c_uint32),
+                ("interpolate_z", C.c_int32), ("interpolate_w", C.c_int32),
+                ("avar_count", C.c_int32), ("pvar_count", C.c_int32),
+                ("launch_triangles", C.c_void_p), ("launch_fragments", C.c_void_p),
+                ("name", C.c_char_p)]
 
 
 class DrawStats(C.Structure):

@@ -33,7 +33,8 @@ class CullMode:
 
 # program ids
 VS_DUMMY, VS_RAW, VS_POS_RGB, VS_MVP_POS_RGB_UV, VS_MVP_POS_UV = 0, 1, 2, 3, 4
-PS_NULL, PS_DEPTH_CHECKER, PS_RGB, PS_COVERAGE, PS_HASH, PS_LOG, PS_VISIBILITY, PS_TEXTURE = range(8)
+(PS_NULL, PS_DEPTH_CHECKER, PS_RGB, PS_COVERAGE, PS_HASH, PS_LOG, PS_VISIBILITY, PS_TEXTURE,
+ PS_MODULE) = range(9)
 
 RASTERIZER_VERTEX = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("w", "<f4"),
                               ("avar", "<f4", (_abi.MAX_AVARS,)),
@@ -84,6 +85,13 @@ class PixelShaderBase:
         d.checker_cells = float(getattr(self, "checker_cells", 8.0))
         tex = getattr(self, "texture", None)
         d.texture = tex.handle if tex is not None else None
+        mod = getattr(self, "module", None)
+        if mod is not None:
+            keep.append(mod)
+            d.module = C.addressof(mod.desc)
+            p, _, k = _buf(self.uniforms) if self.uniforms is not None else (None, 0, None)
+            keep.append(k)
+            d.uniforms = p
         log = getattr(self, "log", None)
         if log is not None:
             p, nbytes, k = _buf(log)
@@ -226,6 +234,47 @@ class TextureShader(PixelShaderBase):
 
     def targets(self):
         return (self.color, None, None) + _hw(self.color)
+
+
+class PixelModule:
+    """A pixel program compiled outside the library (include/swr/pixel_program.h): the shared
+    library at `path`, whose entry point `entry` (SWR_PIXEL_MODULE) fills the module
+    descriptor.  The library stays loaded while this object lives."""
+
+    def __init__(self, path: str, entry: str):
+        self._lib = C.CDLL(path)
+        fn = getattr(self._lib, entry)
+        fn.restype = C.c_int
+        fn.argtypes = [C.POINTER(_abi.PixelModule)]
+        self.desc = _abi.PixelModule()
+        rc = fn(C.byref(self.desc))
+        if rc != 0:
+            raise _abi.SwrError(rc, f"pixel module entry {entry} failed")
+        self.path, self.entry = path, entry
+
+    @property
+    def name(self) -> str:
+        return (self.desc.name or b"").decode()
+
+
+class ModuleShader(PixelShaderBase):
+    """A pixel module bound with its uniforms (device memory the program reads, e.g. a
+    tensor holding a struct) and the size of the target area in pixels.  The statics
+    (InterpolateZ, InterpolateW, AVarCount, PVarCount) are the module's."""
+
+    Program = PS_MODULE
+
+    def __init__(self, module: PixelModule, uniforms, width: int, height: int,
+                 pitch: Optional[int] = None):
+        self.module, self.uniforms = module, uniforms
+        self.width, self.height = width, height
+        self.pitch = width if pitch is None else pitch
+        d = module.desc
+        self.InterpolateZ, self.InterpolateW = bool(d.interpolate_z), bool(d.interpolate_w)
+        self.AVarCount, self.PVarCount = d.avar_count, d.pvar_count
+
+    def targets(self):
+        return None, None, None, self.width, self.height, self.pitch
 
 
 class VertexShaderBase:

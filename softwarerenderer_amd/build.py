@@ -57,6 +57,37 @@ def build_library(verbose: bool = False) -> str:
     return LIB
 
 
+def build_pixel_module(src: str, out: str) -> str:
+    """Compile a pixel program module (a .hip file using include/swr/pixel_program.h) into
+    the shared library `out`, with the library's own flags (the raster chains must round as
+    the library's do).  Returns `out`."""
+    if shutil.which(HIPCC) is None and not os.path.exists(HIPCC):
+        raise RuntimeError(f"hipcc not found at {HIPCC}")
+    tmp = out + ".tmp"
+    r = subprocess.run([HIPCC, *FLAGS, "-shared", src, "-o", tmp], capture_output=True,
+                       text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")
+    os.replace(tmp, out)
+    return out
+
+
+TEST_MODULES = os.path.join(ROOT, "tests", "modules")
+
+
+def build_test_modules() -> list:
+    """tests/modules/*.hip -> tests/modules/lib*.so (the pixel modules the tests load)."""
+    outs = []
+    if not os.path.isdir(TEST_MODULES):
+        return outs
+    srcs = sorted(f for f in os.listdir(TEST_MODULES) if f.endswith(".hip"))
+    with cf.ThreadPoolExecutor(max_workers=max(1, min(8, len(srcs)))) as ex:
+        outs = list(ex.map(lambda f: build_pixel_module(
+            os.path.join(TEST_MODULES, f),
+            os.path.join(TEST_MODULES, "lib" + os.path.splitext(f)[0] + ".so")), srcs))
+    return outs
+
+
 def build_cpp_example() -> str:
     """tests/cpp/render_scene: the C++ host API (include/swr/Renderer.h) driving the library."""
     src = os.path.join(ROOT, "tests", "cpp", "render_scene.cpp")
@@ -73,3 +104,4 @@ def build_cpp_example() -> str:
 if __name__ == "__main__":
     build_library(verbose=True)
     build_cpp_example()
+    build_test_modules()

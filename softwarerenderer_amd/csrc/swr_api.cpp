@@ -49,6 +49,7 @@ struct swr_context {
     int raster_mode = SWR_RASTER_SPAN;
     int sc[4] = {0, 0, 0, 0};
     swr_pixel_shader_desc ps{};
+    swr_pixel_module module{};                // SWR_PS_MODULE: the bound module (ps.module)
 
     // VertexProcessor (VertexProcessor.cpp:29-35 defaults)
     int vp[4] = {0, 0, 0, 0};
@@ -281,6 +282,9 @@ int map_targets(swr_context *c, TargetMap &m)
     if (ps.width < 0 || ps.height < 0 || ps.pitch < ps.width)
         return fail(c, SWR_ERR_INVALID_ARG, "bad target size %dx%d pitch %d", ps.width, ps.height,
                     ps.pitch);
+    // a module's uniforms are read by its kernels as they are: nothing stages host memory
+    if (ps.program == SWR_PS_MODULE && ps.uniforms && !is_device_ptr(ps.uniforms))
+        return fail(c, SWR_ERR_INVALID_ARG, "a pixel module's uniforms must be device memory");
     bool need[3];
     targets_needed(ps.program, need);
     void *const t[3] = {ps.target0, ps.target1, ps.target2};
@@ -847,6 +851,7 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
         r.log_cap = ps.log_capacity;
         r.log_count = tm.logcount;
         r.checker_cells = ps.checker_cells;
+        r.user = ps.uniforms;
         if (bt) {
             r.scenes = g.scenes;
             r.nscenes = g.nscenes;
@@ -1065,6 +1070,7 @@ int draw_lines_points(swr_context *c, int mode, const int32_t *indices, size_t c
         r.log_cap = ps.log_capacity;
         r.log_count = tm.logcount;
         r.checker_cells = ps.checker_cells;
+        r.user = ps.uniforms;
         HIP_TRY(c, launch_lines_raster(ps, L, r, c->stream));
         record(c, 6);
         HIP_TRY(c, hipMemcpyAsync(c->status_host, c->status.p, ST_WORDS * 8,
@@ -1285,18 +1291,46 @@ int swr_set_pixel_shader(swr_context *c, const swr_pixel_shader_desc *d)
 {
     if (!c || !d)
         return fail(c, SWR_ERR_INVALID_ARG, "null pixel shader");
-    if (d->program < SWR_PS_NULL || d->program > SWR_PS_TEXTURE)
+    if (d->program < SWR_PS_NULL || d->program > SWR_PS_MODULE)
         return fail(c, SWR_ERR_UNSUPPORTED, "pixel program %d is not built", d->program);
     if (d->program == SWR_PS_TEXTURE) {
         const swr_texture *t = static_cast<const swr_texture *>(d->texture);
         if (!t || t->ctx != c)
             return fail(c, SWR_ERR_INVALID_ARG, "the texture program needs a texture of this context");
     }
+    swr_pixel_shader_desc ps = *d;
+    if (d->program == SWR_PS_MODULE) {
+        // a program built outside the library: it must have been compiled against this
+        // library's raster argument block; its statics replace the descriptor's counts
+        const swr_pixel_module *m = d->module;
+        if (!m)
+            return fail(c, SWR_ERR_INVALID_ARG, "SWR_PS_MODULE needs a module");
+        if (m->abi != SWR_PIXEL_MODULE_ABI || m->args_bytes != (uint32_t)sizeof(RasterArgs))
+            return fail(c, SWR_ERR_UNSUPPORTED,
+                        "pixel module %s was built for another library (abi %u, %u-byte "
+                        "arguments; this library: abi %u, %zu bytes)",
+                        m->name ? m->name : "?", m->abi, m->args_bytes, SWR_PIXEL_MODULE_ABI,
+                        sizeof(RasterArgs));
+        if (!m->launch_triangles || !m->launch_fragments)
+            return fail(c, SWR_ERR_INVALID_ARG, "pixel module without launchers");
+        if (m->avar_count < 0 || m->avar_count > SWR_MAX_AVARS || m->pvar_count < 0 ||
+            m->pvar_count > SWR_MAX_PVARS)
+            return fail(c, SWR_ERR_UNSUPPORTED, "pixel module with %d avars / %d pvars",
+                        m->avar_count, m->pvar_count);
+        ps.interpolate_z = m->interpolate_z ? 1 : 0;
+        ps.interpolate_w = m->interpolate_w ? 1 : 0;
+        ps.avar_count = m->avar_count;
+        ps.pvar_count = m->pvar_count;
+    }
     ParamLayout L;
-    if (d->program != SWR_PS_NULL && !raster_layout(*d, L))
+    if (ps.program != SWR_PS_NULL && !raster_layout(ps, L))
         return fail(c, SWR_ERR_UNSUPPORTED, "pixel program %d cannot run with these counts",
-                    d->program);
-    c->ps = *d;
+                    ps.program);
+    if (ps.program == SWR_PS_MODULE) {
+        c->module = *d->module;
+        ps.module = &c->module;
+    }
+    c->ps = ps;
     return SWR_OK;
 }
 

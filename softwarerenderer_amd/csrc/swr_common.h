@@ -309,6 +309,7 @@ struct RasterArgs {
     int xcd;                    // 1: tile blocks grouped per XCD
     const TexDev *tex;          // SWR_PS_TEXTURE
     int fold;                   // 1: an asynchronous draw; its failures go to ST_STICKY
+    const void *user;           // SWR_PS_MODULE: the program's uniforms (device memory)
 };
 
 // An asynchronous draw's failures, kept for the next swr_synchronize: once every earlier

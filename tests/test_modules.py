@@ -1,0 +1,45 @@
+"""Pixel modules built outside the library load and describe themselves without a GPU: the
+entry point fills the module descriptor (ABI version, the raster argument block's size and
+the program's statics) and the library refuses a descriptor from another build."""
+import ctypes as C
+import os
+
+import pytest
+
+from softwarerenderer_amd import _abi, api
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MODULES = os.path.join(ROOT, "tests", "modules")
+
+
+@pytest.mark.parametrize("lib,entry,statics", [
+    ("libfragment_hash.so", "swr_module_hash_zw32", (1, 1, 3, 2)),
+    ("libfragment_hash.so", "swr_module_hash_p4", (0, 0, 0, 4)),
+    ("libdepth_checker.so", "swr_module_depth_checker", (1, 1, 3, 2)),
+])
+def test_module_descriptor(lib, entry, statics):
+    path = os.path.join(MODULES, lib)
+    if not os.path.exists(path):
+        pytest.skip(f"{lib} is not built")
+    m = api.PixelModule(path, entry)
+    d = m.desc
+    assert d.abi == _abi.PIXEL_MODULE_ABI
+    assert d.args_bytes > 0 and d.args_bytes % 8 == 0
+    assert (d.interpolate_z, d.interpolate_w, d.avar_count, d.pvar_count) == statics
+    assert d.launch_triangles and d.launch_fragments
+    assert m.name
+    # all modules are built against the same raster argument block
+    other = api.PixelModule(os.path.join(MODULES, "libdepth_checker.so"),
+                            "swr_module_depth_checker")
+    assert other.desc.args_bytes == d.args_bytes
+
+
+def test_module_entry_rejects_null():
+    path = os.path.join(MODULES, "libdepth_checker.so")
+    if not os.path.exists(path):
+        pytest.skip("libdepth_checker.so is not built")
+    lib = C.CDLL(path)
+    fn = lib.swr_module_depth_checker
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_void_p]
+    assert fn(None) == -1
