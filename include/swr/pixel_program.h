@@ -123,13 +123,14 @@ struct ModuleProgram {
                 for (int k = 0; k < NP; ++k)
                     p.pvar[k] = v.pvar(k, p.w);
             }
-            const float *rec = PX::REPLAY ? v.rec : nullptr;
+            const float *rec =
+                PX::REPLAY ? a.prims + (size_t)v.prim * (size_t)a.stride : nullptr;
             p.planes_invw = rec && NW ? rec + REC_PARAM + 3 * L::IW : nullptr;
             p.planes_pvar = rec && NP ? rec + REC_PARAM + 3 * L::P : nullptr;
             Shader::drawPixel(p, a.user);
         }
         template <class PX>
-        __device__ static void store(const State &, const RasterArgs &, size_t) {}
+        __device__ static void store(const State &, const RasterArgs &, size_t, int, int) {}
     };
 
     static int launch_triangles(const void *args, void *stream, int waves)

@@ -29,7 +29,7 @@ struct VertexPixel {
     __device__ static float w_of(float raw) { return raw; }
     __device__ static float pvar_of(float raw, float) { return raw; }
     static constexpr bool REPLAY = false;  // no planes: programs keep the values
-    const float *rec = nullptr;
+    uint32_t prim = 0;
 };
 
 // One wave64 per 8x8 tile, lane = pixel: the tile's fragments, 64 at a time, in order.
@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(256) k_lines_raster(RasterArgs a)
         }
     }
     if (inside)
-        Prog::template store<VertexPixel<L>>(st, a, o);
+        Prog::template store<VertexPixel<L>>(st, a, o, px, py);
 }
 
 template <template <class> class ProgT, int NZ, int NW, int NA, int NP>

@@ -32,7 +32,7 @@ __device__ inline float clamp01(float x) { return x > 0.0f ? (x < 1.0f ? x : 1.0
 template <class L>
 struct ChainPixel {
     const float *v;
-    const float *rec;  // the primitive's record (its planes), for programs with PLANES set
+    uint32_t prim;     // the primitive's record index (its planes), for programs with PLANES set
     __device__ float z() const { return v[L::Z]; }
     __device__ float invw() const { return v[L::IW]; }
     __device__ float w() const { return 1.0f / v[L::IW]; }
@@ -59,12 +59,22 @@ __device__ inline float chain_at(const float *rec, int p, int x, int y, int sc_m
                 pc = rec[REC_PARAM + 3 * p + 2];
     const uint32_t flags = __float_as_uint(rec[REC_FLAGS]);
     if (flags & FLAG_BLOCK) {
+        // y - by additions of b, then x - bx of a (0..7 each), as selects from the running
+        // sums: no per-lane trip counts
         const int bx = x & ~(TILE - 1), by = y & ~(TILE - 1);
-        float r = pa * ((float)bx + 0.5f) + pb * ((float)by + 0.5f) + pc;
-        for (int k = by; k < y; ++k)
-            r = r + pb;
-        for (int i = bx; i < x; ++i)
-            r = r + pa;
+        const int dy = y - by, dx = x - bx;
+        float r = pa * ((float)bx + 0.5f) + pb * ((float)by + 0.5f) + pc, t = r;
+#pragma unroll
+        for (int k = 1; k < TILE; ++k) {
+            t = t + pb;
+            r = k <= dy ? t : r;
+        }
+        t = r;
+#pragma unroll
+        for (int i = 1; i < TILE; ++i) {
+            t = t + pa;
+            r = i <= dx ? t : r;
+        }
         return r;
     }
     int xl = 0, xr = 0;
@@ -85,7 +95,7 @@ struct ProgCoverage {
         s.hit = true;
     }
     template <class PX = void>
-    __device__ static void store(const State &s, const RasterArgs &a, size_t o)
+    __device__ static void store(const State &s, const RasterArgs &a, size_t o, int x, int y)
     {
         if (s.hit)
             static_cast<uint32_t *>(a.t0)[o] = 1u;
@@ -100,12 +110,12 @@ struct ProgRGB {
     static constexpr int KEY_WORD = REC_KEY;
     static constexpr bool PLANES = true;
     static constexpr int CHAINED = 0;
-    struct State { uint32_t c; const float *rec; bool hit; };
+    struct State { uint32_t c; uint32_t prim; bool hit; };
     __device__ static void load(State &s, const RasterArgs &, size_t)
     {
         s.hit = false;
         s.c = 0;
-        s.rec = nullptr;
+        s.prim = 0;
     }
     __device__ static uint32_t pack(float r0, float g0, float b0)
     {
@@ -118,22 +128,22 @@ struct ProgRGB {
     __device__ static void shade(State &s, const RasterArgs &, const PX &v, uint32_t, int, int)
     {
         if constexpr (PX::REPLAY)
-            s.rec = v.rec;
+            s.prim = v.prim;
         else
             s.c = pack(v.avar(0), v.avar(1), v.avar(2));
         s.hit = true;
     }
     template <class PX>
-    __device__ static void store(const State &s, const RasterArgs &a, size_t o)
+    __device__ static void store(const State &s, const RasterArgs &a, size_t o, int x, int y)
     {
         if (!s.hit)
             return;
         uint32_t c = s.c;
         if constexpr (PX::REPLAY) {
-            const int y = (int)(o / (size_t)a.pitch), x = (int)(o - (size_t)y * a.pitch);
-            c = pack(chain_at(s.rec, L::A, x, y, a.sc_minx, a.sc_maxx),
-                     chain_at(s.rec, L::A + 1, x, y, a.sc_minx, a.sc_maxx),
-                     chain_at(s.rec, L::A + 2, x, y, a.sc_minx, a.sc_maxx));
+            const float *rec = a.prims + (size_t)s.prim * (size_t)a.stride;
+            c = pack(chain_at(rec, L::A, x, y, a.sc_minx, a.sc_maxx),
+                     chain_at(rec, L::A + 1, x, y, a.sc_minx, a.sc_maxx),
+                     chain_at(rec, L::A + 2, x, y, a.sc_minx, a.sc_maxx));
         }
         static_cast<uint32_t *>(a.t0)[o] = c;
     }
@@ -149,12 +159,12 @@ struct ProgDepthChecker {
     static constexpr int KEY_WORD = REC_KEY;
     static constexpr bool PLANES = true;
     static constexpr int CHAINED = 1;  // z
-    struct State { float depth, wr, p0, p1, a0, a1, a2; const float *rec; bool hit; };
+    struct State { float depth, wr, p0, p1, a0, a1, a2; uint32_t prim; bool hit; };
     __device__ static void load(State &s, const RasterArgs &a, size_t o)
     {
         s.depth = static_cast<const float *>(a.t1)[o];
         s.wr = s.p0 = s.p1 = s.a0 = s.a1 = s.a2 = 0.0f;
-        s.rec = nullptr;
+        s.prim = 0;
         s.hit = false;
     }
     template <class PX>
@@ -164,7 +174,7 @@ struct ProgDepthChecker {
         if (z < s.depth) {
             s.depth = z;
             if constexpr (PX::REPLAY) {
-                s.rec = v.rec;
+                s.prim = v.prim;
             } else {
                 s.wr = v.w_raw();
                 s.p0 = v.p_raw(0);
@@ -177,18 +187,18 @@ struct ProgDepthChecker {
         }
     }
     template <class PX>
-    __device__ static void store(const State &s0, const RasterArgs &a, size_t o)
+    __device__ static void store(const State &s0, const RasterArgs &a, size_t o, int x, int y)
     {
         if (s0.hit) {
             State s = s0;
             if constexpr (PX::REPLAY) {
-                const int y = (int)(o / (size_t)a.pitch), x = (int)(o - (size_t)y * a.pitch);
-                s.wr = chain_at(s.rec, L::IW, x, y, a.sc_minx, a.sc_maxx);
-                s.a0 = chain_at(s.rec, L::A, x, y, a.sc_minx, a.sc_maxx);
-                s.a1 = chain_at(s.rec, L::A + 1, x, y, a.sc_minx, a.sc_maxx);
-                s.a2 = chain_at(s.rec, L::A + 2, x, y, a.sc_minx, a.sc_maxx);
-                s.p0 = chain_at(s.rec, L::P, x, y, a.sc_minx, a.sc_maxx);
-                s.p1 = chain_at(s.rec, L::P + 1, x, y, a.sc_minx, a.sc_maxx);
+                const float *rec = a.prims + (size_t)s.prim * (size_t)a.stride;
+                s.wr = chain_at(rec, L::IW, x, y, a.sc_minx, a.sc_maxx);
+                s.a0 = chain_at(rec, L::A, x, y, a.sc_minx, a.sc_maxx);
+                s.a1 = chain_at(rec, L::A + 1, x, y, a.sc_minx, a.sc_maxx);
+                s.a2 = chain_at(rec, L::A + 2, x, y, a.sc_minx, a.sc_maxx);
+                s.p0 = chain_at(rec, L::P, x, y, a.sc_minx, a.sc_maxx);
+                s.p1 = chain_at(rec, L::P + 1, x, y, a.sc_minx, a.sc_maxx);
             }
             const float w = PX::w_of(s.wr);  // only the depth-passing pixels read pvar
             const float u = PX::pvar_of(s.p0, w), t = PX::pvar_of(s.p1, w);
@@ -238,7 +248,7 @@ struct ProgHash {
         s.n += 1;
     }
     template <class PX = void>
-    __device__ static void store(const State &s, const RasterArgs &a, size_t o)
+    __device__ static void store(const State &s, const RasterArgs &a, size_t o, int x, int y)
     {
         if (s.n != s.n0) {
             static_cast<uint32_t *>(a.t0)[o] = s.h;
@@ -282,7 +292,7 @@ struct ProgLog {
         }
     }
     template <class PX = void>
-    __device__ static void store(const State &s, const RasterArgs &a, size_t o)
+    __device__ static void store(const State &s, const RasterArgs &a, size_t o, int x, int y)
     {
         if (s.n != s.n0)
             static_cast<uint32_t *>(a.t0)[o] = s.n;
@@ -314,7 +324,7 @@ struct ProgVisibility {
         }
     }
     template <class PX = void>
-    __device__ static void store(const State &s, const RasterArgs &a, size_t o)
+    __device__ static void store(const State &s, const RasterArgs &a, size_t o, int x, int y)
     {
         if (s.hit) {
             static_cast<float *>(a.t1)[o] = s.depth;
@@ -479,11 +489,11 @@ struct ProgTexture {
     // every fragment overwrites the pixel: shade keeps the last one's record and store
     // samples once, from the planes chained again at the pixel (chain_at)
     static constexpr int CHAINED = 0;
-    struct State { const float *rec; bool hit; };
+    struct State { uint32_t prim; bool hit; };
     __device__ static void load(State &s, const RasterArgs &, size_t)
     {
         s.hit = false;
-        s.rec = nullptr;
+        s.prim = 0;
     }
     __device__ static void derivs(const float *pl, const float *iw, float xf, float yf,
                                   float &ddx, float &ddy)
@@ -496,19 +506,19 @@ struct ProgTexture {
     template <class PX>
     __device__ static void shade(State &s, const RasterArgs &, const PX &v, uint32_t, int, int)
     {
-        s.rec = v.rec;
+        s.prim = v.prim;
         s.hit = true;
     }
     template <class PX>
-    __device__ static void store(const State &s, const RasterArgs &a, size_t o)
+    __device__ static void store(const State &s, const RasterArgs &a, size_t o, int x, int y)
     {
         if (!s.hit)
             return;
-        const int y = (int)(o / (size_t)a.pitch), x = (int)(o - (size_t)y * a.pitch);
-        const float w = PX::w_of(chain_at(s.rec, L::IW, x, y, a.sc_minx, a.sc_maxx));
-        const float u = PX::pvar_of(chain_at(s.rec, L::P, x, y, a.sc_minx, a.sc_maxx), w);
-        const float t = PX::pvar_of(chain_at(s.rec, L::P + 1, x, y, a.sc_minx, a.sc_maxx), w);
-        const float *P = s.rec + REC_PARAM;
+        const float *rec = a.prims + (size_t)s.prim * (size_t)a.stride;
+        const float w = PX::w_of(chain_at(rec, L::IW, x, y, a.sc_minx, a.sc_maxx));
+        const float u = PX::pvar_of(chain_at(rec, L::P, x, y, a.sc_minx, a.sc_maxx), w);
+        const float t = PX::pvar_of(chain_at(rec, L::P + 1, x, y, a.sc_minx, a.sc_maxx), w);
+        const float *P = rec + REC_PARAM;
         const float xf = (float)x + 0.5f, yf = (float)y + 0.5f;
         float dudx, dudy, dvdx, dvdy;
         derivs(P + 3 * (L::P + 0), P + 3 * L::IW, xf, yf, dudx, dudy);

@@ -233,25 +233,32 @@ __global__ void __launch_bounds__(512) k_raster(RasterArgs a0)
                     if (cls == 2) {
                         rowmask = 0xFFu;
                     } else {
-                        rowmask = 0xFFu;
+                        // the three edges' values at this row's start, then stepped along
+                        // the row together: a pixel is in when all three tests pass
+                        float v[3], ea[3];
+                        bool tie[3];
 #pragma unroll
                         for (int e = 0; e < 3; ++e) {
-                            const float ea = rec[REC_EDGE + 3 * e], eb = rec[REC_EDGE + 3 * e + 1],
+                            ea[e] = rec[REC_EDGE + 3 * e];
+                            const float eb = rec[REC_EDGE + 3 * e + 1],
                                         ec = rec[REC_EDGE + 3 * e + 2];
-                            const bool tie = (flags >> (1 + e)) & 1u;
-                            float r = ea * xf + eb * yf + ec, v = r;
+                            tie[e] = (flags >> (1 + e)) & 1u;
+                            float r = ea[e] * xf + eb * yf + ec;
+                            v[e] = r;
 #pragma unroll
                             for (int k = 1; k < 8; ++k) {
                                 r = r + eb;
-                                v = k <= row ? r : v;
+                                v[e] = k <= row ? r : v[e];
                             }
-                            uint32_t m = 0;
+                        }
 #pragma unroll
-                            for (int i = 0; i < 8; ++i) {
-                                m |= edge_pass(v, tie) ? (1u << i) : 0u;
-                                v = v + ea;
-                            }
-                            rowmask &= m;
+                        for (int i = 0; i < 8; ++i) {
+                            const bool in = edge_pass(v[0], tie[0]) && edge_pass(v[1], tie[1]) &&
+                                            edge_pass(v[2], tie[2]);
+                            rowmask |= in ? (1u << i) : 0u;
+#pragma unroll
+                            for (int e = 0; e < 3; ++e)
+                                v[e] = v[e] + ea[e];
                         }
                     }
                 } else if (rowv) {
@@ -311,19 +318,16 @@ __global__ void __launch_bounds__(512) k_raster(RasterArgs a0)
                 if (s2 < cnt) {
                     const unsigned long long m = s_cov[wv][s2];
                     const uint32_t k2 = __builtin_amdgcn_readlane(key, s2 * 8);
-                    const float *rec2 = nullptr;
-                    if (Prog::PLANES) {
-                        const uint32_t pv2 = __builtin_amdgcn_readlane(pv, s2 * 8);
-                        rec2 = a.prims + (size_t)(pv2 & PAIR_PRIM_MASK) * (size_t)a.stride;
-                    }
+                    const uint32_t prim2 =
+                        Prog::PLANES ? __builtin_amdgcn_readlane(pv, s2 * 8) & PAIR_PRIM_MASK : 0u;
                     if (((m >> lane) & 1ull) && inside)
-                        Prog::shade(st, a, ChainPixel<L>{val[s2], rec2}, k2, px, py);
+                        Prog::shade(st, a, ChainPixel<L>{val[s2], prim2}, k2, px, py);
                 }
             }
             lds_sync();
         }
         if (inside && drew)
-            Prog::template store<ChainPixel<L>>(st, a, o);
+            Prog::template store<ChainPixel<L>>(st, a, o, px, py);
     }
 }
 
