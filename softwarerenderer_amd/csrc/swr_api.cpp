@@ -42,6 +42,8 @@ struct swr_context {
     int device = 0;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;               // a second stream for independent kernels
+    hipEvent_t fork = nullptr, join = nullptr;
     std::string err;
     int err_code = SWR_OK;                    // code of the last failure (with err)
 
@@ -1102,6 +1104,16 @@ hipStream_t context_stream(swr_context *c)
     return c->stream;
 }
 
+bool context_side(swr_context *c, hipStream_t *side, hipEvent_t *fork, hipEvent_t *join)
+{
+    if (!c->side || !c->fork || !c->join)
+        return false;
+    *side = c->side;
+    *fork = c->fork;
+    *join = c->join;
+    return true;
+}
+
 int context_fail(swr_context *c, int code, const char *msg) { return fail(c, code, "%s", msg); }
 
 int context_last_code(swr_context *c) { return c->err_code != SWR_OK ? c->err_code : SWR_ERR_HIP; }
@@ -1209,6 +1221,13 @@ int swr_create(swr_context **out, int device)
     }
     for (int i = 0; i <= S_COUNT; ++i)
         (void)hipEventCreate(&c->ev[i]);
+    // optional: without it the kernels it would run run on the context stream
+    if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        c->side = nullptr;
+    }
     *out = c;
     return SWR_OK;
 }
@@ -1234,6 +1253,14 @@ int swr_destroy(swr_context *c)
             (void)hipEventDestroy(c->ev[i]);
     if (c->status_host)
         (void)hipHostFree(c->status_host);
+    if (c->side) {
+        (void)hipStreamSynchronize(c->side);
+        (void)hipStreamDestroy(c->side);
+    }
+    if (c->fork)
+        (void)hipEventDestroy(c->fork);
+    if (c->join)
+        (void)hipEventDestroy(c->join);
     if (c->own)
         (void)hipStreamDestroy(c->own);
     delete c;

@@ -587,23 +587,14 @@ __device__ inline void acc_vertex(const DiffArgs &d, const VertexAcc &A, int vie
     }
 }
 
-// The three vertices of one pixel's triangle: the first probes of the three lookups are
-// issued together (one LDS round trip instead of three in the common case that the vertex
-// already has its slot), then the adds.
+// The same with the three slots already found (acc_slot; -1: straight to global memory).
 template <int C>
-__device__ inline void acc_triangle(const DiffArgs &d, const VertexAcc &A, int view,
-                                   const int (&v)[3], const float (&g)[3][3 + C])
+__device__ inline void acc_triangle_at(const DiffArgs &d, const VertexAcc &A, int view,
+                                       const int (&v)[3], const int (&sl)[3],
+                                       const float (&g)[3][3 + C])
 {
-    int sl[3];
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
-        const int h = (int)((((unsigned)v[m] * 2654435761u) >> 24) & (ACC_SLOTS - 1));
-        sl[m] = __atomic_load_n(&A.key[h], __ATOMIC_RELAXED) == v[m] ? h : -1;
-    }
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-        if (sl[m] < 0)
-            sl[m] = acc_slot(A, v[m]);
         if (sl[m] >= 0) {
             float *x = &A.val[sl[m] * acc_vals<C>() * ACC_COPIES + acc_copy()];
 #pragma unroll
@@ -619,6 +610,31 @@ __device__ inline void acc_triangle(const DiffArgs &d, const VertexAcc &A, int v
                 atomicAdd(d.gattr + crow(d, view, v[m]) * C + c, g[m][3 + c]);
         }
     }
+}
+
+// The slots of the three vertices of one pixel's triangle: the first probes of the three
+// lookups are issued together (one LDS round trip instead of three in the common case that
+// the vertex already has its slot), the rest probe on.
+__device__ inline void acc_find3(const VertexAcc &A, const int (&v)[3], int (&sl)[3])
+{
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        const int h = (int)((((unsigned)v[m] * 2654435761u) >> 24) & (ACC_SLOTS - 1));
+        sl[m] = __atomic_load_n(&A.key[h], __ATOMIC_RELAXED) == v[m] ? h : -1;
+    }
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+        if (sl[m] < 0)
+            sl[m] = acc_slot(A, v[m]);
+}
+
+template <int C>
+__device__ inline void acc_triangle(const DiffArgs &d, const VertexAcc &A, int view,
+                                   const int (&v)[3], const float (&g)[3][3 + C])
+{
+    int sl[3];
+    acc_find3(A, v, sl);
+    acc_triangle_at<C>(d, A, view, v, sl, g);
 }
 
 template <int C>
@@ -1372,8 +1388,18 @@ __global__ void __launch_bounds__(256) k_diff_reduce_attr(DiffArgs d, int C)
     }
 }
 
+// The backward's interior and antialiasing kernels add into the same gradients (commutative
+// atomics) and read nothing the other writes, so with a second stream they run side by side:
+// the interior kernel is bound by LDS atomics, the edge kernel by its arithmetic.  Below
+// SIDE_MIN_TILES tiles (a few 512x512 views) the fork and join cost more than the overlap.
+constexpr uint32_t SIDE_MIN_TILES = 8192;
+struct SideStream {
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+
 template <int C>
-static hipError_t launch_diff(const DiffArgs &d, int which, hipStream_t s)
+static hipError_t launch_diff(const DiffArgs &d, int which, hipStream_t s, const SideStream &side)
 {
     const uint32_t blocks = (uint32_t)((size_t)d.tiles_per_view * d.V);
     if (!blocks)
@@ -1385,8 +1411,25 @@ static hipError_t launch_diff(const DiffArgs &d, int which, hipStream_t s)
         hipLaunchKernelGGL(k_diff_blend<C>, dim3(blocks), dim3(256), 0, s, d);
     } else {
         if (d.pairs == 2) {
+            hipStream_t se = s;
+            const bool fork = side.s && blocks >= SIDE_MIN_TILES;
+            if (fork) {
+                hipError_t e = hipEventRecord(side.fork, s);
+                if (e == hipSuccess)
+                    e = hipStreamWaitEvent(side.s, side.fork, 0);
+                if (e != hipSuccess)
+                    return e;
+                se = side.s;
+            }
+            hipLaunchKernelGGL(k_diff_bwd_edges<C>, dim3(groups), dim3(256), 0, se, d);
             hipLaunchKernelGGL(k_diff_bwd_interior<C>, dim3(blocks), dim3(256), 0, s, d);
-            hipLaunchKernelGGL(k_diff_bwd_edges<C>, dim3(groups), dim3(256), 0, s, d);
+            if (fork) {
+                hipError_t e = hipEventRecord(side.join, side.s);
+                if (e == hipSuccess)
+                    e = hipStreamWaitEvent(s, side.join, 0);
+                if (e != hipSuccess)
+                    return e;
+            }
         } else {
             hipLaunchKernelGGL(k_diff_backward<C>, dim3(blocks), dim3(256), 0, s, d);
         }
@@ -1538,12 +1581,15 @@ static int diff_run(swr_context *c, const swr_diff_desc *dd, const float *gimage
         d.pairs = std::memcmp(held, tag, sizeof tag) == 0 ? 2 : 0;
     }
     hipStream_t s = context_stream(c);
+    SideStream side;
+    if (!context_side(c, &side.s, &side.fork, &side.join))
+        side = SideStream();
     hipError_t e;
     switch (q.channels) {
-    case 1: e = launch_diff<1>(d, which, s); break;
-    case 2: e = launch_diff<2>(d, which, s); break;
-    case 3: e = launch_diff<3>(d, which, s); break;
-    default: e = launch_diff<4>(d, which, s); break;
+    case 1: e = launch_diff<1>(d, which, s, side); break;
+    case 2: e = launch_diff<2>(d, which, s, side); break;
+    case 3: e = launch_diff<3>(d, which, s, side); break;
+    default: e = launch_diff<4>(d, which, s, side); break;
     }
     if (e != hipSuccess)
         return context_fail(c, SWR_ERR_HIP, hipGetErrorString(e));

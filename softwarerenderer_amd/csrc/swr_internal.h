@@ -64,6 +64,8 @@ uint32_t pair_sentinel(uint32_t tiles);
 
 // context access for the kernel files' own entry points (swr_api.cpp)
 hipStream_t context_stream(swr_context *c);
+// The context's second stream with a fork and a join event (false: none, one stream only).
+bool context_side(swr_context *c, hipStream_t *side, hipEvent_t *fork, hipEvent_t *join);
 int context_fail(swr_context *c, int code, const char *msg);
 // The status code of the context's last failure (SWR_ERR_HIP if none was recorded).
 int context_last_code(swr_context *c);
