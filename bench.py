@@ -410,6 +410,17 @@ def terrain_bench(args, dev):
                     "HIP graph replay"}
 
 
+def tuning_summary(tu: dict) -> dict:
+    """The autotuner's pick, and the best time of each raster work distribution: one
+    workgroup per bin (hardware dispatch) and persistent workgroups on the bin queues."""
+    def best(queue: bool):
+        ts = [t for c, t in tu["table"] if (c.get("raster_queue", 0) > 0) == queue]
+        return round(min(ts), 4) if ts else None
+    return {"chosen": tu["best"], "tuned_ms": round(tu["ms"], 4),
+            "default_ms": round(tu["default_ms"], 4), "configs": len(tu["table"]),
+            "best_dispatch_ms": best(False), "best_queue_ms": best(True)}
+
+
 def batch_bench(args, dev):
     """The many-scene workload (256 UV spheres, 1k..200k triangles, 256x256..1920x1080,
     Span / Block, depth-checker program): one batched pass (swr_batch_draw, enqueued
@@ -472,8 +483,7 @@ def batch_bench(args, dev):
     tuning = None
     if not args.no_tune:
         tu = autotune(r, step_batch, reps=3, warmup=1, rounds=1)
-        tuning = {"chosen": tu["best"], "tuned_ms": round(tu["ms"], 4),
-                  "default_ms": round(tu["default_ms"], 4), "configs": len(tu["table"])}
+        tuning = tuning_summary(tu)
     reps = 5
     s = torch.cuda.current_stream(dev)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -562,8 +572,7 @@ def main():
     if not args.no_tune:
         from softwarerenderer_amd.autotune import autotune
         tu = autotune(r, lambda: step(False))
-        tuning = {"chosen": tu["best"], "tuned_ms": round(tu["ms"], 4),
-                  "default_ms": round(tu["default_ms"], 4), "configs": len(tu["table"])}
+        tuning = tuning_summary(tu)
     for _ in range(max(1, args.warmup)):
         step(True)
     st = r.stats()

@@ -378,7 +378,10 @@ typedef struct swr_tuning {
     int32_t bin_size;        /* screen bins of 8, 16 or 32 pixels (8) */
     int32_t waves_per_bin;   /* bins of 16 or 32 pixels: waves sharing a bin's 8x8 blocks:
                                 1, 2, 4 (4) */
-    int32_t reserved;
+    int32_t raster_queue;    /* 1: persistent raster workgroups (as many as the GPU holds)
+                                whose waves take bins from per-XCD queues, stealing from the
+                                other XCDs' when their own is empty; -1: one workgroup per bin
+                                (default) */
 } swr_tuning;
 int  swr_set_tuning(swr_context *ctx, const swr_tuning *t);
 int  swr_get_tuning(swr_context *ctx, swr_tuning *t);

@@ -88,7 +88,7 @@ class Tuning(C.Structure):
     _fields_ = [("raster_waves", C.c_int32), ("raster_group", C.c_int32),
                 ("xcd_order", C.c_int32), ("expand_chunk", C.c_int32),
                 ("vertex_pass", C.c_int32), ("bin_size", C.c_int32),
-                ("waves_per_bin", C.c_int32), ("reserved", C.c_int32)]
+                ("waves_per_bin", C.c_int32), ("raster_queue", C.c_int32)]
 
 
 # name -> (restype, argtypes): every entry point include/swr_hip.h declares

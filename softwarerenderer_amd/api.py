@@ -396,10 +396,10 @@ class Rasterizer:
 
     def setTuning(self, raster_waves: int = 0, raster_group: int = 0, xcd_order: int = 0,
                   expand_chunk: int = 0, vertex_pass: int = 0, bin_size: int = 0,
-                  waves_per_bin: int = 0):
+                  waves_per_bin: int = 0, raster_queue: int = 0):
         """Launch shapes of the draw path (0 = built-in default); results never change."""
         t = _abi.Tuning(raster_waves, raster_group, xcd_order, expand_chunk, vertex_pass,
-                        bin_size, waves_per_bin, 0)
+                        bin_size, waves_per_bin, raster_queue)
         self._check(self._lib.swr_set_tuning(self._ctx, C.byref(t)), "setTuning")
 
     def tuning(self) -> dict:
@@ -408,7 +408,7 @@ class Rasterizer:
         return {"raster_waves": t.raster_waves, "raster_group": t.raster_group,
                 "xcd_order": t.xcd_order, "expand_chunk": t.expand_chunk,
                 "vertex_pass": t.vertex_pass, "bin_size": t.bin_size,
-                "waves_per_bin": t.waves_per_bin}
+                "waves_per_bin": t.waves_per_bin, "raster_queue": t.raster_queue}
 
     # drawing through the IRasterizer seam (Rasterizer.h:99-141)
     def _list(self, fn, what, vertices, indices):

@@ -13,7 +13,8 @@ stream to torch's current stream for the duration, so its HIP events bracket the
 The default search runs in two stages.  First the binning shape: screen bins of 8 pixels with
 1, 2, 4 or 8 bins (one wave each) per raster workgroup, or bins of 16 or 32 pixels with 1, 2
 or 4 waves sharing a bin's 8x8 blocks, each with and without the vertex pass.  Then, for the
-fastest shape, primitives per row-walk group, XCD-grouped bin order and the expansion chunk.
+fastest shape, primitives per row-walk group, XCD-grouped bin order, the expansion chunk and
+persistent raster workgroups fed by per-XCD bin queues.
 """
 from __future__ import annotations
 
@@ -23,18 +24,20 @@ from typing import Callable, Iterable, Optional
 import torch
 
 KNOBS = ("raster_waves", "raster_group", "xcd_order", "expand_chunk", "vertex_pass",
-         "bin_size", "waves_per_bin")
+         "bin_size", "waves_per_bin", "raster_queue")
 
 # stage 1: (bin size, raster workgroup shape) x vertex pass
 SHAPES = ([dict(bin_size=8, raster_waves=w) for w in (1, 2, 4, 8)] +
           [dict(bin_size=b, waves_per_bin=w) for b in (16, 32) for w in (1, 2, 4)])
 VERTEX_PASS = (1, -1)
 # stage 2, for the fastest shape: primitives per raster row-walk group (0: the pixel
-# program's default), XCD-grouped bin order on/off, tile pairs per expansion workgroup
+# program's default), XCD-grouped bin order on/off, tile pairs per expansion workgroup,
+# persistent raster workgroups fed by per-XCD bin queues on/off
 REFINE = {
     "raster_group": (0, 1, 2, 4),
     "xcd_order": (1, -1),
     "expand_chunk": (512, 1024, 2048),
+    "raster_queue": (-1, 1),
 }
 
 

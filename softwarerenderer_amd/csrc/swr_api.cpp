@@ -432,7 +432,7 @@ FixedSizes fixed_sizes(size_t T, uint64_t ngrp, uint32_t tiles)
     f.live = T * 4;
     const size_t blocks = (T + 255) / 256, per_region = ((blocks + 63) / 64) * 256;
     f.clipq = 64 * per_region * 4;
-    f.clipq_cnt = 64 * 16 * 4;
+    f.clipq_cnt = DRAW_COUNTER_WORDS * 4;
     f.grp = ngrp * 8;
     f.scan_tmp = (scan_tmp_elems(T) + scan_tmp_elems(ngrp)) * 8;
     f.ranges = (size_t)tiles * 8;
@@ -859,6 +859,7 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
             r.nscenes = g.nscenes;
         }
         r.xcd = c->tune.xcd_order >= 0 ? 1 : 0;
+        r.queue = c->tune.raster_queue > 0 ? A.clipq_cnt + CLIPQ_COUNTERS : nullptr;
         r.fold = sync ? 0 : 1;
         if (ps.program == SWR_PS_TEXTURE)
             r.tex = (const TexDev *)static_cast<const swr_texture *>(ps.texture)->desc.p;
@@ -1705,7 +1706,8 @@ int swr_set_tuning(swr_context *c, const swr_tuning *t)
     if (!pow2_in(t->raster_waves, 1, 8) || !pow2_in(t->raster_group, 1, 8) ||
         !pow2_in(t->expand_chunk, 256, 4096) || t->xcd_order < -1 || t->xcd_order > 1 ||
         t->vertex_pass < -1 || t->vertex_pass > 1 || !pow2_in(t->bin_size, 8, 32) ||
-        !pow2_in(t->waves_per_bin, 1, 4) || t->reserved != 0)
+        !pow2_in(t->waves_per_bin, 1, 4) || t->raster_queue < -1 ||
+        t->raster_queue > 1)
         return fail(c, SWR_ERR_INVALID_ARG, "tuning value out of range");
     c->tune = *t;
     return SWR_OK;

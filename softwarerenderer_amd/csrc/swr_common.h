@@ -58,6 +58,14 @@ constexpr int ST_STICKY = 7;       // failures of asynchronous draws not yet rep
                                    // a draw, only by the host once it has reported them
 constexpr int ST_WORDS = 16;       // a draw clears every word but ST_STICKY
 
+// Counters a triangle draw's begin kernel clears, in one small buffer: the clip queue's 64
+// regions (16 words apart), then the persistent raster's RASTER_QUEUES bin queues (32 words,
+// 128 bytes, apart: one cache line each), RASTER_QUEUES / 8 per XCD.
+constexpr uint32_t CLIPQ_COUNTERS = 64 * 16;
+constexpr uint32_t RASTER_QUEUES = 64;
+constexpr uint32_t RASTER_QUEUE_STRIDE = 32;
+constexpr uint32_t DRAW_COUNTER_WORDS = CLIPQ_COUNTERS + RASTER_QUEUES * RASTER_QUEUE_STRIDE;
+
 // record word 6: index of the primitive's first (tile, primitive) pair
 constexpr int REC_PAIRBASE = 6;
 // record word 7: the input triangle the primitive comes from (scene-local index)
@@ -263,7 +271,7 @@ struct GeomArgs {
     uint32_t *prim_pb;          // [prim_cap] index of each primitive's first pair
     uint32_t *chunk_kept;       // [pair_cap / expansion chunk] kept pairs per chunk (or null)
     uint32_t *clipq;            // [tris + 64*256] triangles that straddle a clip plane
-    uint32_t *clipq_cnt;        // [64 * 16] per-region queue counters
+    uint32_t *clipq_cnt;        // [DRAW_COUNTER_WORDS]: clip queue regions, raster queues
     // post-transform vertices (the vertex pass; vwin == nullptr: off): window-space
     // (x, y, z, clip w) and the clip mask of each vertex [0, nverts) of attribute 0
     float4 *vwin;
@@ -310,6 +318,8 @@ struct RasterArgs {
     const TexDev *tex;          // SWR_PS_TEXTURE
     int fold;                   // 1: an asynchronous draw; its failures go to ST_STICKY
     const void *user;           // SWR_PS_MODULE: the program's uniforms (device memory)
+    uint32_t *queue;            // persistent raster: RASTER_QUEUES bin-queue heads, zeroed by the
+                                // draw's begin kernel (nullptr: one workgroup per bin)
 };
 
 // An asynchronous draw's failures, kept for the next swr_synchronize: once every earlier

@@ -662,6 +662,7 @@ __device__ __forceinline__ bool one_plane_case(const Pos4 *P, int cm, float *dp)
 // region b % CLIPQ_REGIONS) so that enqueueing does not serialise on one address.
 constexpr uint32_t CLIPQ_REGIONS = 64;
 constexpr uint32_t CLIPQ_STRIDE = 16;  // u32 words between counters
+static_assert(CLIPQ_REGIONS * CLIPQ_STRIDE == CLIPQ_COUNTERS, "the counter buffer layout");
 
 __device__ inline uint32_t clipq_region_cap(uint32_t tris)
 {
@@ -952,6 +953,8 @@ __global__ void __launch_bounds__(256) k_begin(GeomArgs a)
         a.status[i] = 0;
     if (i < CLIPQ_REGIONS)
         a.clipq_cnt[i * CLIPQ_STRIDE] = 0;
+    if (i < RASTER_QUEUES)
+        a.clipq_cnt[CLIPQ_COUNTERS + i * RASTER_QUEUE_STRIDE] = 0;
     for (uint32_t k = i; k < 2 * a.tiles; k += gridDim.x * blockDim.x)
         a.ranges[k] = 0;
     // the last batch's unused (batch, phase, j) slots count nothing

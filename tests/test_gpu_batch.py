@@ -101,8 +101,10 @@ def test_batch_matches_single_draws_and_oracle(gpu, cpu_oracle):
 
 
 @pytest.mark.parametrize("tune", [dict(bin_size=16, waves_per_bin=2),
-                                  dict(bin_size=32, waves_per_bin=4, raster_group=2)],
-                         ids=["bin16", "bin32"])
+                                  dict(bin_size=32, waves_per_bin=4, raster_group=2),
+                                  dict(bin_size=16, waves_per_bin=4, raster_queue=1),
+                                  dict(raster_waves=4, raster_queue=1)],
+                         ids=["bin16", "bin32", "bin16-queue", "bin8-queue"])
 @pytest.mark.parametrize("ps", [O.PS_HASH, O.PS_DEPTH_CHECKER])
 def test_batch_larger_bins(gpu, cpu_oracle, tune, ps):
     """Every scene of a mixed batch with 16x16 and 32x32 screen bins (each scene's bins

@@ -18,9 +18,13 @@ TUNINGS = [
          waves_per_bin=4),
     dict(bin_size=16, waves_per_bin=4, raster_group=2, vertex_pass=1),
     dict(bin_size=32, waves_per_bin=2, expand_chunk=256),
+    # persistent raster workgroups taking bins from the per-XCD queues
+    dict(raster_waves=2, raster_queue=1),
+    dict(bin_size=16, waves_per_bin=4, expand_chunk=512, raster_queue=1),
+    dict(bin_size=32, waves_per_bin=2, raster_group=2, raster_queue=1),
 ]
 KNOB_DEFAULTS = dict(raster_waves=0, raster_group=0, xcd_order=0, expand_chunk=0, vertex_pass=0,
-                     bin_size=0, waves_per_bin=0)
+                     bin_size=0, waves_per_bin=0, raster_queue=0)
 
 
 def _tid(t):
@@ -106,7 +110,8 @@ def test_soup_tuned(gpu, cpu_oracle):
                                  dict(raster_group=5), dict(expand_chunk=128),
                                  dict(expand_chunk=3000), dict(xcd_order=2),
                                  dict(vertex_pass=2), dict(bin_size=24), dict(bin_size=64),
-                                 dict(waves_per_bin=8), dict(waves_per_bin=3)])
+                                 dict(waves_per_bin=8), dict(waves_per_bin=3),
+                                 dict(raster_queue=2), dict(raster_queue=-2)])
 def test_invalid_tuning(gpu, bad):
     from softwarerenderer_amd import _abi, api
     r = api.Rasterizer(0)
