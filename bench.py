@@ -61,6 +61,9 @@ def parse():
                    help="skip the Span-mode workloads (sphere, the reference's Benchmark scene)")
     p.add_argument("--no-tune", action="store_true",
                    help="skip the launch-shape autotuner (built-in defaults)")
+    p.add_argument("--tuning", default="",
+                   help="a fixed launch shape for the 1M frame instead of the autotuner's, "
+                        "e.g. bin_size=16,waves_per_bin=4,expand_chunk=512")
     return p.parse_args()
 
 
@@ -89,16 +92,33 @@ def _counters(workload: str) -> dict:
     return t
 
 
-def measured_traffic(workload: str):
+# the knobs that select the raster kernel's launch and work split: counter figures measured
+# under one setting of them do not describe another
+RASTER_KNOBS = ("bin_size", "waves_per_bin", "raster_waves", "raster_group", "raster_queue")
+
+
+def raster_shape(tuning: dict) -> dict:
+    return {k: int(tuning.get(k, 0)) for k in RASTER_KNOBS}
+
+
+def _counters_for(workload: str, tuning: dict) -> dict:
+    """The recorded counters, when they were also taken with this raster launch shape."""
+    t = _counters(workload)
+    if not t or raster_shape(t.get("tuning", {})) != raster_shape(tuning):
+        return {}
+    return t
+
+
+def measured_traffic(workload: str, tuning: dict):
     """HBM bytes per raster launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled
     by the copy-probe factors recorded with them)."""
-    return _counters(workload).get("raster_bytes")
+    return _counters_for(workload, tuning).get("raster_bytes")
 
 
-def valu_issue(workload: str, raster_ms: float):
+def valu_issue(workload: str, tuning: dict, raster_ms: float):
     """The raster kernel's VALU issue rate: SQ_INSTS_VALU per launch (rocprofv3) over the
     launch time measured here, against the wave64 fp32 issue peak."""
-    t = _counters(workload)
+    t = _counters_for(workload, tuning)
     insts, peak = t.get("valu_insts_per_launch"), t.get("valu_peak_winst_per_s")
     if not insts or not peak or raster_ms <= 0:
         return None
@@ -569,10 +589,15 @@ def main():
     # the first synchronous draw sizes the arenas
     step(True)
     tuning = None
-    if not args.no_tune:
+    if args.tuning:
+        fixed = {k: int(v) for k, v in (kv.split("=") for kv in args.tuning.split(","))}
+        r.setTuning(**fixed)
+        tuning = {"chosen": r.tuning(), "fixed": True}
+    elif not args.no_tune:
         from softwarerenderer_amd.autotune import autotune
         tu = autotune(r, lambda: step(False))
         tuning = tuning_summary(tu)
+    shape = r.tuning()
     for _ in range(max(1, args.warmup)):
         step(True)
     st = r.stats()
@@ -644,10 +669,11 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_raster<DepthChecker>",
                      "achieved": round(raster_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(raster_gbs / HBM_PEAK_GBS, 5),
-                     "traffic": measured_traffic("soup-1M-1080p-block"),
+                     "traffic": measured_traffic("soup-1M-1080p-block", shape),
                      "source": source_revision(),
+                     "launch_shape": raster_shape(shape),
                      "algorithmic_bytes": raster_bytes,
-                     "valu_issue": valu_issue("soup-1M-1080p-block", raster_ms)},
+                     "valu_issue": valu_issue("soup-1M-1080p-block", shape, raster_ms)},
         "parity": parity,
         "tuning": tuning,
         "frame": {"algorithmic_bytes": frame_bytes, "kernel_ms": round(kern_ms, 4),
