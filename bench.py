@@ -67,6 +67,11 @@ def parse():
     return p.parse_args()
 
 
+def progress(msg: str) -> None:
+    """A progress line on stderr (stdout carries only the result line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def source_revision() -> str:
     """A digest of the kernel and C-ABI sources (softwarerenderer_amd/csrc): counter figures
     recorded for other sources do not describe this build."""
@@ -94,11 +99,17 @@ def _counters(workload: str) -> dict:
 
 # the knobs that select the raster kernel's launch and work split: counter figures measured
 # under one setting of them do not describe another
-RASTER_KNOBS = ("bin_size", "waves_per_bin", "raster_waves", "raster_group", "raster_queue")
-
-
 def raster_shape(tuning: dict) -> dict:
-    return {k: int(tuning.get(k, 0)) for k in RASTER_KNOBS}
+    """The raster launch a tuning selects, with the built-in defaults filled in (0 = default;
+    raster_queue -1 and 0 both mean one workgroup per bin)."""
+    bins = int(tuning.get("bin_size", 0)) or 8
+    shape = {"bin_size": bins, "raster_group": int(tuning.get("raster_group", 0)),
+             "raster_queue": 1 if int(tuning.get("raster_queue", 0)) > 0 else 0}
+    if bins == 8:
+        shape["raster_waves"] = int(tuning.get("raster_waves", 0)) or 4
+    else:
+        shape["waves_per_bin"] = min(int(tuning.get("waves_per_bin", 0)) or 4, (bins // 8) ** 2)
+    return shape
 
 
 def _counters_for(workload: str, tuning: dict) -> dict:
@@ -682,16 +693,22 @@ def main():
     }
     r.close()
 
+    progress(f"frame: {out['ms_per_step']} ms per frame")
     if not args.no_span:
         out["span"] = span_bench(args, dev)
+        progress("span legs done")
     if not args.no_diff:
         out["diff"] = diff_bench(args, dev)
+        progress(f"diff: {out['diff']['batched_ms_per_step']} ms per step")
     if not args.no_batch:
         out["batch"] = batch_bench(args, dev)
+        progress(f"batch: {out['batch']['batched_ms_per_step']} ms per step")
     if not args.no_terrain:
         out["terrain"] = terrain_bench(args, dev)
+        progress(f"terrain: {out['terrain']['batched_ms_per_step']} ms per step")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_runs)
+        progress("cpu baseline done")
     print(json.dumps(out))
 
 
@@ -748,6 +765,7 @@ def cpu_baseline(sc, runs: int):
             cpu.draw_elements(pipe, t, [(sc.vertices, sc.stride)], O.DRAW_TRIANGLE, sc.indices)
             if i:
                 times.append(time.perf_counter() - s)
+        progress(f"cpu {kind}: {statistics.median(times) * 1e3:.0f} ms per frame")
         return statistics.median(times)
 
     legs = {}

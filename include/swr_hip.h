@@ -71,9 +71,10 @@ typedef struct swr_rasterizer_vertex {
 
 /* ---- vertex programs ------------------------------------------------------------------
  * The reference binds a C++ static function (VertexProcessor.h:77-85).  Across a C ABI a
- * shader is a device program id plus its uniforms; the statics AttribCount / AVarCount /
- * PVarCount keep their meaning (VertexShaderBase.h:37-43): the clipper interpolates only
- * the first avar_count / pvar_count varyings (PolyClipper.h:42-45).
+ * shader is a device program id plus its uniforms, or a vertex module (SWR_VS_MODULE,
+ * below); the statics AttribCount / AVarCount / PVarCount keep their meaning
+ * (VertexShaderBase.h:37-43): the clipper interpolates only the first avar_count /
+ * pvar_count varyings (PolyClipper.h:42-45).
  *
  *   SWR_VS_DUMMY          writes nothing (DummyVertexShader, VertexShaderBase.h:53)
  *   SWR_VS_RAW            attrib 0 holds {x,y,z,w, avar[avar_count], pvar[pvar_count]} floats
@@ -90,6 +91,32 @@ typedef struct swr_rasterizer_vertex {
 #define SWR_VS_POS_RGB         2
 #define SWR_VS_MVP_POS_RGB_UV  3
 #define SWR_VS_MVP_POS_UV      4
+#define SWR_VS_MODULE          5
+
+/* A vertex shader compiled outside the library (include/swr/vertex_program.h): the
+ * reference's VertexShaderBase<Derived> with processVertex (VertexShaderBase.h:33-51) as a
+ * __device__ function that also receives the program's uniforms.  The module's entry point
+ * (SWR_VERTEX_MODULE) fills this struct, which swr_set_vertex_shader copies.  `shade` runs
+ * processVertex for vertices 0 .. vertex_count-1, vertex v reading attribute i at
+ * attribs[i] + strides[i] * v (VertexProcessor.cpp:134-150), and writes row v of `out`:
+ * {x, y, z, w, avar[avar_count], pvar[pvar_count]} floats.  A draw with a vertex module
+ * shades every vertex of its attribute buffers once (the buffers must be device memory with
+ * their sizes given to swr_set_vertex_attrib_pointer; the vertex count is the number of
+ * whole strides in the smallest of them) and then draws those rows as SWR_VS_RAW would:
+ * the same results as the reference, whose VertexCache shades each referenced index once
+ * (VertexProcessor.cpp:86-117; vertex programs are pure functions of their inputs).  The
+ * asynchronous draw enqueues the shading on the context stream; swr_draw_elements_noalloc
+ * and batched draws do not take vertex modules. */
+#define SWR_VERTEX_MODULE_ABI  1u
+typedef struct swr_vertex_module {
+    uint32_t abi;              /* SWR_VERTEX_MODULE_ABI                                 */
+    int32_t  attrib_count;     /* VertexShaderBase::AttribCount (1..SWR_MAX_VERTEX_ATTRIBS) */
+    int32_t  avar_count;       /* VertexShaderBase::AVarCount (0..SWR_MAX_AVARS)        */
+    int32_t  pvar_count;       /* VertexShaderBase::PVarCount (0..SWR_MAX_PVARS)        */
+    int    (*shade)(const void *const *attribs, const int64_t *strides, uint32_t vertex_count,
+                    float *out, const void *uniforms, void *stream);
+    const char *name;
+} swr_vertex_module;
 
 typedef struct swr_vertex_shader_desc {
     int32_t program;
@@ -97,6 +124,9 @@ typedef struct swr_vertex_shader_desc {
     int32_t avar_count;        /* VertexShaderBase::AVarCount   */
     int32_t pvar_count;        /* VertexShaderBase::PVarCount   */
     float   mvp[16];           /* uniform block of the MVP programs */
+    const swr_vertex_module *module;  /* SWR_VS_MODULE: the module (its counts replace the
+                                         three above)                                    */
+    const void *uniforms;      /* SWR_VS_MODULE: the program's uniforms, device memory  */
 } swr_vertex_shader_desc;
 
 /* ---- pixel programs -------------------------------------------------------------------

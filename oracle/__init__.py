@@ -40,7 +40,7 @@ PS_NULL, PS_DEPTH_CHECKER, PS_RGB, PS_COVERAGE, PS_HASH, PS_LOG, PS_VISIBILITY, 
 class VertexShaderDesc(C.Structure):
     _fields_ = [("program", C.c_int32), ("attrib_count", C.c_int32),
                 ("avar_count", C.c_int32), ("pvar_count", C.c_int32),
-                ("mvp", C.c_float * 16)]
+                ("mvp", C.c_float * 16), ("module", C.c_void_p), ("uniforms", C.c_void_p)]
 
 
 class PixelShaderDesc(C.Structure):

@@ -26,7 +26,7 @@ HASH_SEED = 2166136261
 class VertexShaderDesc(C.Structure):
     _fields_ = [("program", C.c_int32), ("attrib_count", C.c_int32),
                 ("avar_count", C.c_int32), ("pvar_count", C.c_int32),
-                ("mvp", C.c_float * 16)]
+                ("mvp", C.c_float * 16), ("module", C.c_void_p), ("uniforms", C.c_void_p)]
 
 
 class PixelShaderDesc(C.Structure):
@@ -41,6 +41,16 @@ class PixelShaderDesc(C.Structure):
 
 
 PIXEL_MODULE_ABI = 1
+
+
+VERTEX_MODULE_ABI = 1
+
+
+class VertexModule(C.Structure):
+    """swr_vertex_module: a vertex program compiled outside the library."""
+    _fields_ = [("abi", C.c_uint32), ("attrib_count", C.c_int32),
+                ("avar_count", C.c_int32), ("pvar_count", C.c_int32),
+                ("shade", C.c_void_p), ("name", C.c_char_p)]
 
 
 class PixelModule(C.Structure):

@@ -32,7 +32,7 @@ class CullMode:
 
 
 # program ids
-VS_DUMMY, VS_RAW, VS_POS_RGB, VS_MVP_POS_RGB_UV, VS_MVP_POS_UV = 0, 1, 2, 3, 4
+VS_DUMMY, VS_RAW, VS_POS_RGB, VS_MVP_POS_RGB_UV, VS_MVP_POS_UV, VS_MODULE = 0, 1, 2, 3, 4, 5
 (PS_NULL, PS_DEPTH_CHECKER, PS_RGB, PS_COVERAGE, PS_HASH, PS_LOG, PS_VISIBILITY, PS_TEXTURE,
  PS_MODULE) = range(9)
 
@@ -316,6 +316,47 @@ class PosRGBVertexShader(VertexShaderBase):
     Program = VS_POS_RGB
 
 
+class VertexModule:
+    """A vertex program compiled outside the library (include/swr/vertex_program.h): the
+    shared library at `path`, whose entry point `entry` (SWR_VERTEX_MODULE) fills the module
+    descriptor.  The library stays loaded while this object lives."""
+
+    def __init__(self, path: str, entry: str):
+        self._lib = C.CDLL(path)
+        fn = getattr(self._lib, entry)
+        fn.restype = C.c_int
+        fn.argtypes = [C.POINTER(_abi.VertexModule)]
+        self.desc = _abi.VertexModule()
+        rc = fn(C.byref(self.desc))
+        if rc != 0:
+            raise _abi.SwrError(rc, f"vertex module entry {entry} failed")
+        self.path, self.entry = path, entry
+
+    @property
+    def name(self) -> str:
+        return (self.desc.name or b"").decode()
+
+
+class ModuleVertexShader(VertexShaderBase):
+    """A vertex module bound with its uniforms (device memory the program reads, e.g. a
+    tensor holding a matrix; None for none).  The statics (AttribCount, AVarCount,
+    PVarCount) are the module's.  Its attribute buffers must be device memory."""
+
+    Program = VS_MODULE
+
+    def __init__(self, module: VertexModule, uniforms=None):
+        self.module, self.uniforms = module, uniforms
+        d = module.desc
+        self.AttribCount, self.AVarCount, self.PVarCount = (d.attrib_count, d.avar_count,
+                                                            d.pvar_count)
+
+    def describe(self) -> _abi.VertexShaderDesc:
+        d = super().describe()
+        d.module = C.cast(C.pointer(self.module.desc), C.c_void_p)
+        d.uniforms = _buf(self.uniforms)[0] if self.uniforms is not None else None
+        return d
+
+
 class RawVertexShader(VertexShaderBase):
     """Clip-space input {x,y,z,w, avar[A], pvar[P]} copied to the output."""
 
@@ -460,6 +501,7 @@ class VertexProcessor:
 
     def setVertexShader(self, shader: VertexShaderBase):
         shader = shader() if isinstance(shader, type) else shader
+        self._vshader = shader  # a module shader's uniforms stay referenced while bound
         d = shader.describe()
         self._r._check(self._r._lib.swr_set_vertex_shader(self._r._ctx, C.byref(d)),
                        "setVertexShader")

@@ -58,9 +58,10 @@ def build_library(verbose: bool = False) -> str:
 
 
 def build_pixel_module(src: str, out: str) -> str:
-    """Compile a pixel program module (a .hip file using include/swr/pixel_program.h) into
-    the shared library `out`, with the library's own flags (the raster chains must round as
-    the library's do).  Returns `out`."""
+    """Compile a program module (a .hip file using include/swr/pixel_program.h or
+    include/swr/vertex_program.h) into the shared library `out`, with the library's own flags
+    (the raster chains and vertex arithmetic must round as the library's do).  Returns
+    `out`."""
     if shutil.which(HIPCC) is None and not os.path.exists(HIPCC):
         raise RuntimeError(f"hipcc not found at {HIPCC}")
     tmp = out + ".tmp"
@@ -72,11 +73,14 @@ def build_pixel_module(src: str, out: str) -> str:
     return out
 
 
+build_vertex_module = build_pixel_module
+
 TEST_MODULES = os.path.join(ROOT, "tests", "modules")
 
 
 def build_test_modules() -> list:
-    """tests/modules/*.hip -> tests/modules/lib*.so (the pixel modules the tests load)."""
+    """tests/modules/*.hip -> tests/modules/lib*.so (the pixel and vertex modules the tests
+    load)."""
     outs = []
     if not os.path.isdir(TEST_MODULES):
         return outs

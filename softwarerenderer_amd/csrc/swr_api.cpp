@@ -58,6 +58,7 @@ struct swr_context {
     float dn = 0.0f, df = 1.0f;
     int cull = SWR_CULL_CW;
     swr_vertex_shader_desc vs{};
+    swr_vertex_module vmodule{};              // SWR_VS_MODULE: the bound module (vs.module)
     struct Attr {
         const void *ptr = nullptr;
         int stride = 0;
@@ -72,6 +73,7 @@ struct swr_context {
         clipq_cnt,  // its per-region counters
         chunk_kept, // pairs kept per expansion chunk
         vwin, vmask,  // post-transform vertices of the vertex pass
+        vmod_out,     // rows a vertex module wrote
         zscratch[2];  // scratch of the differentiation kernels
     uint64_t prim_cap = 0, pair_cap = 0;
     unsigned long long *status_host = nullptr;
@@ -1244,8 +1246,8 @@ int swr_destroy(swr_context *c)
                       &c->scan_tmp, &c->prims, &c->pk_a, &c->pv_a, &c->pk_b, &c->pv_b,
                       &c->ranges, &c->hist, &c->hist_ex, &c->hist_tmp, &c->status, &c->tgt[0],
                       &c->tgt[1], &c->tgt[2], &c->log, &c->logcount, &c->bigq, &c->clipq,
-                      &c->clipq_cnt, &c->chunk_kept, &c->vwin, &c->vmask, &c->zscratch[0], &c->zscratch[1],
-                      &c->diff_table};
+                      &c->clipq_cnt, &c->chunk_kept, &c->vwin, &c->vmask, &c->vmod_out,
+                      &c->zscratch[0], &c->zscratch[1], &c->diff_table};
     for (DevBuf *b : bufs)
         if (b->p)
             (void)hipFree(b->p);
@@ -1394,8 +1396,27 @@ int swr_set_vertex_shader(swr_context *c, const swr_vertex_shader_desc *d)
 {
     if (!c || !d)
         return fail(c, SWR_ERR_INVALID_ARG, "null vertex shader");
-    if (d->program < SWR_VS_DUMMY || d->program > SWR_VS_MVP_POS_UV)
+    if (d->program < SWR_VS_DUMMY || d->program > SWR_VS_MODULE)
         return fail(c, SWR_ERR_UNSUPPORTED, "vertex program %d is not built", d->program);
+    if (d->program == SWR_VS_MODULE) {
+        const swr_vertex_module *m = d->module;
+        if (!m || m->abi != SWR_VERTEX_MODULE_ABI || !m->shade)
+            return fail(c, SWR_ERR_INVALID_ARG,
+                        "vertex module missing or built for another library version");
+        if (m->attrib_count < 1 || m->attrib_count > SWR_MAX_VERTEX_ATTRIBS ||
+            m->avar_count < 0 || m->avar_count > SWR_MAX_AVARS || m->pvar_count < 0 ||
+            m->pvar_count > SWR_MAX_PVARS)
+            return fail(c, SWR_ERR_INVALID_ARG, "vertex module counts out of range");
+        if (d->uniforms && !is_device_ptr(d->uniforms))
+            return fail(c, SWR_ERR_INVALID_ARG, "vertex module uniforms must be device memory");
+        c->vmodule = *m;
+        c->vs = *d;
+        c->vs.module = &c->vmodule;
+        c->vs.attrib_count = m->attrib_count;
+        c->vs.avar_count = m->avar_count;
+        c->vs.pvar_count = m->pvar_count;
+        return SWR_OK;
+    }
     if (d->attrib_count < 0 || d->attrib_count > SWR_MAX_VERTEX_ATTRIBS || d->avar_count < 0 ||
         d->avar_count > SWR_MAX_AVARS || d->pvar_count < 0 || d->pvar_count > SWR_MAX_PVARS)
         return fail(c, SWR_ERR_INVALID_ARG, "vertex shader counts out of range");
@@ -1414,6 +1435,62 @@ int swr_set_vertex_attrib_pointer(swr_context *c, int index, int stride, const v
     return SWR_OK;
 }
 
+// A draw whose vertex program is a module: every vertex of the attribute buffers is shaded
+// once into rows {x, y, z, w, avar, pvar} (VertexProcessor.cpp:134-150 per vertex; the
+// reference's VertexCache shades each referenced index once, and vertex programs are pure),
+// then the draw reads those rows as SWR_VS_RAW.  The context's vertex state is restored
+// when the guard goes out of scope.
+struct VertexModuleDraw {
+    swr_context *c = nullptr;
+    swr_vertex_shader_desc vs{};
+    swr_context::Attr a0{};
+    ~VertexModuleDraw()
+    {
+        if (c) {
+            c->vs = vs;
+            c->attr[0] = a0;
+        }
+    }
+};
+
+int shade_vertex_module(swr_context *c, VertexModuleDraw &g)
+{
+    const swr_vertex_module &m = c->vmodule;
+    uint64_t nv = ~0ull;
+    const void *ptrs[SWR_MAX_VERTEX_ATTRIBS] = {};
+    int64_t strides[SWR_MAX_VERTEX_ATTRIBS] = {};
+    for (int i = 0; i < m.attrib_count; ++i) {
+        const swr_context::Attr &A = c->attr[i];
+        if (!A.ptr || !is_device_ptr(A.ptr) || !A.bytes || A.stride <= 0)
+            return fail(c, SWR_ERR_INVALID_ARG,
+                        "vertex module %s: attribute %d must be device memory with its size "
+                        "and a positive stride",
+                        m.name ? m.name : "", i);
+        nv = std::min<uint64_t>(nv, A.bytes / (uint64_t)A.stride);
+        ptrs[i] = A.ptr;
+        strides[i] = A.stride;
+    }
+    if (nv >= (1ull << 31))
+        return fail(c, SWR_ERR_CAPACITY, "vertex module over %llu vertices",
+                    (unsigned long long)nv);
+    const uint64_t row = 4ull * (4 + m.avar_count + m.pvar_count);
+    HIP_TRY(c, ensure(c, c->vmod_out, std::max<uint64_t>(nv, 1) * row));
+    const int rc = m.shade(ptrs, strides, (uint32_t)nv, static_cast<float *>(c->vmod_out.p),
+                           c->vs.uniforms, c->stream);
+    if (rc != (int)hipSuccess)
+        return fail(c, SWR_ERR_HIP, "vertex module %s: %s", m.name ? m.name : "",
+                    hipGetErrorString((hipError_t)rc));
+    g.c = c;
+    g.vs = c->vs;
+    g.a0 = c->attr[0];
+    c->vs.program = SWR_VS_RAW;
+    c->vs.attrib_count = 1;
+    c->attr[0].ptr = c->vmod_out.p;
+    c->attr[0].stride = (int)row;
+    c->attr[0].bytes = (size_t)(nv * row);
+    return SWR_OK;
+}
+
 int swr_draw_elements(swr_context *c, int mode, size_t count, const int32_t *indices)
 {
     if (!c || (count && !indices))
@@ -1421,6 +1498,12 @@ int swr_draw_elements(swr_context *c, int mode, size_t count, const int32_t *ind
     if (mode != SWR_DRAW_TRIANGLE && mode != SWR_DRAW_POINT && mode != SWR_DRAW_LINE)
         return fail(c, SWR_ERR_INVALID_ARG, "bad draw mode %d", mode);
     (void)hipSetDevice(c->device);
+    VertexModuleDraw vmd;
+    if (c->vs.program == SWR_VS_MODULE) {
+        const int rc = shade_vertex_module(c, vmd);
+        if (rc)
+            return rc;
+    }
     if (mode != SWR_DRAW_TRIANGLE)
         return draw_lines_points(c, mode, indices, count, false, nullptr, 0);
     return draw_triangles(c, indices, count, false, nullptr, 0, true);
@@ -1433,6 +1516,12 @@ int swr_draw_elements_async(swr_context *c, int mode, size_t count, const int32_
     if (mode != SWR_DRAW_TRIANGLE)
         return fail(c, SWR_ERR_UNSUPPORTED, "async draws take triangles");
     (void)hipSetDevice(c->device);
+    VertexModuleDraw vmd;
+    if (c->vs.program == SWR_VS_MODULE) {
+        const int rc = shade_vertex_module(c, vmd);
+        if (rc)
+            return rc;
+    }
     return draw_triangles(c, indices, count, false, nullptr, 0, false);
 }
 
@@ -1442,6 +1531,10 @@ int swr_draw_elements_noalloc(swr_context *c, int mode, size_t count, const int3
         return fail(c, SWR_ERR_INVALID_ARG, "null indices");
     if (mode != SWR_DRAW_TRIANGLE)
         return fail(c, SWR_ERR_UNSUPPORTED, "the no-allocation draw takes triangles");
+    if (c->vs.program == SWR_VS_MODULE)
+        return fail(c, SWR_ERR_UNSUPPORTED,
+                    "the no-allocation draw does not take vertex modules (their output rows "
+                    "need device memory)");
     (void)hipSetDevice(c->device);
     return draw_triangles(c, indices, count, false, nullptr, 0, false, nullptr, true);
 }
@@ -1592,6 +1685,8 @@ int swr_batch_draw(swr_context *c, const swr_batch *b, int async)
 {
     if (!c || !b || b->ctx != c)
         return fail(c, SWR_ERR_INVALID_ARG, "batch does not belong to this context");
+    if (c->vs.program == SWR_VS_MODULE)
+        return fail(c, SWR_ERR_UNSUPPORTED, "batched draws do not take vertex modules");
     (void)hipSetDevice(c->device);
     return draw_triangles(c, nullptr, 0, false, nullptr, 0, async == 0, b);
 }

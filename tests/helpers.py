@@ -24,10 +24,12 @@ def run_cpu(renderer, pipe: O.Pipeline, scene=None, mode=O.DRAW_TRIANGLE, indice
 
 def run_gpu(pipe: O.Pipeline, scene=None, mode=O.DRAW_TRIANGLE, indices=None, vertices=None,
             prim="triangle", log_capacity=0, device_buffers=True, rasterizer=None,
-            pixel_shader=None):
+            pixel_shader=None, vertex_shader=None, attribs=None, draw_async=False):
     """Draw through the C ABI (softwarerenderer_amd.api); returns numpy arrays shaped like
     oracle.Targets (t0, t1, log records, log_count).  With `pixel_shader` that shader draws
-    instead of the pipeline's program (the caller reads its own buffers)."""
+    instead of the pipeline's program (the caller reads its own buffers); with
+    `vertex_shader` that shader replaces the pipeline's vertex program, reading the
+    attribute buffers `attribs` ([(stride, array), ...]; default the scene's vertices)."""
     import torch
     from softwarerenderer_amd import api
 
@@ -73,14 +75,24 @@ def run_gpu(pipe: O.Pipeline, scene=None, mode=O.DRAW_TRIANGLE, indices=None, ve
         vp.setViewport(*pipe.viewport)
         vp.setDepthRange(*pipe.depth_range)
         vp.setCullMode(pipe.cull_mode)
-        vs = api.VertexShaderBase()
-        vs.Program = pipe.vs_program
-        vs.AttribCount, vs.AVarCount, vs.PVarCount = pipe.vs_counts
-        vs.mvp = pipe.mvp
-        vp.setVertexShader(vs)
-        vp.setVertexAttribPointer(0, scene.stride, torch.from_numpy(scene.vertices).to(dev))
+        if vertex_shader is None:
+            vs = api.VertexShaderBase()
+            vs.Program = pipe.vs_program
+            vs.AttribCount, vs.AVarCount, vs.PVarCount = pipe.vs_counts
+            vs.mvp = pipe.mvp
+            vp.setVertexShader(vs)
+        else:
+            vp.setVertexShader(vertex_shader)
+        for i, (stride, arr) in enumerate(attribs or [(scene.stride, scene.vertices)]):
+            vp.setVertexAttribPointer(i, stride, torch.from_numpy(np.ascontiguousarray(arr))
+                                      .to(dev))
         idx = scene.indices if indices is None else indices
-        vp.drawElements(mode, torch.from_numpy(np.ascontiguousarray(idx, np.int32)).to(dev))
+        it = torch.from_numpy(np.ascontiguousarray(idx, np.int32)).to(dev)
+        if draw_async:
+            vp.drawElementsAsync(mode, it)
+            r.synchronize()
+        else:
+            vp.drawElements(mode, it)
     out = O.Targets(pipe.ps_program, W, H, log_capacity)
     if t0 is not None:
         out.t0 = t0.cpu().numpy().view(ref.t0.dtype).copy()

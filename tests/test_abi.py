@@ -49,6 +49,7 @@ def _c_layout(code: str) -> dict:
                                       ("swr_scene", _abi.Scene),
                                       ("swr_tuning", _abi.Tuning),
                                       ("swr_pixel_module", _abi.PixelModule),
+                                      ("swr_vertex_module", _abi.VertexModule),
                                       ("swr_pixel_shader_desc", O.PixelShaderDesc),
                                       ("swr_vertex_shader_desc", O.VertexShaderDesc),
                                       ("swr_pipeline_state", O.PipelineState)])

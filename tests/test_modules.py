@@ -1,6 +1,6 @@
-"""Pixel modules built outside the library load and describe themselves without a GPU: the
-entry point fills the module descriptor (ABI version, the raster argument block's size and
-the program's statics) and the library refuses a descriptor from another build."""
+"""Pixel and vertex modules built outside the library load and describe themselves without a
+GPU: the entry point fills the module descriptor (ABI version, the raster argument block's
+size and the program's statics)."""
 import ctypes as C
 import os
 
@@ -43,3 +43,23 @@ def test_module_entry_rejects_null():
     fn.restype = C.c_int
     fn.argtypes = [C.c_void_p]
     assert fn(None) == -1
+
+
+@pytest.mark.parametrize("entry,statics", [
+    ("swr_vmodule_mvp_interleaved", (1, 3, 2)),
+    ("swr_vmodule_mvp_split", (2, 3, 2)),
+])
+def test_vertex_module_descriptor(entry, statics):
+    path = os.path.join(MODULES, "libvertex_mvp.so")
+    if not os.path.exists(path):
+        pytest.skip("libvertex_mvp.so is not built")
+    m = api.VertexModule(path, entry)
+    d = m.desc
+    assert d.abi == _abi.VERTEX_MODULE_ABI
+    assert (d.attrib_count, d.avar_count, d.pvar_count) == statics
+    assert d.shade
+    assert m.name in ("MvpInterleaved", "MvpSplit")
+    vs = api.ModuleVertexShader(m)
+    assert (vs.AttribCount, vs.AVarCount, vs.PVarCount) == statics
+    desc = vs.describe()
+    assert desc.program == api.VS_MODULE and desc.module and not desc.uniforms
