@@ -141,21 +141,22 @@ __device__ inline int classify_block(const float *e, uint32_t flags, int x, int 
 // (PixelShaderBase.h:55-94) can pass all three edge tests.  The chained value of pixel
 // (i, j) is init = (a*xf + b*yf) + c followed by j additions of b and i of a: at most 18
 // roundings, each at most 2^-24 of a partial sum bounded by M = |a xf| + |b yf| + |c| +
-// 7|a| + 7|b|.  A block is dropped only when an edge's exact maximum over the block (a
-// corner: the function is linear) is below -2^-19 M, more than twice that error bound.
-// ext = 7 is one 8x8 block; a larger ext tests a square of (ext + 1)^2 pixels (a bin of
-// blocks: the bound then covers every block's chain, which starts inside the square).
+// 7|a| + 7|b|, so below 2^-19.8 M in all.  The edge's maximum over the block (a corner: the
+// function is linear) is evaluated in fp32, with at most 9 roundings of terms bounded by M
+// (below 2^-20.8 M), and the block is dropped only when that evaluation is below -2^-18 M:
+// the exact maximum is then below -2^-18 M + 2^-20.8 M, more negative than any chain's
+// rounding can lift.  A coefficient or bound that is not finite keeps the block.  ext = 7 is
+// one 8x8 block; a larger ext tests a square of (ext + 1)^2 pixels (a bin of blocks: the
+// bound then covers every block's chain, which starts inside the square).
 __device__ inline bool block_may_cover(const float *e, int x, int y, int ext = 7)
 {
-    const double xf = (double)x + 0.5, yf = (double)y + 0.5, s = (double)ext;
+    const float xf = (float)x + 0.5f, yf = (float)y + 0.5f, s = (float)ext;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const double a = e[3 * k], b = e[3 * k + 1], c = e[3 * k + 2];
-        if (!__builtin_isfinite(a) || !__builtin_isfinite(b) || !__builtin_isfinite(c))
-            return true;
-        const double m = fabs(a * xf) + fabs(b * yf) + fabs(c) + s * (fabs(a) + fabs(b));
-        const double mx = a * xf + b * yf + c + fmax(0.0, s * a) + fmax(0.0, s * b);
-        if (mx < -m * 1.9073486328125e-06)  // 2^-19
+        const float a = e[3 * k], b = e[3 * k + 1], c = e[3 * k + 2];
+        const float m = fabsf(a * xf) + fabsf(b * yf) + fabsf(c) + s * (fabsf(a) + fabsf(b));
+        const float mx = a * xf + b * yf + c + fmaxf(0.0f, s * a) + fmaxf(0.0f, s * b);
+        if (mx < -m * 3.814697265625e-06f)  // 2^-18; false for NaN and infinite bounds
             return false;
     }
     return true;

@@ -402,11 +402,11 @@ __global__ void __launch_bounds__(512) k_raster(RasterArgs a0)
 }
 
 // Workgroups of `threads` threads of `kernel` that the GPU holds at once (its CUs times the
-// kernel's occupancy per CU), kept per device and workgroup size; 0 when the runtime cannot
-// tell.
-inline uint32_t resident_groups(const void *kernel, uint32_t threads)
+// kernel's occupancy per CU), kept in `known` per device and workgroup size (each kernel
+// keeps its own); 0 when the runtime cannot tell.
+inline uint32_t resident_groups(const void *kernel, uint32_t threads,
+                                std::atomic<uint32_t> (&known)[16][9])
 {
-    static std::atomic<uint32_t> known[16][9];  // [device][threads / 64]
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16 || threads / 64 > 8)
         return 0;
@@ -432,9 +432,10 @@ template <template <class> class ProgT, int NZ, int NW, int NA, int NP, int GSX,
 inline hipError_t launch_mode(const RasterArgs &a, uint32_t blocks, uint32_t w, hipStream_t s)
 {
     if (a.queue) {
+        static std::atomic<uint32_t> known[16][9];  // [device][threads / 64]
         const uint32_t res = resident_groups(
             reinterpret_cast<const void *>(&k_raster<ProgT, NZ, NW, NA, NP, GSX, MODE, true>),
-            64 * w);
+            64 * w, known);
         if (res) {
             hipLaunchKernelGGL((k_raster<ProgT, NZ, NW, NA, NP, GSX, MODE, true>),
                                dim3(blocks < res ? blocks : res), dim3(64 * w), 0, s, a);
