@@ -106,7 +106,8 @@ typedef struct swr_rasterizer_vertex {
  * the same results as the reference, whose VertexCache shades each referenced index once
  * (VertexProcessor.cpp:86-117; vertex programs are pure functions of their inputs).  The
  * asynchronous draw enqueues the shading on the context stream; swr_draw_elements_noalloc
- * and batched draws do not take vertex modules. */
+ * and batched draws do not take vertex modules.  The module's shared library must stay
+ * loaded while the shader is bound; uniforms may be null for a program that reads none. */
 #define SWR_VERTEX_MODULE_ABI  1u
 typedef struct swr_vertex_module {
     uint32_t abi;              /* SWR_VERTEX_MODULE_ABI                                 */
@@ -188,7 +189,8 @@ typedef struct swr_vertex_shader_desc {
  * primitives in draw order).  include/swr/pixel_program.h builds it from such a class; the
  * module's entry point (SWR_PIXEL_MODULE) fills this struct, which swr_set_pixel_shader
  * copies.  The two launchers enqueue the raster stage of a triangle draw and of a point or
- * line draw; `args` is the library's raster argument block, of args_bytes bytes. */
+ * line draw; `args` is the library's raster argument block, of args_bytes bytes.  The
+ * module's shared library must stay loaded while the shader is bound. */
 #define SWR_PIXEL_MODULE_ABI   1u
 typedef struct swr_pixel_module {
     uint32_t abi;              /* SWR_PIXEL_MODULE_ABI                                  */
