@@ -24,6 +24,17 @@
 
 namespace swr {
 
+// 1 / x approximately (the hardware reciprocal on the GPU, a rounded division on the host);
+// chain_add corrects what the estimate gets wrong.
+__host__ __device__ inline double approx_rcp(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcp(x);
+#else
+    return 1.0 / x;
+#endif
+}
+
 // fp32 value after n sequential additions of a: `for (k = 0; k < n; ++k) v = v + a;`
 // (round to nearest even, subnormals kept, no contraction).
 __host__ __device__ inline float chain_add(float v, float a, int n)
@@ -51,7 +62,8 @@ __host__ __device__ inline float chain_add(float v, float a, int n)
         const double lo = __builtin_bit_cast(double, (uint64_t)(e + 1023) << 52);  // 2^e
         const double hi = 2.0 * lo;
         const double u = __builtin_bit_cast(double, (uint64_t)(e - 23 + 1023) << 52);
-        const double q = (double)a / u;  // exact: a power-of-two scaling
+        const double u_inv = __builtin_bit_cast(double, (uint64_t)(23 - e + 1023) << 52);
+        const double q = (double)a * u_inv;  // exact: a power-of-two scaling (a / u)
         const double fq = __builtin_floor(q);
         const bool tie = q - fq == 0.5;
         if (tie && !even) {  // one addition inside the binade makes v / u even
@@ -79,7 +91,9 @@ __host__ __device__ inline float chain_add(float v, float a, int n)
         // the largest k with every |v| + j*d (1 <= j <= k) inside [lo + u, hi - u]
         double room = d > 0.0 ? (hi - u) - m0 : m0 - (lo + u);
         const double step = d > 0.0 ? d : -d;
-        double k = room < 0.0 ? 0.0 : __builtin_floor(room / step);
+        // floor(room / step) is at most 2^23 (room < 2^e, step >= u): estimated from a
+        // reciprocal, then made exact by the two loops
+        double k = room < 0.0 ? 0.0 : __builtin_floor(room * approx_rcp(step));
         while (k > 0.0 && k * step > room)
             k -= 1.0;
         while ((k + 1.0) * step <= room)
