@@ -106,6 +106,7 @@ struct swr_batch {
     uint64_t tris = 0;           // input triangles
     uint64_t bins[3] = {0, 0, 0};  // sum of the scenes' bins of 8, 16 and 32 pixels
     unsigned tmask = 7;          // bit i: every scene provides target i
+    bool spans = false;          // some scene draws in Span or Adaptive mode
 };
 
 namespace {
@@ -861,6 +862,8 @@ int draw_triangles(swr_context *c, const int32_t *indices, size_t count, bool wi
             r.nscenes = g.nscenes;
         }
         r.xcd = c->tune.xcd_order >= 0 ? 1 : 0;
+        // Block mode draws every triangle with Block traversal (Rasterizer.h:433-445)
+        r.spans = bt ? (bt->spans ? 1 : 0) : (c->raster_mode != SWR_RASTER_BLOCK ? 1 : 0);
         r.queue = c->tune.raster_queue > 0 ? A.clipq_cnt + CLIPQ_COUNTERS : nullptr;
         r.fold = sync ? 0 : 1;
         if (ps.program == SWR_PS_TEXTURE)
@@ -1666,6 +1669,8 @@ int swr_batch_create(swr_context *c, const swr_scene *sc, size_t n, swr_batch **
     for (int l = 0; l < 3; ++l)
         b->bins[l] = bins[l];
     b->tmask = n ? tmask : 0;
+    for (size_t i = 0; i < n; ++i)
+        b->spans = b->spans || sc[i].raster_mode != SWR_RASTER_BLOCK;
     hipError_t e = ensure(c, b->scenes, std::max<size_t>(1, n) * sizeof(SceneDev));
     if (e == hipSuccess)
         e = ensure(c, b->bmap, std::max<size_t>(1, bmap.size()) * 4);

@@ -319,6 +319,7 @@ struct RasterArgs {
     const TexDev *tex;          // SWR_PS_TEXTURE
     int fold;                   // 1: an asynchronous draw; its failures go to ST_STICKY
     const void *user;           // SWR_PS_MODULE: the program's uniforms (device memory)
+    int spans;                  // 0: every primitive is a Block primitive (raster mode Block)
     uint32_t *queue;            // persistent raster: RASTER_QUEUES bin-queue heads, zeroed by the
                                 // draw's begin kernel (nullptr: one workgroup per bin)
 };

@@ -16,6 +16,7 @@ constexpr int FR_X = 0, FR_Y = 1, FR_KEY = 2, FR_SRC = 3, FR_Z = 4, FR_W = 5, FR
 // (Rasterizer.h:160-173): w = v.w, invw = 1/v.w, pvar = v.pvar (no perspective divide).
 template <class L>
 struct VertexPixel {
+    static constexpr bool SPANS = true;  // (no primitive records to replay)
     float zv, wv;
     float av[L::NA > 0 ? L::NA : 1];
     float pv[L::NP > 0 ? L::NP : 1];

@@ -29,8 +29,10 @@ __device__ inline float clamp01(float x) { return x > 0.0f ? (x < 1.0f ? x : 1.0
 //
 // Triangle pixels: the chained planes v[] in Layout order (z, invw, avar..., pvarTemp...);
 // w = 1/invw and pvar = pvarTemp * w at the pixel (PixelData.h:84-103).
-template <class L>
+template <class L, bool SP = true>
 struct ChainPixel {
+    // false: every primitive of the draw is a Block primitive (no Span chains to replay)
+    static constexpr bool SPANS = SP;
     const float *v;
     uint32_t prim;     // the primitive's record index (its planes), for programs with PLANES set
     __device__ float z() const { return v[L::Z]; }
@@ -53,12 +55,14 @@ struct ChainPixel {
 // centre, +b per row, +a per column; Span (PixelShaderBase.h:96-112): init at the span's
 // first pixel xl of row y, +a per pixel after it.  (x, y) must be a pixel the primitive
 // covers (for Span, xl <= x).
+// SPANS = false: the record is known to be a Block primitive.
+template <bool SPANS = true>
 __device__ inline float chain_at(const float *rec, int p, int x, int y, int sc_minx, int sc_maxx)
 {
     const float pa = rec[REC_PARAM + 3 * p], pb = rec[REC_PARAM + 3 * p + 1],
                 pc = rec[REC_PARAM + 3 * p + 2];
     const uint32_t flags = __float_as_uint(rec[REC_FLAGS]);
-    if (flags & FLAG_BLOCK) {
+    if (!SPANS || (flags & FLAG_BLOCK)) {
         // y - by additions of b, then x - bx of a (0..7 each), as selects from the running
         // sums: no per-lane trip counts
         const int bx = x & ~(TILE - 1), by = y & ~(TILE - 1);
@@ -141,9 +145,9 @@ struct ProgRGB {
         uint32_t c = s.c;
         if constexpr (PX::REPLAY) {
             const float *rec = a.prims + (size_t)s.prim * (size_t)a.stride;
-            c = pack(chain_at(rec, L::A, x, y, a.sc_minx, a.sc_maxx),
-                     chain_at(rec, L::A + 1, x, y, a.sc_minx, a.sc_maxx),
-                     chain_at(rec, L::A + 2, x, y, a.sc_minx, a.sc_maxx));
+            c = pack(chain_at<PX::SPANS>(rec, L::A, x, y, a.sc_minx, a.sc_maxx),
+                     chain_at<PX::SPANS>(rec, L::A + 1, x, y, a.sc_minx, a.sc_maxx),
+                     chain_at<PX::SPANS>(rec, L::A + 2, x, y, a.sc_minx, a.sc_maxx));
         }
         static_cast<uint32_t *>(a.t0)[o] = c;
     }
@@ -193,12 +197,12 @@ struct ProgDepthChecker {
             State s = s0;
             if constexpr (PX::REPLAY) {
                 const float *rec = a.prims + (size_t)s.prim * (size_t)a.stride;
-                s.wr = chain_at(rec, L::IW, x, y, a.sc_minx, a.sc_maxx);
-                s.a0 = chain_at(rec, L::A, x, y, a.sc_minx, a.sc_maxx);
-                s.a1 = chain_at(rec, L::A + 1, x, y, a.sc_minx, a.sc_maxx);
-                s.a2 = chain_at(rec, L::A + 2, x, y, a.sc_minx, a.sc_maxx);
-                s.p0 = chain_at(rec, L::P, x, y, a.sc_minx, a.sc_maxx);
-                s.p1 = chain_at(rec, L::P + 1, x, y, a.sc_minx, a.sc_maxx);
+                s.wr = chain_at<PX::SPANS>(rec, L::IW, x, y, a.sc_minx, a.sc_maxx);
+                s.a0 = chain_at<PX::SPANS>(rec, L::A, x, y, a.sc_minx, a.sc_maxx);
+                s.a1 = chain_at<PX::SPANS>(rec, L::A + 1, x, y, a.sc_minx, a.sc_maxx);
+                s.a2 = chain_at<PX::SPANS>(rec, L::A + 2, x, y, a.sc_minx, a.sc_maxx);
+                s.p0 = chain_at<PX::SPANS>(rec, L::P, x, y, a.sc_minx, a.sc_maxx);
+                s.p1 = chain_at<PX::SPANS>(rec, L::P + 1, x, y, a.sc_minx, a.sc_maxx);
             }
             const float w = PX::w_of(s.wr);  // only the depth-passing pixels read pvar
             const float u = PX::pvar_of(s.p0, w), t = PX::pvar_of(s.p1, w);
@@ -515,9 +519,11 @@ struct ProgTexture {
         if (!s.hit)
             return;
         const float *rec = a.prims + (size_t)s.prim * (size_t)a.stride;
-        const float w = PX::w_of(chain_at(rec, L::IW, x, y, a.sc_minx, a.sc_maxx));
-        const float u = PX::pvar_of(chain_at(rec, L::P, x, y, a.sc_minx, a.sc_maxx), w);
-        const float t = PX::pvar_of(chain_at(rec, L::P + 1, x, y, a.sc_minx, a.sc_maxx), w);
+        const float w = PX::w_of(chain_at<PX::SPANS>(rec, L::IW, x, y, a.sc_minx, a.sc_maxx));
+        const float u =
+            PX::pvar_of(chain_at<PX::SPANS>(rec, L::P, x, y, a.sc_minx, a.sc_maxx), w);
+        const float t =
+            PX::pvar_of(chain_at<PX::SPANS>(rec, L::P + 1, x, y, a.sc_minx, a.sc_maxx), w);
         const float *P = rec + REC_PARAM;
         const float xf = (float)x + 0.5f, yf = (float)y + 0.5f;
         float dudx, dudy, dvdx, dvdy;

@@ -101,8 +101,10 @@ __device__ inline uint32_t take_item(uint32_t *queue, uint32_t items, uint32_t h
 // rows), in list order.
 // MODE 0: 8x8 bins; 1: 16x16 bins whose pair values carry the block classes; 2: larger bins
 // whose blocks the kernel classifies.
+// SPANS = false: every primitive of the draw is a Block primitive (raster mode Block), so the
+// kernel carries no Span code and its registers.
 template <template <class> class ProgT, int NZ, int NW, int NA, int NP, int GSX, int MODE,
-          bool QUEUE>
+          bool QUEUE, bool SPANS>
 __global__ void __launch_bounds__(512) k_raster(RasterArgs a0)
 {
     constexpr bool FILT = MODE != 0;
@@ -239,7 +241,7 @@ __global__ void __launch_bounds__(512) k_raster(RasterArgs a0)
                             pv = a.vals[i] & PAIR_PRIM_MASK;
                             const float *rec = a.prims + (size_t)pv * (size_t)a.stride;
                             const uint32_t flags = __float_as_uint(rec[REC_FLAGS]);
-                            if (flags & FLAG_BLOCK) {
+                            if (!SPANS || (flags & FLAG_BLOCK)) {
                                 // a block of the primitive's block grid (the scissor clamps the
                                 // grid, Rasterizer.h:240-250, not the triangle), then its class
                                 const uint32_t *ru = reinterpret_cast<const uint32_t *>(rec);
@@ -283,7 +285,7 @@ __global__ void __launch_bounds__(512) k_raster(RasterArgs a0)
                 const int cls = (int)(pv >> 30);
                 const uint32_t flags = act ? __float_as_uint(rec[REC_FLAGS]) : 0u;
                 const uint32_t key = act ? __float_as_uint(rec[Prog::KEY_WORD]) : 0u;
-                const bool blk = (flags & FLAG_BLOCK) != 0;
+                const bool blk = !SPANS || (flags & FLAG_BLOCK) != 0;
 
                 // Span: this row's part and [xl, xr) (Rasterizer.h:370-380, 397-407)
                 int xl = 0, xr = 0;
@@ -390,13 +392,14 @@ __global__ void __launch_bounds__(512) k_raster(RasterArgs a0)
                             Prog::PLANES ? __builtin_amdgcn_readlane(pv, s2 * 8) & PAIR_PRIM_MASK
                                          : 0u;
                         if (((m >> lane) & 1ull) && inside)
-                            Prog::shade(st, a, ChainPixel<L>{val[s2], prim2}, k2, px, py);
+                            Prog::shade(st, a, ChainPixel<L, SPANS>{val[s2], prim2}, k2, px,
+                                        py);
                     }
                 }
                 lds_sync();
             }
             if (inside && drew)
-                Prog::template store<ChainPixel<L>>(st, a, o, px, py);
+                Prog::template store<ChainPixel<L, SPANS>>(st, a, o, px, py);
         }
     }
 }
@@ -428,22 +431,24 @@ inline uint32_t resident_groups(const void *kernel, uint32_t threads,
 
 // One raster launch of `blocks` workgroups of `w` waves; with a queue, as many persistent
 // workgroups as the GPU holds at once (at most `blocks`).
-template <template <class> class ProgT, int NZ, int NW, int NA, int NP, int GSX, int MODE>
+template <template <class> class ProgT, int NZ, int NW, int NA, int NP, int GSX, int MODE,
+          bool SPANS>
 inline hipError_t launch_mode(const RasterArgs &a, uint32_t blocks, uint32_t w, hipStream_t s)
 {
     if (a.queue) {
         static std::atomic<uint32_t> known[16][9];  // [device][threads / 64]
         const uint32_t res = resident_groups(
-            reinterpret_cast<const void *>(&k_raster<ProgT, NZ, NW, NA, NP, GSX, MODE, true>),
+            reinterpret_cast<const void *>(
+                &k_raster<ProgT, NZ, NW, NA, NP, GSX, MODE, true, SPANS>),
             64 * w, known);
         if (res) {
-            hipLaunchKernelGGL((k_raster<ProgT, NZ, NW, NA, NP, GSX, MODE, true>),
+            hipLaunchKernelGGL((k_raster<ProgT, NZ, NW, NA, NP, GSX, MODE, true, SPANS>),
                                dim3(blocks < res ? blocks : res), dim3(64 * w), 0, s, a);
             return hipGetLastError();
         }
     }
-    hipLaunchKernelGGL((k_raster<ProgT, NZ, NW, NA, NP, GSX, MODE, false>), dim3(blocks),
-                       dim3(64 * w), 0, s, a);
+    hipLaunchKernelGGL((k_raster<ProgT, NZ, NW, NA, NP, GSX, MODE, false, SPANS>),
+                       dim3(blocks), dim3(64 * w), 0, s, a);
     return hipGetLastError();
 }
 
@@ -459,11 +464,19 @@ inline hipError_t launch(const RasterArgs &a, hipStream_t s, int waves)
     }
     if (!blocks)
         return hipSuccess;
-    if (a.bin_shift > 3 && a.cls4)
-        return launch_mode<ProgT, NZ, NW, NA, NP, GSX, 1>(a, blocks, w, s);
-    if (a.bin_shift > 3)
-        return launch_mode<ProgT, NZ, NW, NA, NP, GSX, 2>(a, blocks, w, s);
-    return launch_mode<ProgT, NZ, NW, NA, NP, GSX, 0>(a, blocks, w, s);
+    const int mode = a.bin_shift > 3 ? (a.cls4 ? 1 : 2) : 0;
+    if (a.spans) {
+        if (mode == 1)
+            return launch_mode<ProgT, NZ, NW, NA, NP, GSX, 1, true>(a, blocks, w, s);
+        if (mode == 2)
+            return launch_mode<ProgT, NZ, NW, NA, NP, GSX, 2, true>(a, blocks, w, s);
+        return launch_mode<ProgT, NZ, NW, NA, NP, GSX, 0, true>(a, blocks, w, s);
+    }
+    if (mode == 1)
+        return launch_mode<ProgT, NZ, NW, NA, NP, GSX, 1, false>(a, blocks, w, s);
+    if (mode == 2)
+        return launch_mode<ProgT, NZ, NW, NA, NP, GSX, 2, false>(a, blocks, w, s);
+    return launch_mode<ProgT, NZ, NW, NA, NP, GSX, 0, false>(a, blocks, w, s);
 }
 
 }  // namespace swr
