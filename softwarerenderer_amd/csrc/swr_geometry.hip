@@ -1004,27 +1004,30 @@ __global__ void __launch_bounds__(256) k_geom_emit(GeomArgs a)
 // The emit pass over the live triangles only, in draw order (the list the scan compacted).
 // In a batch of scenes a wave's triangles may come from several scenes: the wave takes them
 // one scene at a time (a waterfall over the scene of its first pending lane), so the scene's
-// state stays wave-uniform (scalar loads and registers).
+// state stays wave-uniform (scalar loads and registers).  One thread per live triangle, no
+// grid-stride loop: a loop keeps every argument it uses live in scalar registers across
+// iterations, and their spills to vector lanes cost this latency-bound kernel time.
 template <int NV, bool SCENES>
 __global__ void __launch_bounds__(256) k_geom_emit_listed(GeomArgs a)
 {
     if (a.status[ST_OVERFLOW] | a.status[ST_ERROR])
         return;
     const uint32_t n = (uint32_t)a.status[ST_LIVE];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t t = a.live[i];
-        if (!SCENES) {
-            process_triangle<NV, true, true>(a, t);
-            continue;
-        }
-        const uint32_t sid = a.batch_scene[t >> 10];
-        bool pending = true;
-        while (pending) {
-            const uint32_t s0 = __builtin_amdgcn_readfirstlane(sid);
-            if (sid == s0) {
-                process_triangle<NV, true, true>(a, t, s0);
-                pending = false;
-            }
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t t = a.live[i];
+    if (!SCENES) {
+        process_triangle<NV, true, true>(a, t);
+        return;
+    }
+    const uint32_t sid = a.batch_scene[t >> 10];
+    bool pending = true;
+    while (pending) {
+        const uint32_t s0 = __builtin_amdgcn_readfirstlane(sid);
+        if (sid == s0) {
+            process_triangle<NV, true, true>(a, t, s0);
+            pending = false;
         }
     }
 }
@@ -1346,8 +1349,8 @@ static void emit_kernels(const GeomArgs &a, hipStream_t s, bool q)
     dim3 g((a.tris + 255) / 256), blk(256), gq(clip_blocks(a.tris)), bq(CLIPQ_THREADS);
     if (q)
         hipLaunchKernelGGL(k_clip_emit<NV == 0 ? 1 : NV>, gq, bq, 0, s, a);
-    if (a.live) {  // at most 2048 workgroups walk the live list
-        const dim3 gl(g.x < 2048 ? g.x : 2048);
+    if (a.live) {  // a thread per live triangle
+        const dim3 gl(g.x);
         if (a.nscenes)
             hipLaunchKernelGGL((k_geom_emit_listed<NV, true>), gl, blk, 0, s, a);
         else
