@@ -95,6 +95,18 @@ __host__ __device__ inline float ref_max(float a, float b) { return (a < b) ? b 
 __host__ __device__ inline int imin(int a, int b) { return (b < a) ? b : a; }
 __host__ __device__ inline int imax(int a, int b) { return (a < b) ? b : a; }
 
+// Orders one wave's LDS traffic: LDS operations of a wave complete in issue order, so
+// keeping the compiler from moving them across this point and waiting for the wave's own
+// LDS traffic makes its earlier LDS writes visible to its later reads (a barrier for data
+// that only the calling wave shares).
+__device__ inline void lds_sync()
+{
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+    __builtin_amdgcn_wave_barrier();
+    __asm__ volatile("" ::: "memory");
+}
+
 // EdgeEquation::test (EdgeEquation.h:53-61): v > 0 || (v == 0 && tie), as one compare:
 // v >= 0 with the tie, v >= the smallest denormal without it (the kernels keep fp32
 // denormals, so nothing lies strictly between 0 and it; NaN fails both forms).

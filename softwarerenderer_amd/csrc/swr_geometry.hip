@@ -1008,12 +1008,8 @@ __global__ void __launch_bounds__(256) k_geom_emit(GeomArgs a)
 // grid-stride loop: a loop keeps every argument it uses live in scalar registers across
 // iterations, and their spills to vector lanes cost this latency-bound kernel time.
 template <int NV, bool SCENES>
-__global__ void __launch_bounds__(256) k_geom_emit_listed(GeomArgs a)
+__device__ __forceinline__ void emit_listed_one(const GeomArgs &a, uint32_t n, uint32_t i)
 {
-    if (a.status[ST_OVERFLOW] | a.status[ST_ERROR])
-        return;
-    const uint32_t n = (uint32_t)a.status[ST_LIVE];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
     const uint32_t t = a.live[i];
@@ -1030,6 +1026,15 @@ __global__ void __launch_bounds__(256) k_geom_emit_listed(GeomArgs a)
             pending = false;
         }
     }
+}
+
+template <int NV, bool SCENES>
+__global__ void __launch_bounds__(256) k_geom_emit_listed(GeomArgs a)
+{
+    if (a.status[ST_OVERFLOW] | a.status[ST_ERROR])
+        return;
+    emit_listed_one<NV, SCENES>(a, (uint32_t)a.status[ST_LIVE],
+                                blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---- the clip queue: a wave per queued triangle -------------------------------------------
@@ -1080,7 +1085,7 @@ __device__ inline WinV shfl_win(const WinV &w, int src)
 template <int NV, bool EMIT>
 __device__ void clip_queued(const GeomArgs &a0, uint32_t t, ClipStage<NV> &st)
 {
-    const uint32_t lane = threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
     t = __builtin_amdgcn_readfirstlane(t);
     GeomArgs a = a0;
     const uint32_t b = t >> 10, j = t & 1023u;
@@ -1150,7 +1155,7 @@ __device__ void clip_queued(const GeomArgs &a0, uint32_t t, ClipStage<NV> &st)
 #pragma unroll
         for (int k = 0; k < NV; ++k)
             vn[k] = EMIT ? __shfl(var[k], nx) : 0.0f;
-        __syncthreads();  // the previous pass has read the stage
+        lds_sync();  // the previous pass has read the stage (the stage is the wave's own)
         if (vin) {
             st.p[slot] = P;
 #pragma unroll
@@ -1175,7 +1180,7 @@ __device__ void clip_queued(const GeomArgs &a0, uint32_t t, ClipStage<NV> &st)
                     st.v[o][k] = lerped ? lerp1(var[k], vn[k], tt) : 0.0f;
             }
         }
-        __syncthreads();
+        lds_sync();
         n = m;
         if ((int)lane < n) {
             P = st.p[lane];
@@ -1233,27 +1238,26 @@ __device__ void clip_queued(const GeomArgs &a0, uint32_t t, ClipStage<NV> &st)
 }
 
 // The queued triangles, densely packed (order does not matter: each writes its own (batch,
-// phase, j) counts and draw-order positions).  Each workgroup prefix-sums the region counts,
-// then takes dense indices blockIdx.x, blockIdx.x + gridDim.x, ... and maps each to its
-// (region, slot).
+// phase, j) counts and draw-order positions).  Wave `wid` of `nwaves` prefix-sums the region
+// counts into its own s_off, then takes dense indices wid, wid + nwaves, ... and maps each
+// to its (region, slot); s_off and st are the wave's own LDS.
 template <int NV, bool EMIT>
-__device__ void run_clip_queue(const GeomArgs &a)
+__device__ void run_clip_queue(const GeomArgs &a, uint32_t wid, uint32_t nwaves,
+                               uint32_t *s_off, ClipStage<NV> &st)
 {
-    __shared__ uint32_t s_off[CLIPQ_REGIONS + 1];
-    __shared__ ClipStage<NV> st;
     static_assert(CLIPQ_REGIONS == CLIPQ_THREADS, "one region per lane");
     {
         // the 64 counters load in parallel, one per lane; an inclusive scan across the wave
-        const uint32_t lane = threadIdx.x;
+        const uint32_t lane = threadIdx.x & 63u;
         const uint32_t cnt = a.clipq_cnt[lane * CLIPQ_STRIDE];
         const uint32_t incl = wave_incl_sum(cnt, lane);
         s_off[lane] = incl - cnt;
         if (lane == 63)
             s_off[CLIPQ_REGIONS] = incl;
     }
-    __syncthreads();
+    lds_sync();
     const uint32_t n = s_off[CLIPQ_REGIONS], cap = clipq_region_cap(a.tris);
-    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    for (uint32_t i = wid; i < n; i += nwaves) {
         uint32_t lo = 0, hi = CLIPQ_REGIONS;  // last region with s_off <= i
         while (hi - lo > 1) {
             uint32_t mid = (lo + hi) >> 1;
@@ -1268,7 +1272,9 @@ __device__ void run_clip_queue(const GeomArgs &a)
 
 __global__ void __launch_bounds__(CLIPQ_THREADS) k_clip_count(GeomArgs a)
 {
-    run_clip_queue<1, false>(a);
+    __shared__ uint32_t s_off[CLIPQ_REGIONS + 1];
+    __shared__ ClipStage<1> st;
+    run_clip_queue<1, false>(a, blockIdx.x, gridDim.x, s_off, st);
 }
 
 template <int NV>
@@ -1276,7 +1282,35 @@ __global__ void __launch_bounds__(CLIPQ_THREADS) k_clip_emit(GeomArgs a)
 {
     if (a.status[ST_OVERFLOW] | a.status[ST_ERROR])
         return;
-    run_clip_queue<NV, true>(a);
+    __shared__ uint32_t s_off[CLIPQ_REGIONS + 1];
+    __shared__ ClipStage<NV> st;
+    run_clip_queue<NV, true>(a, blockIdx.x, gridDim.x, s_off, st);
+}
+
+// The emit pass over the live triangles with the clip queue's emit folded in: workgroups
+// [0, ceil(live / 256)) emit the listed triangles as k_geom_emit_listed does, the ones after
+// them (the launch adds CLIP_GROUPS for the queue) clip queued triangles a wave each.  Those
+// workgroups start as soon as the listed ones have all been dispatched, so the queue's long
+// per-triangle chains run beside the last listed workgroups instead of in a launch before
+// them.
+template <int NV, bool SCENES>
+__global__ void __launch_bounds__(256) k_geom_emit_clip(GeomArgs a)
+{
+    static_assert(NV == 3 || NV == 5, "instantiated for the classes emit_kernels joins");
+    constexpr int NQ = NV;
+    if (a.status[ST_OVERFLOW] | a.status[ST_ERROR])
+        return;
+    const uint32_t n = (uint32_t)a.status[ST_LIVE];
+    const uint32_t nb = (n + 255u) / 256u;
+    if (blockIdx.x < nb) {
+        emit_listed_one<NV, SCENES>(a, n, blockIdx.x * 256u + threadIdx.x);
+        return;
+    }
+    __shared__ uint32_t s_off[4][CLIPQ_REGIONS + 1];
+    __shared__ ClipStage<NQ> st[4];
+    const uint32_t w = threadIdx.x >> 6;
+    run_clip_queue<NQ, true>(a, (blockIdx.x - nb) * 4u + w, (gridDim.x - nb) * 4u, s_off[w],
+                             st[w]);
 }
 
 // ---- host launchers -----------------------------------------------------------------------
@@ -1347,6 +1381,18 @@ template <int NV>
 static void emit_kernels(const GeomArgs &a, hipStream_t s, bool q)
 {
     dim3 g((a.tris + 255) / 256), blk(256), gq(clip_blocks(a.tris)), bq(CLIPQ_THREADS);
+    // the listed emit and the queue's emit in one launch, for the varying classes where the
+    // joint kernel keeps the listed emit's registers (with 8 or 32 varyings it spills; without
+    // varyings it holds 14 registers more, a wave per SIMD less)
+    if (q && a.live && (NV == 3 || NV == 5)) {
+        const dim3 gc(g.x + (gq.x + 3) / 4);  // at least one queue wave per queue workgroup
+        constexpr int NJ = NV == 3 ? 3 : 5;  // only 3 and 5 reach here
+        if (a.nscenes)
+            hipLaunchKernelGGL((k_geom_emit_clip<NJ, true>), gc, blk, 0, s, a);
+        else
+            hipLaunchKernelGGL((k_geom_emit_clip<NJ, false>), gc, blk, 0, s, a);
+        return;
+    }
     if (q)
         hipLaunchKernelGGL(k_clip_emit<NV == 0 ? 1 : NV>, gq, bq, 0, s, a);
     if (a.live) {  // a thread per live triangle

@@ -48,16 +48,6 @@ __device__ inline unsigned xcd_block(unsigned b, unsigned n)
     return x * q + (x < r ? x : r) + i;
 }
 
-__device__ inline void lds_sync()
-{
-    // LDS operations of one wave complete in issue order; keep the compiler from moving
-    // them across this point and wait for the wave's own LDS traffic.
-    __asm__ volatile("" ::: "memory");
-    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
-    __builtin_amdgcn_wave_barrier();
-    __asm__ volatile("" ::: "memory");
-}
-
 // Planes phase A chains for a program: all of them, unless the program declares CHAINED
 // (it reads only the first CHAINED per fragment and chains the others itself later).
 template <class P, int N, class = void>
