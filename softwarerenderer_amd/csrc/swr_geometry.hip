@@ -1289,10 +1289,11 @@ __global__ void __launch_bounds__(CLIPQ_THREADS) k_clip_emit(GeomArgs a)
 
 // The emit pass over the live triangles with the clip queue's emit folded in: workgroups
 // [0, ceil(live / 256)) emit the listed triangles as k_geom_emit_listed does, the ones after
-// them (the launch adds CLIP_GROUPS for the queue) clip queued triangles a wave each.  Those
-// workgroups start as soon as the listed ones have all been dispatched, so the queue's long
-// per-triangle chains run beside the last listed workgroups instead of in a launch before
-// them.
+// them (the launch sizes the grid for every input triangle plus a quarter of the queue
+// kernel's workgroups, so at least that many are left) clip queued triangles a wave each.
+// Those workgroups start as soon as the listed ones have all been dispatched, so the queue's
+// long per-triangle chains run beside the last listed workgroups instead of in a launch
+// before them.  The two halves write disjoint records and read nothing the other writes.
 template <int NV, bool SCENES>
 __global__ void __launch_bounds__(256) k_geom_emit_clip(GeomArgs a)
 {
