@@ -1288,30 +1288,26 @@ __global__ void __launch_bounds__(CLIPQ_THREADS) k_clip_emit(GeomArgs a)
 }
 
 // The emit pass over the live triangles with the clip queue's emit folded in: workgroups
-// [0, ceil(live / 256)) emit the listed triangles as k_geom_emit_listed does, the ones after
-// them (the launch sizes the grid for every input triangle plus a quarter of the queue
-// kernel's workgroups, so at least that many are left) clip queued triangles a wave each.
-// Those workgroups start as soon as the listed ones have all been dispatched, so the queue's
-// long per-triangle chains run beside the last listed workgroups instead of in a launch
-// before them.  The two halves write disjoint records and read nothing the other writes.
+// [0, nq) clip queued triangles a wave each (4 nq waves take dense queue indices), the ones
+// after them emit the listed triangles as k_geom_emit_listed does.  The queue's waves are
+// dispatched first, so its long per-triangle chains of dependent loads run while the listed
+// workgroups fill the rest of the GPU, instead of in a launch of their own before them.
+// The two parts write disjoint records and read nothing the other writes.
 template <int NV, bool SCENES>
-__global__ void __launch_bounds__(256) k_geom_emit_clip(GeomArgs a)
+__global__ void __launch_bounds__(256) k_geom_emit_clip(GeomArgs a, uint32_t nq)
 {
     static_assert(NV == 3 || NV == 5, "instantiated for the classes emit_kernels joins");
-    constexpr int NQ = NV;
     if (a.status[ST_OVERFLOW] | a.status[ST_ERROR])
         return;
-    const uint32_t n = (uint32_t)a.status[ST_LIVE];
-    const uint32_t nb = (n + 255u) / 256u;
-    if (blockIdx.x < nb) {
-        emit_listed_one<NV, SCENES>(a, n, blockIdx.x * 256u + threadIdx.x);
+    if (blockIdx.x >= nq) {
+        emit_listed_one<NV, SCENES>(a, (uint32_t)a.status[ST_LIVE],
+                                    (blockIdx.x - nq) * 256u + threadIdx.x);
         return;
     }
     __shared__ uint32_t s_off[4][CLIPQ_REGIONS + 1];
-    __shared__ ClipStage<NQ> st[4];
+    __shared__ ClipStage<NV> st[4];
     const uint32_t w = threadIdx.x >> 6;
-    run_clip_queue<NQ, true>(a, (blockIdx.x - nb) * 4u + w, (gridDim.x - nb) * 4u, s_off[w],
-                             st[w]);
+    run_clip_queue<NV, true>(a, blockIdx.x * 4u + w, nq * 4u, s_off[w], st[w]);
 }
 
 // ---- host launchers -----------------------------------------------------------------------
@@ -1386,12 +1382,13 @@ static void emit_kernels(const GeomArgs &a, hipStream_t s, bool q)
     // joint kernel keeps the listed emit's registers (with 8 or 32 varyings it spills; without
     // varyings it holds 14 registers more, a wave per SIMD less)
     if (q && a.live && (NV == 3 || NV == 5)) {
-        const dim3 gc(g.x + (gq.x + 3) / 4);  // at least one queue wave per queue workgroup
+        const uint32_t nq = (gq.x + 3) / 4;  // as many queue waves as the queue kernel's
+        const dim3 gc(nq + g.x);
         constexpr int NJ = NV == 3 ? 3 : 5;  // only 3 and 5 reach here
         if (a.nscenes)
-            hipLaunchKernelGGL((k_geom_emit_clip<NJ, true>), gc, blk, 0, s, a);
+            hipLaunchKernelGGL((k_geom_emit_clip<NJ, true>), gc, blk, 0, s, a, nq);
         else
-            hipLaunchKernelGGL((k_geom_emit_clip<NJ, false>), gc, blk, 0, s, a);
+            hipLaunchKernelGGL((k_geom_emit_clip<NJ, false>), gc, blk, 0, s, a, nq);
         return;
     }
     if (q)
